@@ -1,0 +1,57 @@
+// Internal interface between the runtime (regf_runtime.hip) and the kernels
+// (regf_kernels.hip).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace abyss {
+
+// Device-resident j-set, SoA with stride `stride` (>= n).  The reference
+// packs the same seven arrays [x|y|z|vx|vy|vz|m] with stride N into one
+// buffer (src/cuda/cuda_my_acceleration.cu:399-405) plus r_nb^2 in a second.
+struct JState {
+    const double *x, *y, *z, *vx, *vy, *vz;
+    const double *m;      // FP64 mass (FP64 kernel)
+    const float *m32;     // FP32 mass (split-precision kernel)
+    const double *rnb2;   // r_nb^2 per particle
+    int n;
+};
+
+// Output of one query launch.  With S j-splits the arrays are indexed
+// [s][k] (k = target position); with S == 1 they are the final results.
+struct QueryOut {
+    double *acc;    // [S][nt][3]
+    double *adot;   // [S][nt][3]
+    int *cnt;       // [S][nt]  true neighbour count of the split's j-range
+    int *list;      // [S][nt][cap]
+};
+
+struct Plan {
+    int ipl = 1;     // targets per lane
+    int splits = 1;  // j-splits (grid.y)
+    int jspan = 0;   // j per split (multiple of the tile)
+    int grid_x = 0;  // target blocks
+    int tile = 256;  // j per LDS tile
+};
+
+constexpr int kBlock = 256;   // threads per workgroup (4 wave64)
+constexpr int kTile = 256;    // j per LDS tile
+constexpr int kFold = 64;     // j per FP32 partial before the FP64 fold
+
+// Launch the pairwise kernel for nt targets against js (all j), split into
+// plan.splits contiguous j-ranges.  prec: 0 split precision, 1 FP64.
+hipError_t launch_regf(const JState &js, const int *tgt, int nt, const Plan &plan,
+                       const QueryOut &out, int cap, int prec, hipStream_t st);
+
+// Sum the S partial results in split order and concatenate the neighbour
+// sub-lists in split (= ascending j) order into the final outputs.
+hipError_t launch_finalize(const QueryOut &part, int S, int nt, int cap,
+                           double *acc, double *adot, int *nnb, int *list,
+                           hipStream_t st);
+
+// Largest per-target count (clipped to cap) and number of overflowed
+// targets, for sizing the device->host list copy.
+hipError_t launch_count_stats(const int *nnb, int nt, int cap, int *d_stats,
+                              hipStream_t st);
+
+}  // namespace abyss

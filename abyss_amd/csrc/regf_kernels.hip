@@ -1,0 +1,350 @@
+// Regular-force + neighbour-list kernels for gfx950 (CDNA4, wave64).
+//
+// Work decomposition (one launch = one query):
+//   grid.x : blocks of kBlock*IPL targets; each lane owns IPL targets and
+//            keeps their FP64 position/velocity, r_nb^2, FP64 sums and
+//            neighbour cursor in registers.
+//   grid.y : S contiguous j-ranges ("splits", a multiple of the tile each).
+//            S > 1 exposes parallelism when few targets are active; the
+//            partial sums and per-split sub-lists are merged in split order
+//            by finalize_kernel, so results do not depend on scheduling.
+// Each workgroup streams its j-range through LDS in tiles of kTile j
+// (double-buffered: the next tile's coalesced SoA loads are in flight while
+// the current tile is evaluated; one barrier per tile).  Every lane of a
+// wave reads the same j record from LDS (broadcast), so j costs 3 b128 + 1
+// b32 LDS reads per wave and is reused by the wave's 64*IPL targets.
+//
+// Pair evaluation, split precision (the reference formula is
+// src_mpi/cuda/cuda_kernels.cu:331-354, FP64 form src/cuda/cuda_kernels.cu
+// :283-311):
+//   dx = (float)(x_j - x_i)  [FP64 subtract, one rounding]   (same for v)
+//   r2 = dx.dx (FP32), rinv = v_rsq_f32(r2)
+//   regular  <=> r2 > r_nb,i^2 ; otherwise neighbour if j != i (index)
+//   a    += m_j rinv^3 dx
+//   jdot += m_j rinv^3 (dv - 3 (dx.dv) rinv^2 dx)
+// A neighbour (or the target itself, or a coincident j) gets rinv = 0, so it
+// adds exactly zero and never produces inf/NaN.  The FP32 terms are summed
+// over kFold consecutive j and each such partial is added to an FP64 sum.
+// Neighbours are appended to the lane's own list in ascending j order; the
+// append branch is taken only for neighbours, so the self test (j != i)
+// costs nothing on the common path.
+#include "regf_internal.h"
+
+namespace abyss {
+namespace {
+
+template <bool F64>
+struct MassT { using type = float; };
+template <>
+struct MassT<true> { using type = double; };
+
+template <int IPL, bool F64>
+struct Targets {
+    double x[IPL], y[IPL], z[IPL], vx[IPL], vy[IPL], vz[IPL];
+    double acc[IPL][6];       // FP64 per-particle sums (a, jdot)
+    float rnbf[IPL];          // r_nb^2 (FP32 compare, split kernel)
+    double rnbd[IPL];         // r_nb^2 (FP64 compare, FP64 kernel)
+    int idx[IPL];             // j-index of the target (self exclusion)
+    int cnt[IPL];             // neighbours found so far in this j-range
+    int *lst[IPL];            // this (split, target)'s list row
+};
+
+// Append j to target k's list if it is not the target itself.
+template <int IPL, bool F64>
+__device__ __forceinline__ void append(Targets<IPL, F64> &T, int k, int jg, int cap)
+{
+    if (jg != T.idx[k]) {
+        if (T.cnt[k] < cap) T.lst[k][T.cnt[k]] = jg;
+        T.cnt[k] += 1;
+    }
+}
+
+// Evaluate j records [jb, je) of LDS buffer (pos, mass) against all targets.
+// jg0 is the global index of record 0 of the buffer.
+template <int IPL, bool F64, bool FULL>
+__device__ __forceinline__ void run_tile(Targets<IPL, F64> &T, const double2 *__restrict__ pos,
+                                         const typename MassT<F64>::type *__restrict__ mass,
+                                         int count, int jg0, int cap)
+{
+    if constexpr (!F64) {
+        for (int f = 0; f < (FULL ? kTile : count); f += kFold) {
+            float pa[IPL][6];
+#pragma unroll
+            for (int k = 0; k < IPL; ++k)
+#pragma unroll
+                for (int c = 0; c < 6; ++c) pa[k][c] = 0.0f;
+            const int fe = FULL ? kFold : min(kFold, count - f);
+#pragma unroll 4
+            for (int jj = 0; jj < fe; ++jj) {
+                const int r = f + jj;
+                const double2 p0 = pos[3 * r + 0];   // x, y
+                const double2 p1 = pos[3 * r + 1];   // z, vx
+                const double2 p2 = pos[3 * r + 2];   // vy, vz
+                const float mj = mass[r];
+#pragma unroll
+                for (int k = 0; k < IPL; ++k) {
+                    const float dx = (float)(p0.x - T.x[k]);
+                    const float dy = (float)(p0.y - T.y[k]);
+                    const float dz = (float)(p1.x - T.z[k]);
+                    const float dvx = (float)(p1.y - T.vx[k]);
+                    const float dvy = (float)(p2.x - T.vy[k]);
+                    const float dvz = (float)(p2.y - T.vz[k]);
+                    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                    const bool far = r2 > T.rnbf[k];
+                    float rinv = __builtin_amdgcn_rsqf(r2);
+                    rinv = far ? rinv : 0.0f;
+                    if (!far) append(T, k, jg0 + r, cap);
+                    const float rinv2 = rinv * rinv;
+                    const float mr3 = mj * rinv * rinv2;
+                    const float rv = fmaf(dz, dvz, fmaf(dy, dvy, dx * dvx));
+                    const float al = -3.0f * rv * rinv2;
+                    pa[k][0] = fmaf(mr3, dx, pa[k][0]);
+                    pa[k][1] = fmaf(mr3, dy, pa[k][1]);
+                    pa[k][2] = fmaf(mr3, dz, pa[k][2]);
+                    pa[k][3] = fmaf(mr3, fmaf(al, dx, dvx), pa[k][3]);
+                    pa[k][4] = fmaf(mr3, fmaf(al, dy, dvy), pa[k][4]);
+                    pa[k][5] = fmaf(mr3, fmaf(al, dz, dvz), pa[k][5]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < IPL; ++k)
+#pragma unroll
+                for (int c = 0; c < 6; ++c) T.acc[k][c] += (double)pa[k][c];
+        }
+    } else {
+        const int c_end = FULL ? kTile : count;
+#pragma unroll 2
+        for (int r = 0; r < c_end; ++r) {
+            const double2 p0 = pos[3 * r + 0];
+            const double2 p1 = pos[3 * r + 1];
+            const double2 p2 = pos[3 * r + 2];
+            const double mj = mass[r];
+#pragma unroll
+            for (int k = 0; k < IPL; ++k) {
+                const double dx = p0.x - T.x[k];
+                const double dy = p0.y - T.y[k];
+                const double dz = p1.x - T.z[k];
+                const double dvx = p1.y - T.vx[k];
+                const double dvy = p2.x - T.vy[k];
+                const double dvz = p2.y - T.vz[k];
+                const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+                const bool far = r2 > T.rnbd[k];
+                double rinv = far ? rsqrt(r2) : 0.0;
+                if (!far) append(T, k, jg0 + r, cap);
+                const double rinv2 = rinv * rinv;
+                const double mr3 = mj * rinv * rinv2;
+                const double rv = fma(dz, dvz, fma(dy, dvy, dx * dvx));
+                const double al = -3.0 * rv * rinv2;
+                T.acc[k][0] = fma(mr3, dx, T.acc[k][0]);
+                T.acc[k][1] = fma(mr3, dy, T.acc[k][1]);
+                T.acc[k][2] = fma(mr3, dz, T.acc[k][2]);
+                T.acc[k][3] = fma(mr3, fma(al, dx, dvx), T.acc[k][3]);
+                T.acc[k][4] = fma(mr3, fma(al, dy, dvy), T.acc[k][4]);
+                T.acc[k][5] = fma(mr3, fma(al, dz, dvz), T.acc[k][5]);
+            }
+        }
+    }
+}
+
+template <int IPL, bool F64>
+__global__ __launch_bounds__(kBlock) void regf_kernel(JState js, const int *__restrict__ tgt,
+                                                      int nt, int jspan, QueryOut out, int cap,
+                                                      int final_counts)
+{
+    using MT = typename MassT<F64>::type;
+    __shared__ double2 s_pos[2][kTile * 3];
+    __shared__ MT s_m[2][kTile];
+
+    const int tid = threadIdx.x;
+    const int s = blockIdx.y;
+    const int j0 = s * jspan;
+    const int j1 = min(js.n, j0 + jspan);
+
+    Targets<IPL, F64> T;
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        const int q = blockIdx.x * (kBlock * IPL) + k * kBlock + tid;
+        if (q < nt) {
+            const int i = tgt[q];
+            T.idx[k] = i;
+            T.x[k] = js.x[i]; T.y[k] = js.y[i]; T.z[k] = js.z[i];
+            T.vx[k] = js.vx[i]; T.vy[k] = js.vy[i]; T.vz[k] = js.vz[i];
+            T.rnbd[k] = js.rnb2[i];
+            T.lst[k] = out.list + ((size_t)s * nt + q) * (size_t)cap;
+        } else {
+            // Inactive lane: r_nb^2 = -1 makes every j regular, so it never
+            // appends; its sums are discarded.
+            T.idx[k] = -1;
+            T.x[k] = T.y[k] = T.z[k] = 0.0;
+            T.vx[k] = T.vy[k] = T.vz[k] = 0.0;
+            T.rnbd[k] = -1.0;
+            T.lst[k] = nullptr;
+        }
+        T.rnbf[k] = (float)T.rnbd[k];
+        T.cnt[k] = 0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) T.acc[k][c] = 0.0;
+    }
+
+    const int ntile = (j1 - j0 + kTile - 1) / kTile;
+    double rx = 0, ry = 0, rz = 0, rvx = 0, rvy = 0, rvz = 0;
+    MT rm = 0;
+    auto gload = [&](int t) {
+        const int j = j0 + t * kTile + tid;
+        if (j < j1) {
+            rx = js.x[j]; ry = js.y[j]; rz = js.z[j];
+            rvx = js.vx[j]; rvy = js.vy[j]; rvz = js.vz[j];
+            if constexpr (F64) rm = js.m[j]; else rm = js.m32[j];
+        }
+    };
+    auto lstore = [&](int b) {
+        s_pos[b][3 * tid + 0] = make_double2(rx, ry);
+        s_pos[b][3 * tid + 1] = make_double2(rz, rvx);
+        s_pos[b][3 * tid + 2] = make_double2(rvy, rvz);
+        s_m[b][tid] = rm;
+    };
+    static_assert(kTile == kBlock, "one j per thread per tile load");
+
+    if (ntile > 0) {
+        gload(0);
+        lstore(0);
+    }
+    __syncthreads();
+    for (int t = 0; t < ntile; ++t) {
+        const int b = t & 1;
+        if (t + 1 < ntile) gload(t + 1);
+        const int jt = j0 + t * kTile;
+        const int count = min(kTile, j1 - jt);
+        if (count == kTile)
+            run_tile<IPL, F64, true>(T, s_pos[b], s_m[b], count, jt, cap);
+        else
+            run_tile<IPL, F64, false>(T, s_pos[b], s_m[b], count, jt, cap);
+        if (t + 1 < ntile) lstore(b ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        const int q = blockIdx.x * (kBlock * IPL) + k * kBlock + tid;
+        if (q < nt) {
+            const size_t o = (size_t)s * nt + q;
+            out.acc[3 * o + 0] = T.acc[k][0];
+            out.acc[3 * o + 1] = T.acc[k][1];
+            out.acc[3 * o + 2] = T.acc[k][2];
+            out.adot[3 * o + 0] = T.acc[k][3];
+            out.adot[3 * o + 1] = T.acc[k][4];
+            out.adot[3 * o + 2] = T.acc[k][5];
+            const int c = T.cnt[k];
+            out.cnt[o] = (final_counts && c > cap) ? -c : c;
+        }
+    }
+}
+
+// One wave per target: sums the S partials in split order (fixed order, so
+// bitwise reproducible) and concatenates the sub-lists in split order, which
+// is ascending j because splits are contiguous ascending j-ranges.
+__global__ __launch_bounds__(256) void finalize_kernel(QueryOut part, int S, int nt, int cap,
+                                                       double *__restrict__ acc,
+                                                       double *__restrict__ adot,
+                                                       int *__restrict__ nnb,
+                                                       int *__restrict__ list)
+{
+    const int lane = threadIdx.x & 63;
+    const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= nt) return;
+    if (lane < 6) {
+        const double *src = lane < 3 ? part.acc : part.adot;
+        const int c = lane < 3 ? lane : lane - 3;
+        double sum = 0.0;
+        for (int s = 0; s < S; ++s) sum += src[((size_t)s * nt + q) * 3 + c];
+        (lane < 3 ? acc : adot)[(size_t)q * 3 + c] = sum;
+    }
+    int off = 0;
+    int *dst = list + (size_t)q * cap;
+    for (int s = 0; s < S; ++s) {
+        const size_t o = (size_t)s * nt + q;
+        const int c = part.cnt[o];
+        const int take = min(c, cap - off);
+        const int *src = part.list + o * (size_t)cap;
+        for (int e = lane; e < take; e += 64) dst[off + e] = src[e];
+        off += c;
+    }
+    if (lane == 0) nnb[q] = off <= cap ? off : -off;
+}
+
+// Single workgroup: d_stats[0] = max over targets of min(|nnb|, cap),
+// d_stats[1] = number of overflowed targets (nnb < 0).
+__global__ __launch_bounds__(1024) void count_stats_kernel(const int *__restrict__ nnb, int nt,
+                                                           int cap, int *__restrict__ d_stats)
+{
+    __shared__ int s_max[1024];
+    __shared__ int s_ovf[1024];
+    int mx = 0, ov = 0;
+    for (int q = threadIdx.x; q < nt; q += blockDim.x) {
+        const int c = nnb[q];
+        const int a = c < 0 ? -c : c;
+        mx = max(mx, min(a, cap));
+        ov += c < 0 ? 1 : 0;
+    }
+    s_max[threadIdx.x] = mx;
+    s_ovf[threadIdx.x] = ov;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s_max[threadIdx.x] = max(s_max[threadIdx.x], s_max[threadIdx.x + w]);
+            s_ovf[threadIdx.x] += s_ovf[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        d_stats[0] = s_max[0];
+        d_stats[1] = s_ovf[0];
+    }
+}
+
+template <int IPL, bool F64>
+hipError_t launch_t(const JState &js, const int *tgt, int nt, const Plan &p, const QueryOut &out,
+                    int cap, hipStream_t st)
+{
+    dim3 grid(p.grid_x, p.splits);
+    hipLaunchKernelGGL((regf_kernel<IPL, F64>), grid, dim3(kBlock), 0, st, js, tgt, nt, p.jspan,
+                       out, cap, p.splits == 1 ? 1 : 0);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_regf(const JState &js, const int *tgt, int nt, const Plan &p,
+                       const QueryOut &out, int cap, int prec, hipStream_t st)
+{
+    if (prec == 1) {
+        switch (p.ipl) {
+            case 1: return launch_t<1, true>(js, tgt, nt, p, out, cap, st);
+            case 2: return launch_t<2, true>(js, tgt, nt, p, out, cap, st);
+            default: return hipErrorInvalidValue;
+        }
+    }
+    switch (p.ipl) {
+        case 1: return launch_t<1, false>(js, tgt, nt, p, out, cap, st);
+        case 2: return launch_t<2, false>(js, tgt, nt, p, out, cap, st);
+        case 4: return launch_t<4, false>(js, tgt, nt, p, out, cap, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_finalize(const QueryOut &part, int S, int nt, int cap, double *acc,
+                           double *adot, int *nnb, int *list, hipStream_t st)
+{
+    dim3 grid((nt + 3) / 4);
+    hipLaunchKernelGGL(finalize_kernel, grid, dim3(256), 0, st, part, S, nt, cap, acc, adot, nnb,
+                       list);
+    return hipGetLastError();
+}
+
+hipError_t launch_count_stats(const int *nnb, int nt, int cap, int *d_stats, hipStream_t st)
+{
+    hipLaunchKernelGGL(count_stats_kernel, dim3(1), dim3(1024), 0, st, nnb, nt, cap, d_stats);
+    return hipGetLastError();
+}
+
+}  // namespace abyss

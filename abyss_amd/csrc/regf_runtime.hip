@@ -1,0 +1,703 @@
+// Runtime behind the C ABI (include/abyss_regf.h): device contexts, the
+// device-resident j-set, pinned staging, launch planning, result transfer,
+// input validation and the six legacy entry points.
+//
+// Reference counterpart: src_mpi/cuda/cuda_my_acceleration.cu (file-static
+// state :23-88, _ReceiveFromHost :603-778, GetAcceleration :95-344, open /
+// close / profile :987-1062, extern "C" wrappers :1087-1106).  Differences by
+// design: validation happens before any HIP call and is reported through
+// status codes; device buffers are sized from the actual N and targets
+// (grow-only) instead of next-pow2 staging of GridDimY*NNB_per_block*N ints;
+// only the populated columns of the neighbour lists cross PCIe.
+#include "../../include/abyss_regf.h"
+#include "regf_internal.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+
+using abyss::JState;
+using abyss::Plan;
+using abyss::QueryOut;
+
+struct abyss_ctx_s {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    int cap = ABYSS_NUM_NEIGHBOR_MAX;
+    int precision = ABYSS_PREC_SPLIT;
+    int force_ipl = 0, force_splits = 0;
+    unsigned long long scratch_limit = 1ull << 30;
+
+    // j-set: device SoA [x|y|z|vx|vy|vz|m|rnb2] (stride n) + FP32 mass
+    int n = 0;
+    size_t j_bytes = 0;
+    char *d_j = nullptr;
+    char *h_j = nullptr;  // pinned staging, same layout
+    JState js{};
+
+    // targets and outputs
+    int nt = 0;
+    size_t t_capacity = 0;  // targets the output buffers can hold
+    int *d_tgt = nullptr, *h_tgt = nullptr;
+    double *d_acc = nullptr, *d_adot = nullptr;
+    int *d_nnb = nullptr, *d_list = nullptr;
+    int *d_stats = nullptr;
+    double *h_res = nullptr;  // pinned [nt][6] (acc then adot)
+    int *h_nnb = nullptr;
+    int *h_list = nullptr;    // pinned, rows of width <= cap
+    size_t h_list_bytes = 0;
+    int *h_stats = nullptr;
+
+    // split-j scratch
+    size_t scratch_bytes = 0;
+    char *d_scratch = nullptr;
+    QueryOut part{};
+
+    Plan plan{};
+    bool plan_valid = false;
+
+    // timing / counters (ProfileDevice)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timing_pending = false;
+    long long n_send = 0, n_grav = 0, sum_ni = 0;
+    double interactions = 0.0, t_grav_ms = 0.0;
+
+    char detail[512] = {0};
+};
+
+namespace {
+
+int set_detail(abyss_ctx_t c, int code, const char *fmt, ...)
+{
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->detail, sizeof(c->detail), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                            \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            return set_detail(ctx,                                                   \
+                              e_ == hipErrorOutOfMemory ? ABYSS_ERR_OUT_OF_MEMORY    \
+                                                        : ABYSS_ERR_HIP,             \
+                              "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                              __FILE__, __LINE__);                                   \
+        }                                                                            \
+    } while (0)
+
+template <class T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+template <class T>
+void hfree(T *&p)
+{
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+}
+
+size_t grow(size_t need) { return need + need / 2 + 4096; }
+
+// Bytes of the j-set layout for n particles.
+size_t jset_bytes(int n) { return (size_t)n * (8 * sizeof(double) + sizeof(float)); }
+
+void bind_jstate(abyss_ctx_t c, char *base, int n)
+{
+    double *d = reinterpret_cast<double *>(base);
+    c->js.x = d + 0 * (size_t)n;
+    c->js.y = d + 1 * (size_t)n;
+    c->js.z = d + 2 * (size_t)n;
+    c->js.vx = d + 3 * (size_t)n;
+    c->js.vy = d + 4 * (size_t)n;
+    c->js.vz = d + 5 * (size_t)n;
+    c->js.m = d + 6 * (size_t)n;
+    c->js.rnb2 = d + 7 * (size_t)n;
+    c->js.m32 = reinterpret_cast<const float *>(d + 8 * (size_t)n);
+    c->js.n = n;
+}
+
+int ensure_outputs(abyss_ctx_t c, int nt)
+{
+    if ((size_t)nt <= c->t_capacity) return ABYSS_OK;
+    size_t cap_t = grow((size_t)nt);
+    dfree(c->d_tgt); dfree(c->d_acc); dfree(c->d_adot); dfree(c->d_nnb); dfree(c->d_list);
+    hfree(c->h_tgt); hfree(c->h_res); hfree(c->h_nnb);
+    c->t_capacity = 0;
+    HIPCHK(c, hipMalloc(&c->d_tgt, cap_t * sizeof(int)));
+    HIPCHK(c, hipMalloc(&c->d_acc, cap_t * 3 * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->d_adot, cap_t * 3 * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->d_nnb, cap_t * sizeof(int)));
+    HIPCHK(c, hipMalloc(&c->d_list, cap_t * (size_t)c->cap * sizeof(int)));
+    HIPCHK(c, hipHostMalloc(&c->h_tgt, cap_t * sizeof(int), hipHostMallocDefault));
+    HIPCHK(c, hipHostMalloc(&c->h_res, cap_t * 6 * sizeof(double), hipHostMallocDefault));
+    HIPCHK(c, hipHostMalloc(&c->h_nnb, cap_t * sizeof(int), hipHostMallocDefault));
+    c->t_capacity = cap_t;
+    return ABYSS_OK;
+}
+
+Plan make_plan(abyss_ctx_t c, int nt, int n)
+{
+    Plan p;
+    p.tile = abyss::kTile;
+    const int cu = c->cu_count > 0 ? c->cu_count : 256;
+    int ipl = c->force_ipl;
+    if (ipl == 0) {
+        // Two targets per lane halves the LDS traffic per pair once there
+        // are enough targets to give every CU two workgroups at ipl = 2.
+        ipl = (nt >= abyss::kBlock * 2 * cu * 2) ? 2 : 1;
+    }
+    if (c->precision == ABYSS_PREC_FP64 && ipl > 2) ipl = 2;
+    p.ipl = ipl;
+    p.grid_x = (nt + abyss::kBlock * ipl - 1) / (abyss::kBlock * ipl);
+    int S = c->force_splits;
+    if (S == 0) {
+        const int want_wg = cu * 8;
+        S = (want_wg + p.grid_x - 1) / p.grid_x;
+        const int max_by_n = std::max(1, n / (4 * abyss::kTile));
+        S = std::min(S, max_by_n);
+    }
+    if (S > 1) {
+        const double per = (double)nt * ((double)c->cap * 4.0 + 6 * 8.0 + 4.0);
+        const int max_by_mem = (int)std::max(1.0, std::floor((double)c->scratch_limit / per));
+        S = std::min(S, max_by_mem);
+    }
+    S = std::max(1, std::min(S, n));
+    int jspan = (n + S - 1) / S;
+    jspan = ((jspan + p.tile - 1) / p.tile) * p.tile;
+    p.jspan = jspan;
+    p.splits = (n + jspan - 1) / jspan;
+    return p;
+}
+
+int ensure_scratch(abyss_ctx_t c, const Plan &p, int nt)
+{
+    if (p.splits <= 1) return ABYSS_OK;
+    const size_t S = (size_t)p.splits;
+    const size_t b_acc = S * nt * 3 * sizeof(double);
+    const size_t b_cnt = S * nt * sizeof(int);
+    const size_t b_list = S * nt * (size_t)c->cap * sizeof(int);
+    const size_t need = 2 * b_acc + b_cnt + b_list + 4 * 256;
+    if (need > c->scratch_bytes) {
+        // a previous launch may still read the old scratch
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        dfree(c->d_scratch);
+        c->scratch_bytes = 0;
+        HIPCHK(c, hipMalloc(&c->d_scratch, need));
+        c->scratch_bytes = need;
+    }
+    char *b = c->d_scratch;
+    auto take = [&](size_t bytes) {
+        char *r = b;
+        b += (bytes + 255) & ~size_t(255);
+        return r;
+    };
+    c->part.acc = reinterpret_cast<double *>(take(b_acc));
+    c->part.adot = reinterpret_cast<double *>(take(b_acc));
+    c->part.cnt = reinterpret_cast<int *>(take(b_cnt));
+    c->part.list = reinterpret_cast<int *>(take(b_list));
+    return ABYSS_OK;
+}
+
+int collect_timing(abyss_ctx_t c)
+{
+    if (!c->timing_pending) return ABYSS_OK;
+    float ms = 0.0f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->t_grav_ms += ms;
+    c->timing_pending = false;
+    return ABYSS_OK;
+}
+
+void release_all(abyss_ctx_t c)
+{
+    if (c->device >= 0) (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dfree(c->d_j); hfree(c->h_j);
+    dfree(c->d_tgt); dfree(c->d_acc); dfree(c->d_adot); dfree(c->d_nnb); dfree(c->d_list);
+    dfree(c->d_stats); dfree(c->d_scratch);
+    hfree(c->h_tgt); hfree(c->h_res); hfree(c->h_nnb); hfree(c->h_list); hfree(c->h_stats);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    c->ev0 = c->ev1 = nullptr;
+    c->stream = nullptr;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context API
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int abyss_device_count(void)
+{
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+const char *abyss_error_string(int code)
+{
+    switch (code) {
+        case ABYSS_OK: return "ok";
+        case ABYSS_ERR_NOT_OPEN: return "device not open";
+        case ABYSS_ERR_NO_DEVICE: return "no HIP device";
+        case ABYSS_ERR_INVALID_ARG: return "invalid argument";
+        case ABYSS_ERR_TARGET_RANGE: return "target index out of range";
+        case ABYSS_ERR_INVALID_INPUT: return "invalid particle data";
+        case ABYSS_ERR_NO_JSET: return "no j-particles uploaded";
+        case ABYSS_ERR_HIP: return "HIP runtime error";
+        case ABYSS_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case ABYSS_ERR_NEIGHBOR_OVERFLOW: return "neighbour list overflow";
+        case ABYSS_ERR_ADMISSION: return "admission refused (memory budget)";
+        case ABYSS_ERR_BUSY: return "no free slot";
+        default: return "unknown error";
+    }
+}
+
+int abyss_ctx_create(int device, int cap, abyss_ctx_t *out)
+{
+    if (!out || cap <= 0) return ABYSS_ERR_INVALID_ARG;
+    *out = nullptr;
+    const int ndev = abyss_device_count();
+    if (ndev <= 0) return ABYSS_ERR_NO_DEVICE;
+    if (device < 0 || device >= ndev) return ABYSS_ERR_INVALID_ARG;
+    abyss_ctx_t c = new (std::nothrow) abyss_ctx_s();
+    if (!c) return ABYSS_ERR_OUT_OF_MEMORY;
+    c->device = device;
+    c->cap = cap;
+    int rc = ABYSS_OK;
+    do {
+        if (hipSetDevice(device) != hipSuccess) { rc = ABYSS_ERR_NO_DEVICE; break; }
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+            rc = ABYSS_ERR_HIP; break;
+        }
+        if (hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, device) !=
+            hipSuccess) { rc = ABYSS_ERR_HIP; break; }
+        if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+            rc = ABYSS_ERR_HIP; break;
+        }
+        if (hipMalloc(&c->d_stats, 4 * sizeof(int)) != hipSuccess ||
+            hipHostMalloc(&c->h_stats, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+            rc = ABYSS_ERR_OUT_OF_MEMORY; break;
+        }
+    } while (0);
+    if (rc != ABYSS_OK) {
+        release_all(c);
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return ABYSS_OK;
+}
+
+int abyss_ctx_destroy(abyss_ctx_t c)
+{
+    if (!c) return ABYSS_ERR_INVALID_ARG;
+    release_all(c);
+    delete c;
+    return ABYSS_OK;
+}
+
+int abyss_ctx_set_precision(abyss_ctx_t c, int mode)
+{
+    if (!c || (mode != ABYSS_PREC_SPLIT && mode != ABYSS_PREC_FP64)) return ABYSS_ERR_INVALID_ARG;
+    c->precision = mode;
+    c->plan_valid = false;
+    return ABYSS_OK;
+}
+
+int abyss_ctx_set_tuning(abyss_ctx_t c, int ipl, int splits)
+{
+    if (!c) return ABYSS_ERR_INVALID_ARG;
+    if (ipl != 0 && ipl != 1 && ipl != 2 && ipl != 4)
+        return set_detail(c, ABYSS_ERR_INVALID_ARG, "i_per_lane must be 0, 1, 2 or 4");
+    if (splits < 0) return set_detail(c, ABYSS_ERR_INVALID_ARG, "j_splits must be >= 0");
+    c->force_ipl = ipl;
+    c->force_splits = splits;
+    c->plan_valid = false;
+    return ABYSS_OK;
+}
+
+int abyss_ctx_set_scratch_limit(abyss_ctx_t c, unsigned long long bytes)
+{
+    if (!c || bytes == 0) return ABYSS_ERR_INVALID_ARG;
+    c->scratch_limit = bytes;
+    c->plan_valid = false;
+    return ABYSS_OK;
+}
+
+const char *abyss_ctx_error_detail(abyss_ctx_t c) { return c ? c->detail : ""; }
+
+void *abyss_ctx_stream(abyss_ctx_t c) { return c ? (void *)c->stream : nullptr; }
+
+int abyss_ctx_upload(abyss_ctx_t c, int n, const double *m, const double *x, const double *v,
+                     const double *r2)
+{
+    if (!c) return ABYSS_ERR_INVALID_ARG;
+    if (n <= 0 || !m || !x || !v || !r2)
+        return set_detail(c, ABYSS_ERR_INVALID_ARG, "upload: n=%d or null array", n);
+    // Validate everything before touching the device.
+    for (int j = 0; j < n; ++j) {
+        for (int d = 0; d < 3; ++d) {
+            if (!isfinite(x[3 * j + d]) || !isfinite(v[3 * j + d]))
+                return set_detail(c, ABYSS_ERR_INVALID_INPUT,
+                                  "upload: non-finite position/velocity at j=%d", j);
+        }
+        if (!isfinite(m[j]) || m[j] < 0.0)
+            return set_detail(c, ABYSS_ERR_INVALID_INPUT, "upload: bad mass %g at j=%d", m[j], j);
+        if (isnan(r2[j]) || r2[j] < 0.0)
+            return set_detail(c, ABYSS_ERR_INVALID_INPUT, "upload: bad r_nb^2 %g at j=%d", r2[j],
+                              j);
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    // The previous upload / query may still read the staging buffer.
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const size_t need = jset_bytes(n);
+    if (need > c->j_bytes) {
+        dfree(c->d_j);
+        hfree(c->h_j);
+        c->j_bytes = 0;
+        c->n = 0;
+        const size_t cap_b = grow(need);
+        HIPCHK(c, hipMalloc(&c->d_j, cap_b));
+        HIPCHK(c, hipHostMalloc(&c->h_j, cap_b, hipHostMallocDefault));
+        c->j_bytes = cap_b;
+    }
+    // AoS -> SoA pack (src/cuda/cuda_my_acceleration.cu:399-405 packs the
+    // same arrays with stride N).
+    double *h = reinterpret_cast<double *>(c->h_j);
+    float *h32 = reinterpret_cast<float *>(h + 8 * (size_t)n);
+    for (int j = 0; j < n; ++j) {
+        h[0 * (size_t)n + j] = x[3 * j + 0];
+        h[1 * (size_t)n + j] = x[3 * j + 1];
+        h[2 * (size_t)n + j] = x[3 * j + 2];
+        h[3 * (size_t)n + j] = v[3 * j + 0];
+        h[4 * (size_t)n + j] = v[3 * j + 1];
+        h[5 * (size_t)n + j] = v[3 * j + 2];
+        h[6 * (size_t)n + j] = m[j];
+        h[7 * (size_t)n + j] = r2[j];
+        h32[j] = (float)m[j];
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_j, c->h_j, need, hipMemcpyHostToDevice, c->stream));
+    bind_jstate(c, c->d_j, n);
+    if (c->n != n) {
+        // targets set for another j-set may be out of range for this one
+        c->plan_valid = false;
+        c->nt = 0;
+    }
+    c->n = n;
+    c->n_send += 1;
+    return ABYSS_OK;
+}
+
+int abyss_ctx_set_targets(abyss_ctx_t c, int nt, const int *targets)
+{
+    if (!c) return ABYSS_ERR_INVALID_ARG;
+    if (c->n <= 0) return set_detail(c, ABYSS_ERR_NO_JSET, "no j-set uploaded");
+    if (!targets || nt <= 0 || nt > c->n)
+        return set_detail(c, ABYSS_ERR_INVALID_ARG, "targets: need 0 < NumTarget=%d <= N=%d", nt,
+                          c->n);
+    for (int k = 0; k < nt; ++k)
+        if (targets[k] < 0 || targets[k] >= c->n)
+            return set_detail(c, ABYSS_ERR_TARGET_RANGE, "target[%d]=%d outside [0,%d)", k,
+                              targets[k], c->n);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc = ensure_outputs(c, nt);
+    if (rc != ABYSS_OK) return rc;
+    memcpy(c->h_tgt, targets, (size_t)nt * sizeof(int));
+    HIPCHK(c, hipMemcpyAsync(c->d_tgt, c->h_tgt, (size_t)nt * sizeof(int), hipMemcpyHostToDevice,
+                             c->stream));
+    c->nt = nt;
+    c->plan = make_plan(c, nt, c->n);
+    rc = ensure_scratch(c, c->plan, nt);
+    if (rc != ABYSS_OK) return rc;
+    c->plan_valid = true;
+    return ABYSS_OK;
+}
+
+int abyss_ctx_plan(abyss_ctx_t c, int *ipl, int *splits, int *grid_x, int *tile)
+{
+    if (!c) return ABYSS_ERR_INVALID_ARG;
+    if (!c->plan_valid) return set_detail(c, ABYSS_ERR_INVALID_ARG, "no targets set");
+    if (ipl) *ipl = c->plan.ipl;
+    if (splits) *splits = c->plan.splits;
+    if (grid_x) *grid_x = c->plan.grid_x;
+    if (tile) *tile = c->plan.tile;
+    return ABYSS_OK;
+}
+
+int abyss_ctx_launch(abyss_ctx_t c)
+{
+    if (!c) return ABYSS_ERR_INVALID_ARG;
+    if (c->n <= 0) return set_detail(c, ABYSS_ERR_NO_JSET, "no j-set uploaded");
+    if (c->nt <= 0) return set_detail(c, ABYSS_ERR_INVALID_ARG, "no targets set");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->plan_valid) {
+        c->plan = make_plan(c, c->nt, c->n);
+        int rc = ensure_scratch(c, c->plan, c->nt);
+        if (rc != ABYSS_OK) return rc;
+        c->plan_valid = true;
+    }
+    const Plan &p = c->plan;
+    QueryOut fin{c->d_acc, c->d_adot, c->d_nnb, c->d_list};
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (p.splits == 1) {
+        HIPCHK(c, abyss::launch_regf(c->js, c->d_tgt, c->nt, p, fin, c->cap, c->precision,
+                                     c->stream));
+    } else {
+        HIPCHK(c, abyss::launch_regf(c->js, c->d_tgt, c->nt, p, c->part, c->cap, c->precision,
+                                     c->stream));
+        HIPCHK(c, abyss::launch_finalize(c->part, p.splits, c->nt, c->cap, c->d_acc, c->d_adot,
+                                         c->d_nnb, c->d_list, c->stream));
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->timing_pending = true;
+    c->n_grav += 1;
+    c->sum_ni += c->nt;
+    c->interactions += (double)c->nt * (double)(c->n - 1);
+    return ABYSS_OK;
+}
+
+int abyss_ctx_sync(abyss_ctx_t c)
+{
+    if (!c) return ABYSS_ERR_INVALID_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return collect_timing(c);
+}
+
+int abyss_ctx_fetch(abyss_ctx_t c, double *acc, double *adot, int *nnb, int *list)
+{
+    if (!c) return ABYSS_ERR_INVALID_ARG;
+    if (c->nt <= 0) return set_detail(c, ABYSS_ERR_INVALID_ARG, "no targets set");
+    if (!acc || !adot || !nnb || !list)
+        return set_detail(c, ABYSS_ERR_INVALID_ARG, "fetch: null output array");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nt = c->nt;
+    const int cap = c->cap;
+    HIPCHK(c, abyss::launch_count_stats(c->d_nnb, nt, cap, c->d_stats, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_stats, c->d_stats, 2 * sizeof(int), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_res, c->d_acc, (size_t)nt * 3 * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_res + (size_t)nt * 3, c->d_adot, (size_t)nt * 3 * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_nnb, c->d_nnb, (size_t)nt * sizeof(int), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc = collect_timing(c);
+    if (rc != ABYSS_OK) return rc;
+    const int width = c->h_stats[0];  // widest populated row, <= cap
+    const int overflowed = c->h_stats[1];
+    if (width > 0) {
+        const size_t need = (size_t)nt * width * sizeof(int);
+        if (need > c->h_list_bytes) {
+            hfree(c->h_list);
+            c->h_list_bytes = 0;
+            HIPCHK(c, hipHostMalloc(&c->h_list, grow(need), hipHostMallocDefault));
+            c->h_list_bytes = grow(need);
+        }
+        HIPCHK(c, hipMemcpy2DAsync(c->h_list, (size_t)width * sizeof(int), c->d_list,
+                                   (size_t)cap * sizeof(int), (size_t)width * sizeof(int), nt,
+                                   hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    memcpy(acc, c->h_res, (size_t)nt * 3 * sizeof(double));
+    memcpy(adot, c->h_res + (size_t)nt * 3, (size_t)nt * 3 * sizeof(double));
+    memcpy(nnb, c->h_nnb, (size_t)nt * sizeof(int));
+    for (int k = 0; k < nt; ++k) {
+        const int cnt = c->h_nnb[k];
+        const int ncopy = std::min(cnt < 0 ? -cnt : cnt, cap);
+        if (ncopy > 0)
+            memcpy(list + (size_t)k * cap, c->h_list + (size_t)k * width,
+                   (size_t)ncopy * sizeof(int));
+    }
+    if (overflowed > 0)
+        return set_detail(c, ABYSS_ERR_NEIGHBOR_OVERFLOW,
+                          "%d target(s) exceeded the neighbour cap %d", overflowed, cap);
+    return ABYSS_OK;
+}
+
+int abyss_ctx_query(abyss_ctx_t c, int nt, const int *targets, double *acc, double *adot,
+                    int *nnb, int *list)
+{
+    if (!c) return ABYSS_ERR_INVALID_ARG;
+    if (!acc || !adot || !nnb || !list)
+        return set_detail(c, ABYSS_ERR_INVALID_ARG, "query: null output array");
+    int rc = abyss_ctx_set_targets(c, nt, targets);
+    if (rc != ABYSS_OK) return rc;
+    rc = abyss_ctx_launch(c);
+    if (rc != ABYSS_OK) return rc;
+    return abyss_ctx_fetch(c, acc, adot, nnb, list);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Legacy entry points: one process-wide default context.
+// ---------------------------------------------------------------------------
+namespace {
+abyss_ctx_t g_ctx = nullptr;
+bool g_open = false;
+int g_cap = ABYSS_NUM_NEIGHBOR_MAX;
+int g_prec = ABYSS_PREC_SPLIT;
+int g_last = ABYSS_OK;
+char g_detail[512] = {0};
+char g_init_detail[256] = {0};
+
+void legacy_status(int rc)
+{
+    g_last = rc;
+    if (rc == ABYSS_OK) {
+        g_detail[0] = 0;
+        return;
+    }
+    const char *d = g_ctx ? g_ctx->detail : g_init_detail;
+    snprintf(g_detail, sizeof(g_detail), "%s%s%s", abyss_error_string(rc), d[0] ? ": " : "", d);
+    fprintf(stderr, "abyss_regf: %s\n", g_detail);
+}
+
+int legacy_init(int irank)
+{
+    if (g_ctx) return ABYSS_OK;
+    const int ndev = abyss_device_count();
+    if (ndev <= 0) {
+        snprintf(g_init_detail, sizeof(g_init_detail), "no HIP device visible");
+        return ABYSS_ERR_NO_DEVICE;
+    }
+    const int dev = ((irank % ndev) + ndev) % ndev;
+    int rc = abyss_ctx_create(dev, g_cap, &g_ctx);
+    if (rc != ABYSS_OK) {
+        snprintf(g_init_detail, sizeof(g_init_detail), "context creation on device %d failed",
+                 dev);
+        g_ctx = nullptr;
+        return rc;
+    }
+    g_init_detail[0] = 0;
+    abyss_ctx_set_precision(g_ctx, g_prec);
+    return ABYSS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int abyss_last_error(void) { return g_last; }
+const char *abyss_last_error_detail(void) { return g_detail; }
+int abyss_num_neighbor_max(void) { return g_cap; }
+
+int abyss_set_num_neighbor_max(int cap)
+{
+    if (cap <= 0) return ABYSS_ERR_INVALID_ARG;
+    if (g_ctx) return ABYSS_ERR_INVALID_ARG;  // only while closed
+    g_cap = cap;
+    return ABYSS_OK;
+}
+
+int abyss_set_precision(int mode)
+{
+    if (mode != ABYSS_PREC_SPLIT && mode != ABYSS_PREC_FP64) return ABYSS_ERR_INVALID_ARG;
+    g_prec = mode;
+    if (g_ctx) abyss_ctx_set_precision(g_ctx, mode);
+    return ABYSS_OK;
+}
+
+void InitializeDevice(int *irank)
+{
+    const int r = irank ? *irank : 0;
+    int rc = legacy_init(r);
+    if (rc == ABYSS_OK) g_open = true;  // src/cuda/cuda_my_acceleration.cu:454
+    legacy_status(rc);
+}
+
+void OpenDevice(const int *irank)
+{
+    if (g_open) {
+        fprintf(stderr, "abyss_regf: it is already open\n");
+        legacy_status(ABYSS_OK);
+        return;
+    }
+    int rc = legacy_init(irank ? *irank : 0);
+    if (rc == ABYSS_OK) g_open = true;
+    legacy_status(rc);
+}
+
+void CloseDevice(void)
+{
+    if (!g_open) {
+        fprintf(stderr, "abyss_regf: it is already closed\n");
+        legacy_status(ABYSS_OK);
+        return;
+    }
+    if (g_ctx) abyss_ctx_destroy(g_ctx);
+    g_ctx = nullptr;
+    g_open = false;
+    legacy_status(ABYSS_OK);
+}
+
+void ProfileDevice(int *irank)
+{
+    if (!g_ctx) {
+        legacy_status(ABYSS_ERR_NOT_OPEN);
+        return;
+    }
+    abyss_ctx_t c = g_ctx;
+    if (c->n_grav) {
+        const double gflops = c->t_grav_ms > 0 ? 60e-9 * c->interactions / (c->t_grav_ms * 1e-3)
+                                               : 0.0;
+        fprintf(stderr,
+                "[R.%d-D.%d GPU Reg.F ] Nsend %lld  Ngrav %lld  <Ni> %lld  grav(s) %f  "
+                "interactions %.6e  Perf.(Gflops) %f\n",
+                irank ? *irank : 0, c->device, c->n_send, c->n_grav,
+                c->n_send ? c->sum_ni / std::max(1LL, c->n_send) : 0LL, c->t_grav_ms * 1e-3,
+                c->interactions, gflops);
+    }
+    c->n_send = c->n_grav = c->sum_ni = 0;
+    c->interactions = 0.0;
+    c->t_grav_ms = 0.0;
+    legacy_status(ABYSS_OK);
+}
+
+void SendToDevice(int *NNB, double m[], double x[][3], double v[][3], double r2[], double mdot[])
+{
+    (void)mdot;  // ignored, as in the reference
+    if (!g_open || !g_ctx) {
+        legacy_status(ABYSS_ERR_NOT_OPEN);
+        return;
+    }
+    if (!NNB) {
+        legacy_status(set_detail(g_ctx, ABYSS_ERR_INVALID_ARG, "NNB is null"));
+        return;
+    }
+    legacy_status(abyss_ctx_upload(g_ctx, *NNB, m, &x[0][0], &v[0][0], r2));
+}
+
+void CalculateAccelerationOnDevice(int *NumTarget, int *h_target_list, double acc[][3],
+                                   double adot[][3], int NumNeighbor[], int *NeighborList)
+{
+    if (!g_open || !g_ctx) {
+        legacy_status(ABYSS_ERR_NOT_OPEN);
+        return;
+    }
+    if (!NumTarget) {
+        legacy_status(set_detail(g_ctx, ABYSS_ERR_INVALID_ARG, "NumTarget is null"));
+        return;
+    }
+    legacy_status(abyss_ctx_query(g_ctx, *NumTarget, h_target_list, &acc[0][0], &adot[0][0],
+                                  NumNeighbor, NeighborList));
+}
+
+}  // extern "C"

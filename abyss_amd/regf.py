@@ -1,0 +1,311 @@
+"""Python mirror of the regular-force C ABI (include/abyss_regf.h).
+
+The six legacy functions keep the reference's names and argument meaning
+(src_mpi/cuda/cuda_functions.h:3-10): SendToDevice replaces the j-set,
+CalculateAccelerationOnDevice returns the regular acc/adot and ascending
+neighbour lists of the targets.  Context wraps the extended per-cluster API.
+
+This module only binds libabyss_regf.so; there is no CPU fallback.  If the
+library is missing, loading raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libabyss_regf.so")
+
+OK = 0
+ERR_NOT_OPEN = 1
+ERR_NO_DEVICE = 2
+ERR_INVALID_ARG = 3
+ERR_TARGET_RANGE = 4
+ERR_INVALID_INPUT = 5
+ERR_NO_JSET = 6
+ERR_HIP = 7
+ERR_OUT_OF_MEMORY = 8
+ERR_NEIGHBOR_OVERFLOW = 9
+ERR_ADMISSION = 10
+ERR_BUSY = 11
+
+PREC_SPLIT = 0
+PREC_FP64 = 1
+
+# Every exported symbol of include/abyss_regf.h (checked by the CPU tests).
+EXPORTS = [
+    "InitializeDevice", "OpenDevice", "CloseDevice", "ProfileDevice", "SendToDevice",
+    "CalculateAccelerationOnDevice",
+    "abyss_last_error", "abyss_error_string", "abyss_last_error_detail",
+    "abyss_num_neighbor_max", "abyss_set_num_neighbor_max", "abyss_set_precision",
+    "abyss_device_count",
+    "abyss_ctx_create", "abyss_ctx_destroy", "abyss_ctx_set_precision", "abyss_ctx_set_tuning",
+    "abyss_ctx_set_scratch_limit", "abyss_ctx_upload", "abyss_ctx_query",
+    "abyss_ctx_set_targets", "abyss_ctx_launch", "abyss_ctx_sync", "abyss_ctx_fetch",
+    "abyss_ctx_stream", "abyss_ctx_plan", "abyss_ctx_error_detail",
+]
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_lib = None
+
+
+class AbyssError(RuntimeError):
+    def __init__(self, code, detail=""):
+        self.code = code
+        super().__init__(f"{error_string(code)} (code {code}){': ' + detail if detail else ''}")
+
+
+def load():
+    """Load libabyss_regf.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with abyss_amd/build.py "
+                          "(no CPU fallback exists)")
+    lib = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "InitializeDevice": (None, [_ip]),
+        "OpenDevice": (None, [_ip]),
+        "CloseDevice": (None, []),
+        "ProfileDevice": (None, [_ip]),
+        "SendToDevice": (None, [_ip, _dp, _dp, _dp, _dp, _dp]),
+        "CalculateAccelerationOnDevice": (None, [_ip, _ip, _dp, _dp, _ip, _ip]),
+        "abyss_last_error": (ctypes.c_int, []),
+        "abyss_error_string": (ctypes.c_char_p, [ctypes.c_int]),
+        "abyss_last_error_detail": (ctypes.c_char_p, []),
+        "abyss_num_neighbor_max": (ctypes.c_int, []),
+        "abyss_set_num_neighbor_max": (ctypes.c_int, [ctypes.c_int]),
+        "abyss_set_precision": (ctypes.c_int, [ctypes.c_int]),
+        "abyss_device_count": (ctypes.c_int, []),
+        "abyss_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+        "abyss_ctx_destroy": (ctypes.c_int, [_vp]),
+        "abyss_ctx_set_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "abyss_ctx_set_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+        "abyss_ctx_set_scratch_limit": (ctypes.c_int, [_vp, ctypes.c_ulonglong]),
+        "abyss_ctx_upload": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
+        "abyss_ctx_query": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _ip, _ip]),
+        "abyss_ctx_set_targets": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+        "abyss_ctx_launch": (ctypes.c_int, [_vp]),
+        "abyss_ctx_sync": (ctypes.c_int, [_vp]),
+        "abyss_ctx_fetch": (ctypes.c_int, [_vp, _dp, _dp, _ip, _ip]),
+        "abyss_ctx_stream": (_vp, [_vp]),
+        "abyss_ctx_plan": (ctypes.c_int, [_vp, _ip, _ip, _ip, _ip]),
+        "abyss_ctx_error_detail": (ctypes.c_char_p, [_vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def error_string(code):
+    return load().abyss_error_string(code).decode()
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip) if a is not None else None
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a.reshape(shape) if shape is not None else a
+
+
+# --------------------------------------------------------------------------
+# Legacy API (reference names)
+# --------------------------------------------------------------------------
+def last_error():
+    return load().abyss_last_error()
+
+
+def last_error_detail():
+    return load().abyss_last_error_detail().decode()
+
+
+def _check_legacy():
+    rc = last_error()
+    if rc not in (OK, ERR_NEIGHBOR_OVERFLOW):
+        raise AbyssError(rc, last_error_detail())
+    return rc
+
+
+def InitializeDevice(irank=0):
+    load().InitializeDevice(ctypes.byref(ctypes.c_int(irank)))
+    return _check_legacy()
+
+
+def OpenDevice(irank=0):
+    load().OpenDevice(ctypes.byref(ctypes.c_int(irank)))
+    return _check_legacy()
+
+
+def CloseDevice():
+    load().CloseDevice()
+    return _check_legacy()
+
+
+def ProfileDevice(irank=0):
+    load().ProfileDevice(ctypes.byref(ctypes.c_int(irank)))
+    return last_error()
+
+
+def SendToDevice(m, x, v, r2, mdot=None):
+    """Replace the j-set (x, v: [N,3]; m, r2: [N]); mdot is ignored."""
+    x = _f64(x, (-1, 3))
+    n = x.shape[0]
+    v = _f64(v, (n, 3))
+    m = _f64(m, (n,))
+    r2 = _f64(r2, (n,))
+    md = np.zeros(n) if mdot is None else _f64(mdot, (n,))
+    load().SendToDevice(ctypes.byref(ctypes.c_int(n)), _d(m), _d(x), _d(v), _d(r2), _d(md))
+    return _check_legacy()
+
+
+def num_neighbor_max():
+    return load().abyss_num_neighbor_max()
+
+
+def set_num_neighbor_max(cap):
+    rc = load().abyss_set_num_neighbor_max(int(cap))
+    if rc != OK:
+        raise AbyssError(rc, "the stride can only change while the library is closed")
+
+
+def set_precision(mode):
+    rc = load().abyss_set_precision(int(mode))
+    if rc != OK:
+        raise AbyssError(rc)
+
+
+def CalculateAccelerationOnDevice(targets):
+    """Returns (acc[nt,3], adot[nt,3], NumNeighbor[nt], NeighborList[nt, NumNeighborMax])."""
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    nt = t.size
+    cap = num_neighbor_max()
+    acc = np.zeros((max(nt, 1), 3))
+    adot = np.zeros((max(nt, 1), 3))
+    nnb = np.zeros(max(nt, 1), dtype=np.int32)
+    lst = np.full((max(nt, 1), cap), -1, dtype=np.int32)
+    load().CalculateAccelerationOnDevice(ctypes.byref(ctypes.c_int(nt)), _i(t) if nt else None,
+                                         _d(acc), _d(adot), _i(nnb), _i(lst))
+    _check_legacy()
+    return acc[:nt], adot[:nt], nnb[:nt], lst[:nt]
+
+
+def device_count():
+    return load().abyss_device_count()
+
+
+# --------------------------------------------------------------------------
+# Context API
+# --------------------------------------------------------------------------
+class Context:
+    """One device-resident j-set (a cluster) with its own stream and buffers."""
+
+    def __init__(self, device=0, cap=256, precision=PREC_SPLIT):
+        self._lib = load()
+        h = _vp()
+        rc = self._lib.abyss_ctx_create(device, cap, ctypes.byref(h))
+        if rc != OK:
+            raise AbyssError(rc)
+        self.h = h
+        self.cap = cap
+        self.n = 0
+        self.nt = 0
+        if precision != PREC_SPLIT:
+            self.set_precision(precision)
+
+    def _chk(self, rc, allow_overflow=False):
+        if rc == OK or (allow_overflow and rc == ERR_NEIGHBOR_OVERFLOW):
+            return rc
+        raise AbyssError(rc, self._lib.abyss_ctx_error_detail(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self._lib.abyss_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_precision(self, mode):
+        self._chk(self._lib.abyss_ctx_set_precision(self.h, int(mode)))
+
+    def set_tuning(self, i_per_lane=0, j_splits=0):
+        self._chk(self._lib.abyss_ctx_set_tuning(self.h, int(i_per_lane), int(j_splits)))
+
+    def set_scratch_limit(self, nbytes):
+        self._chk(self._lib.abyss_ctx_set_scratch_limit(self.h, int(nbytes)))
+
+    def upload(self, m, x, v, r2):
+        x = _f64(x, (-1, 3))
+        n = x.shape[0]
+        self._chk(self._lib.abyss_ctx_upload(self.h, n, _d(_f64(m, (n,))), _d(x),
+                                             _d(_f64(v, (n, 3))), _d(_f64(r2, (n,)))))
+        self.n = n
+
+    def set_targets(self, targets):
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        self._chk(self._lib.abyss_ctx_set_targets(self.h, t.size, _i(t) if t.size else None))
+        self.nt = t.size
+
+    def launch(self):
+        self._chk(self._lib.abyss_ctx_launch(self.h))
+
+    def sync(self):
+        self._chk(self._lib.abyss_ctx_sync(self.h))
+
+    def fetch(self):
+        nt = self.nt
+        acc = np.zeros((nt, 3))
+        adot = np.zeros((nt, 3))
+        nnb = np.zeros(nt, dtype=np.int32)
+        lst = np.full((nt, self.cap), -1, dtype=np.int32)
+        self.last_status = self._chk(
+            self._lib.abyss_ctx_fetch(self.h, _d(acc), _d(adot), _i(nnb), _i(lst)),
+            allow_overflow=True)
+        return acc, adot, nnb, lst
+
+    def query(self, targets):
+        """Synchronous query: (acc, adot, nnb, lists[nt, cap])."""
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        nt = t.size
+        acc = np.zeros((max(nt, 1), 3))
+        adot = np.zeros((max(nt, 1), 3))
+        nnb = np.zeros(max(nt, 1), dtype=np.int32)
+        lst = np.full((max(nt, 1), self.cap), -1, dtype=np.int32)
+        self.last_status = self._chk(
+            self._lib.abyss_ctx_query(self.h, nt, _i(t) if nt else None, _d(acc), _d(adot),
+                                      _i(nnb), _i(lst)), allow_overflow=True)
+        self.nt = nt
+        return acc[:nt], adot[:nt], nnb[:nt], lst[:nt]
+
+    def plan(self):
+        a, b, c, d = (ctypes.c_int() for _ in range(4))
+        self._chk(self._lib.abyss_ctx_plan(self.h, ctypes.byref(a), ctypes.byref(b),
+                                           ctypes.byref(c), ctypes.byref(d)))
+        return {"i_per_lane": a.value, "j_splits": b.value, "grid_x": c.value, "tile": d.value}
+
+    @property
+    def stream(self):
+        return self._lib.abyss_ctx_stream(self.h)

@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Benchmark: regular-force pair-interactions/s on one MI355X.
+
+Workload (default): King W0=6 cluster, N=262144, Kroupa IMF, 5% of the
+particles in hard binaries, r_nb for ~100 neighbours, list cap 256; one
+step = one full-N regular-force + neighbour-list evaluation (m = N targets)
+through the library's device-resident launch path (inputs already in HBM).
+
+Prints ONE JSON line.  Progress goes to stderr.
+  value     = interactions / s = steps * m * (N - 1) / (HIP-event time)
+  roofline  = FP32 VALU roof: 60 flop per interaction (the reference's
+              convention) / kernel time vs 157.3 TFLOP/s
+  cpu_baseline = the oracle's FP64 C loop (OpenMP over targets) on a bounded
+              sample of the same workload on this host's cores
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3
+FLOP_PER_INTERACTION = 60
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class HipEvents:
+    """hipEvent timing on an explicit stream (the library's context stream)."""
+
+    def __init__(self):
+        lib = None
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                lib = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if lib is None:
+            raise RuntimeError("libamdhip64.so not found")
+        self.lib = lib
+        lib.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        lib.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        lib.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p,
+                                            ctypes.c_void_p]
+        lib.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def create(self):
+        e = ctypes.c_void_p()
+        assert self.lib.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.lib.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, e0, e1):
+        assert self.lib.hipEventSynchronize(e1) == 0
+        ms = ctypes.c_float()
+        assert self.lib.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+        return float(ms.value)
+
+
+def cpu_baseline(js, target_seconds=10.0):
+    """Time the oracle's FP64 loop (all host cores) on a bounded target sample."""
+    import numpy as np
+    import oracle
+    threads = oracle.max_threads()
+    rng = np.random.default_rng(7)
+    order = rng.permutation(js.n).astype(np.int32)
+    # calibrate on a small sample, then size the timed sample to ~target_seconds
+    probe = order[:max(threads * 4, 64)]
+    t0 = time.perf_counter()
+    oracle.regf(js.x, js.v, js.m, js.r2, probe, js.cap, nthreads=threads)
+    dt = time.perf_counter() - t0
+    rate = len(probe) / max(dt, 1e-6)
+    count = int(min(js.n, max(len(probe), rate * target_seconds)))
+    sample = order[:count]
+    t0 = time.perf_counter()
+    oracle.regf(js.x, js.v, js.m, js.r2, sample, js.cap, nthreads=threads)
+    dt = time.perf_counter() - t0
+    inter = float(count) * (js.n - 1)
+    return {"value": inter / dt, "unit": "interactions/s", "cores": threads, "kind": "port",
+            "sample": f"{count} random targets of the same N={js.n} j-set "
+                      f"({inter:.3e} interactions, {dt:.1f} s, FP64 C loop, OpenMP {threads} threads)"}
+
+
+def accuracy_sample(js, res, nsample=256):
+    import numpy as np
+    import oracle
+    from oracle.compare import compare_lists, force_errors
+    acc, adot, nnb, lst = res
+    t = np.random.default_rng(11).permutation(js.n)[:nsample].astype(np.int32)
+    ra, rj, rn, rl, sa, sj = oracle.regf(js.x, js.v, js.m, js.r2, t, js.cap, scales=True)
+    err = force_errors(acc[t], adot[t], ra, rj, sa, sj)
+    lists = compare_lists(nnb[t], lst[t], rn, rl, js.x, js.r2, t)
+    return {"sample_targets": nsample, "a_rel_max": err["a_rel"]["max"],
+            "a_rel_p99": err["a_rel"]["p99"], "jdot_rel_max": err["jdot_rel"]["max"],
+            "jdot_rel_p99": err["jdot_rel"]["p99"], "a_norm_max": err["a_norm"]["max"],
+            "jdot_norm_max": err["jdot_norm"]["max"],
+            "list_hard_mismatches": lists["hard_mismatches"],
+            "list_band_pairs_flipped": lists["band_pairs_flipped"]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=262144)
+    ap.add_argument("--precision", choices=["split", "fp64"], default="split")
+    ap.add_argument("--ipl", type=int, default=0)
+    ap.add_argument("--splits", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-check", action="store_true", help="skip the sampled accuracy check")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the regular-force path runs on a single MI355X "
+              "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from abyss_amd import regf, workloads
+
+    t0 = time.perf_counter()
+    js = workloads.king_binaries(args.n)
+    log(f"workload {js.name}: built in {time.perf_counter() - t0:.1f} s")
+    import numpy as np
+    targets = np.arange(js.n, dtype=np.int32)
+    ctx = regf.Context(device=0, cap=js.cap,
+                       precision=regf.PREC_FP64 if args.precision == "fp64" else regf.PREC_SPLIT)
+    ctx.set_tuning(args.ipl, args.splits)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    ctx.set_targets(targets)
+    plan = ctx.plan()
+    log(f"plan {plan}")
+    for _ in range(args.warmup):
+        ctx.launch()
+    ctx.sync()
+    log("warmup done")
+    ev = HipEvents()
+    e0, e1 = ev.create(), ev.create()
+    stream = ctx.stream
+    w0 = time.perf_counter()
+    ev.record(e0, stream)
+    for _ in range(args.steps):
+        ctx.launch()
+    ev.record(e1, stream)
+    ms = ev.elapsed_ms(e0, e1)
+    ctx.sync()
+    wall = time.perf_counter() - w0
+    ms_per_step = ms / args.steps
+    inter = float(len(targets)) * (js.n - 1)
+    value = inter / (ms_per_step * 1e-3)
+    achieved = FLOP_PER_INTERACTION * inter / (ms_per_step * 1e-3) / 1e12
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step (wall {wall:.3f} s), "
+        f"{value:.4e} interactions/s, {achieved:.2f} TFLOP/s")
+
+    acc_info = None
+    if not args.no_check:
+        res = ctx.fetch()
+        acc_info = accuracy_sample(js, res)
+        log(f"accuracy {acc_info}")
+    ctx.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(js, args.cpu_seconds)
+        log(f"cpu baseline {cpu}")
+
+    line = {
+        "metric": "regular-force pair-interactions/s",
+        "value": value,
+        "unit": "interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 i-j differences and per-particle sums)" if args.precision == "split"
+                 else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{js.name}_fullN", "n": js.n, "active": int(len(targets)),
+                   "neighbours_target": js.meta["k_nb"], "cap": js.cap,
+                   "precision": args.precision, "plan": plan},
+        "pct_fp32_peak": 100.0 * achieved / FP32_PEAK_TFLOPS,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "flop_per_interaction": FLOP_PER_INTERACTION,
+                     "interactions_per_launch": inter},
+        "cpu_baseline": cpu,
+        "accuracy": acc_info,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

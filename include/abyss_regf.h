@@ -1,0 +1,168 @@
+/*
+ * abyss_regf.h -- C ABI of the MI355X (gfx950) regular-force library.
+ *
+ * Part 1 keeps the six entry points ABYSS's host code calls, with their
+ * signatures unchanged and CUDA_REAL = double (the serial build's choice,
+ * src/defs.h:97-102; src_mpi's callers have matching double branches,
+ * src_mpi/cuda/CalculateRegularAcceleration.cpp:152-153,380-429).
+ * Part 2 adds explicit contexts (one per cluster / per thread), a
+ * device-resident launch path, tuning controls and error reporting.
+ * Part 3 is the batched-ensemble and served-ensemble interface.
+ *
+ * Semantics shared by every query (reference: src_mpi/cuda/cuda_kernels.cu
+ * :269-450 and the callers cited below):
+ *   - acc/adot are the REGULAR part only: the sum over j with
+ *     r_ij^2 > r_nb,i^2 of m_j r_ij/|r_ij|^3 and
+ *     m_j [v_ij/|r_ij|^3 - 3 (r_ij . v_ij) r_ij/|r_ij|^5]  (G = 1, no softening),
+ *     with r_ij = x_j - x_i, v_ij = v_j - v_i.
+ *   - j with r_ij^2 <= r_nb,i^2 and j != i (by index) are neighbours; they are
+ *     listed in ascending j order at row k of NeighborList, row stride
+ *     NumNeighborMax; NumNeighbor[k] is their count.  A coincident j != i
+ *     (r = 0) is a neighbour and adds no force.
+ *   - r_nb,i^2 is the TARGET's own value from the last upload (r2[i]).
+ *   - neighbour indices and targets index the j-set of the last upload.
+ *   - If a list would exceed NumNeighborMax, the first NumNeighborMax
+ *     neighbours are written, NumNeighbor[k] = -(true count), and the call's
+ *     status is ABYSS_ERR_NEIGHBOR_OVERFLOW (the reference asserts instead,
+ *     src_mpi/cuda/cuda_kernels.cu:363,432).
+ *
+ * Precision (default ABYSS_PREC_SPLIT): positions and velocities are FP64 on
+ * the device and the i-j differences are taken in FP64; r^2, rsqrt and the
+ * force/jerk terms are FP32, summed in FP32 over short runs of j and folded
+ * into FP64 per-particle sums.  ABYSS_PREC_FP64 evaluates everything in FP64.
+ *
+ * Threading: the legacy entry points share one process-wide default context
+ * and are not re-entrant (as in the reference, whose state is file-static,
+ * src_mpi/cuda/cuda_my_acceleration.cu:23-88).  A context of part 2 may be
+ * used by one host thread at a time; distinct contexts are independent.
+ * All calls are synchronous unless named *_launch / *_async.
+ */
+#ifndef ABYSS_REGF_H
+#define ABYSS_REGF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Neighbour-list row stride of the legacy entry points: a compile-time
+ * contract shared with the caller, which allocates NumTarget*NumNeighborMax
+ * ints (src_mpi/cuda/CalculateRegularAcceleration.cpp:91).  1000 matches
+ * src_mpi/def.h:20; override at build time with -DABYSS_NUM_NEIGHBOR_MAX=...
+ * or at run time (before OpenDevice) with abyss_set_num_neighbor_max(). */
+#ifndef ABYSS_NUM_NEIGHBOR_MAX
+#define ABYSS_NUM_NEIGHBOR_MAX 1000
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+#define ABYSS_OK                    0
+#define ABYSS_ERR_NOT_OPEN          1  /* device not initialised / opened */
+#define ABYSS_ERR_NO_DEVICE         2  /* no usable HIP device */
+#define ABYSS_ERR_INVALID_ARG       3  /* null pointer, bad size or count */
+#define ABYSS_ERR_TARGET_RANGE      4  /* target index outside [0, N) */
+#define ABYSS_ERR_INVALID_INPUT     5  /* non-finite x/v/m, m < 0, r2 < 0 or NaN */
+#define ABYSS_ERR_NO_JSET           6  /* query before any successful upload */
+#define ABYSS_ERR_HIP               7  /* a HIP runtime call failed */
+#define ABYSS_ERR_OUT_OF_MEMORY     8
+#define ABYSS_ERR_NEIGHBOR_OVERFLOW 9  /* results valid; some NumNeighbor < 0 */
+#define ABYSS_ERR_ADMISSION        10  /* served ensemble: HBM budget exhausted */
+#define ABYSS_ERR_BUSY             11  /* served ensemble: no free slot */
+
+#define ABYSS_PREC_SPLIT 0
+#define ABYSS_PREC_FP64  1
+
+/* ===================================================================== */
+/* Part 1: legacy entry points (src_mpi/cuda/cuda_functions.h:3-10,      */
+/* defined at src_mpi/cuda/cuda_my_acceleration.cu:1087-1106;            */
+/* identical list in src/cuda/cuda_functions.h:3-10).                    */
+/* ===================================================================== */
+
+/* Selects device (*irank mod device count), creates the stream and marks
+ * the library open.  Replaces _InitializeDevice
+ * (src/cuda/cuda_my_acceleration.cu:415-455, which also sets is_open). */
+void InitializeDevice(int *irank);
+
+/* Initialises (if needed) and opens; a second open only reports it.
+ * Replaces _OpenDevice (src_mpi/cuda/cuda_my_acceleration.cu:987-1008). */
+void OpenDevice(const int *irank);
+
+/* Closes the library and frees its device and pinned buffers (the reference
+ * only flips is_open, src_mpi/cuda/cuda_my_acceleration.cu:1012-1047). */
+void CloseDevice(void);
+
+/* Prints and resets the call/interaction/time counters, with the 60-flop
+ * interaction convention (src_mpi/cuda/cuda_my_acceleration.cu:1051-1062). */
+void ProfileDevice(int *irank);
+
+/* Replaces the whole j-set: N = *NNB particles, masses m[N], positions
+ * x[N][3], velocities v[N][3], r2[N] = r_nb^2 per particle; mdot is ignored.
+ * Inputs are copied before return.  Replaces _ReceiveFromHost
+ * (src_mpi/cuda/cuda_my_acceleration.cu:603-778; src/...:313-411). */
+void SendToDevice(int *NNB, double m[], double x[][3], double v[][3],
+                  double r2[], double mdot[]);
+
+/* Regular acc/adot and neighbour lists of *NumTarget targets
+ * h_target_list[k] in [0, N), 0 < *NumTarget <= N.  Outputs for k <
+ * *NumTarget: acc[k][3], adot[k][3], NumNeighbor[k],
+ * NeighborList[k*NumNeighborMax + 0..count).  Replaces GetAcceleration
+ * (src/cuda/cuda_my_acceleration.cu:66-302; src_mpi/...:95-344). */
+void CalculateAccelerationOnDevice(int *NumTarget, int *h_target_list,
+                                   double acc[][3], double adot[][3],
+                                   int NumNeighbor[], int *NeighborList);
+
+/* ===================================================================== */
+/* Part 2: status, contexts, device-resident path, tuning.              */
+/* No counterpart in the reference (which has no return codes).         */
+/* ===================================================================== */
+
+/* Status of the last legacy call on this process (ABYSS_OK or an error). */
+int abyss_last_error(void);
+const char *abyss_error_string(int code);
+/* Human-readable detail of the last error of the legacy API. */
+const char *abyss_last_error_detail(void);
+/* Legacy list stride; setter only allowed while the library is closed. */
+int abyss_num_neighbor_max(void);
+int abyss_set_num_neighbor_max(int cap);
+/* Legacy API precision (ABYSS_PREC_SPLIT or ABYSS_PREC_FP64). */
+int abyss_set_precision(int mode);
+/* Number of HIP devices visible (0 without a GPU; never fails). */
+int abyss_device_count(void);
+
+typedef struct abyss_ctx_s *abyss_ctx_t;
+
+/* Creates a context on `device` with neighbour-list stride `cap` (>0). */
+int abyss_ctx_create(int device, int cap, abyss_ctx_t *out);
+int abyss_ctx_destroy(abyss_ctx_t ctx);
+int abyss_ctx_set_precision(abyss_ctx_t ctx, int mode);
+/* Forces i-per-lane (1, 2 or 4) and the j-split count; 0 = automatic. */
+int abyss_ctx_set_tuning(abyss_ctx_t ctx, int i_per_lane, int j_splits);
+/* Upper bound on the split-j scratch (bytes; default 1 GiB). */
+int abyss_ctx_set_scratch_limit(abyss_ctx_t ctx, unsigned long long bytes);
+
+/* Upload a j-set: x, v as [n][3], m and r2 as [n] (FP64, host memory). */
+int abyss_ctx_upload(abyss_ctx_t ctx, int n, const double *m, const double *x,
+                     const double *v, const double *r2);
+/* Synchronous query with host buffers (list row stride = ctx cap). */
+int abyss_ctx_query(abyss_ctx_t ctx, int nt, const int *targets, double *acc,
+                    double *adot, int *nnb, int *list);
+
+/* Device-resident path: upload targets once, then enqueue any number of
+ * launches on the context stream (no host synchronisation, results stay in
+ * HBM), then fetch the last launch's results. */
+int abyss_ctx_set_targets(abyss_ctx_t ctx, int nt, const int *targets);
+int abyss_ctx_launch(abyss_ctx_t ctx);
+int abyss_ctx_sync(abyss_ctx_t ctx);
+int abyss_ctx_fetch(abyss_ctx_t ctx, double *acc, double *adot, int *nnb,
+                    int *list);
+/* hipStream_t of the context (for event timing by the caller). */
+void *abyss_ctx_stream(abyss_ctx_t ctx);
+/* Plan chosen for the current targets: i per lane, j splits, grid x, tile. */
+int abyss_ctx_plan(abyss_ctx_t ctx, int *i_per_lane, int *j_splits,
+                   int *grid_x, int *tile);
+/* Detail string of the context's last error. */
+const char *abyss_ctx_error_detail(abyss_ctx_t ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ABYSS_REGF_H */

@@ -1,0 +1,109 @@
+"""ORACLE -- test infrastructure only (see regf_oracle.c for what it restates).
+
+ctypes wrapper over oracle/liboracle.so.  Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg import this package, and only as the checker.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        from . import build_oracle
+        build_oracle.build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    lib.oracle_regf.restype = ctypes.c_int
+    lib.oracle_regf.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_int, _ip,
+                                _dp, _dp, _ip, _ip, ctypes.c_int, _dp, _dp, ctypes.c_int]
+    lib.oracle_band_pairs.restype = None
+    lib.oracle_band_pairs.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _ip,
+                                      ctypes.c_double, _ip, _ip, ctypes.c_int, ctypes.c_int]
+    lib.oracle_predict2.restype = None
+    lib.oracle_predict2.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.oracle_max_threads.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def regf(x, v, m, rnb2, targets, cap, nthreads=0, scales=False):
+    """FP64 regular force + neighbour lists.
+
+    Returns (acc[nt,3], adot[nt,3], nnb[nt], lists[nt,cap]) and, with
+    scales=True, also (sum_j |a-term|, sum_j |jdot-term|) per target.
+    nnb[k] < 0 flags overflow (-(true count)); the first cap entries are kept.
+    """
+    lib = _load()
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    rnb2 = np.ascontiguousarray(rnb2, dtype=np.float64)
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    n = x.shape[0]
+    assert v.shape == (n, 3) and m.shape == (n,) and rnb2.shape == (n,)
+    if t.size and (t.min() < 0 or t.max() >= n):
+        raise IndexError("target index out of range")
+    nt = t.size
+    acc = np.zeros((nt, 3))
+    adot = np.zeros((nt, 3))
+    nnb = np.zeros(nt, dtype=np.int32)
+    lists = np.full((nt, max(cap, 1)), -1, dtype=np.int32)
+    sa = np.zeros(nt) if scales else None
+    sj = np.zeros(nt) if scales else None
+    lib.oracle_regf(n, _d(x), _d(v), _d(m), _d(rnb2), nt, _i(t), _d(acc), _d(adot),
+                    _i(nnb), _i(lists), cap, _d(sa) if scales else None,
+                    _d(sj) if scales else None, nthreads)
+    if scales:
+        return acc, adot, nnb, lists, sa, sj
+    return acc, adot, nnb, lists
+
+
+def band_pairs(x, rnb2, targets, rel=1e-6, bcap=64, nthreads=0):
+    """Per target: j != i with |r^2 - r_nb^2| <= rel * r_nb^2 (count, list)."""
+    lib = _load()
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    rnb2 = np.ascontiguousarray(rnb2, dtype=np.float64)
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    nt = t.size
+    bcnt = np.zeros(nt, dtype=np.int32)
+    blist = np.full((nt, bcap), -1, dtype=np.int32)
+    lib.oracle_band_pairs(x.shape[0], _d(x), _d(rnb2), nt, _i(t), rel, _i(bcnt),
+                          _i(blist), bcap, nthreads)
+    return bcnt, blist
+
+
+def predict2(x, v, a, adot, dt):
+    """Second-order predictor of the reference caller (per-particle dt)."""
+    lib = _load()
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 3)
+    adot = np.ascontiguousarray(adot, dtype=np.float64).reshape(-1, 3)
+    dt = np.ascontiguousarray(np.broadcast_to(dt, (x.shape[0],)), dtype=np.float64)
+    xp = np.empty_like(x)
+    vp = np.empty_like(v)
+    lib.oracle_predict2(x.shape[0], _d(x), _d(v), _d(a), _d(adot), _d(dt), _d(xp), _d(vp))
+    return xp, vp
+
+
+def max_threads():
+    return _load().oracle_max_threads()

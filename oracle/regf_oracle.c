@@ -1,0 +1,187 @@
+/*
+ * ORACLE -- test infrastructure only.
+ *
+ * Plain-C FP64 restatement of ABYSS's regular-force + neighbour-list step.
+ * It is the checker for the HIP library and the CPU baseline that bench.py
+ * times; nothing in the product path links, loads or calls it.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may use it.
+ *
+ * What it restates (reference paths relative to the reference checkout):
+ *   - pair formula and split rule:
+ *       src/cuda/cuda_kernels.cu:283-317 (FP64 compute_forces, serial build)
+ *       src_mpi/cuda/cuda_kernels.cu:331-364 (same rule, float build)
+ *       src_mpi/Particle/ComputeAcceleration.cpp:253-299 (CPU FP64 loop:
+ *         m_r3 = m/r2/sqrt(r2); a += m_r3*x; adot += m_r3*(v - 3*x*vx/r2))
+ *     j is REGULAR when r2 > r_nb,i^2; otherwise, if j != i by index, j is a
+ *     NEIGHBOUR (src_mpi/Particle/Initialize.cpp:67 uses the same "> is
+ *     regular, <= is neighbour" split).  Self-exclusion is by index only, so a
+ *     coincident j != i (r2 == 0) is a neighbour and adds no force.
+ *   - output meaning (src_mpi/cuda/cuda_functions.h:3-10, reduction and gather
+ *     at src_mpi/cuda/cuda_kernels.cu:384-450): acc/adot hold the regular part
+ *     only; the neighbour list of target k is written at row k with stride
+ *     `cap`, in ascending j order (gather concatenates the j-blocks in order).
+ *   - the target's own r_nb^2 is used (r2[i_ptcl], src_mpi/cuda/cuda_kernels.cu:290).
+ *
+ * Overflow (count > cap) is an error in the reference (device assert,
+ * src_mpi/cuda/cuda_kernels.cu:363,432).  Here the first `cap` entries are
+ * written and nnb[k] = -(true count), the convention the HIP library uses.
+ *
+ * Parity pinning (tests/test_oracle.py): two-body closed forms, a = -grad(phi)
+ * and jdot = d a/dt by finite differences, momentum conservation, homogeneity
+ * scalings, r_nb -> 0 / inf limits, the exact "<=" boundary case, coincident
+ * particles, and a vectorised numpy brute force on small N.
+ *
+ * Build: oracle/build_oracle.py (gcc -O3 -fopenmp -ffp-contract=off).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* One target against all j.  Returns the true neighbour count. */
+static int oracle_one(int n, const double *x, const double *v, const double *m,
+                      int i, double rnb2, double *a_out, double *j_out,
+                      int *list, int cap, double *scale_a, double *scale_j)
+{
+    const double xi = x[3 * i + 0], yi = x[3 * i + 1], zi = x[3 * i + 2];
+    const double vxi = v[3 * i + 0], vyi = v[3 * i + 1], vzi = v[3 * i + 2];
+    double ax = 0, ay = 0, az = 0, jx = 0, jy = 0, jz = 0;
+    double sa = 0, sj = 0;
+    int cnt = 0;
+    for (int j = 0; j < n; ++j) {
+        const double dx = x[3 * j + 0] - xi;
+        const double dy = x[3 * j + 1] - yi;
+        const double dz = x[3 * j + 2] - zi;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 > rnb2) {
+            const double dvx = v[3 * j + 0] - vxi;
+            const double dvy = v[3 * j + 1] - vyi;
+            const double dvz = v[3 * j + 2] - vzi;
+            const double vx = dvx * dx + dvy * dy + dvz * dz;
+            const double m_r3 = m[j] / r2 / sqrt(r2);
+            ax += m_r3 * dx;
+            ay += m_r3 * dy;
+            az += m_r3 * dz;
+            jx += m_r3 * (dvx - 3 * dx * vx / r2);
+            jy += m_r3 * (dvy - 3 * dy * vx / r2);
+            jz += m_r3 * (dvz - 3 * dz * vx / r2);
+            /* magnitude scales used to normalise errors: sum_j |term_j| */
+            const double r = sqrt(r2);
+            sa += fabs(m[j]) / r2;
+            sj += fabs(m_r3) * (sqrt(dvx * dvx + dvy * dvy + dvz * dvz) + 3.0 * fabs(vx) / r);
+        } else if (j != i) {
+            if (cnt < cap) list[cnt] = j;
+            ++cnt;
+        }
+    }
+    a_out[0] = ax; a_out[1] = ay; a_out[2] = az;
+    j_out[0] = jx; j_out[1] = jy; j_out[2] = jz;
+    if (scale_a) *scale_a = sa;
+    if (scale_j) *scale_j = sj;
+    return cnt;
+}
+
+/*
+ * Regular force + neighbour lists for `nt` targets.
+ *   x, v: [n][3] FP64, m: [n], rnb2: [n] (r_nb^2 per particle, as uploaded by
+ *   SendToDevice), targets: [nt] indices into the j-set.
+ *   acc, adot: [nt][3]; nnb: [nt]; list: [nt][cap]
+ *   scale_a, scale_j: optional [nt] sums of |term| (may be NULL)
+ * Returns the number of targets whose list overflowed `cap`.
+ */
+int oracle_regf(int n, const double *x, const double *v, const double *m,
+                const double *rnb2, int nt, const int *targets,
+                double *acc, double *adot, int *nnb, int *list, int cap,
+                double *scale_a, double *scale_j, int nthreads)
+{
+    int overflow = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : overflow)
+#endif
+    for (int k = 0; k < nt; ++k) {
+        const int i = targets[k];
+        int c = oracle_one(n, x, v, m, i, rnb2[i], acc + 3 * k, adot + 3 * k,
+                           list + (int64_t)k * cap, cap,
+                           scale_a ? scale_a + k : NULL,
+                           scale_j ? scale_j + k : NULL);
+        if (c > cap) {
+            nnb[k] = -c;
+            ++overflow;
+        } else {
+            nnb[k] = c;
+        }
+    }
+    (void)nthreads;
+    return overflow;
+}
+
+/*
+ * Pairs whose r^2 lies within `rel` of r_nb,i^2 (|r2 - rnb2| <= rel*rnb2),
+ * j != i.  For each target, up to `bcap` such j are written (ascending) and
+ * the count is returned in bcnt[k].  These are the only pairs whose
+ * neighbour/regular classification may legitimately differ between a
+ * reduced-precision evaluation and this FP64 oracle.
+ */
+void oracle_band_pairs(int n, const double *x, const double *rnb2, int nt,
+                       const int *targets, double rel, int *bcnt, int *blist,
+                       int bcap, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (int k = 0; k < nt; ++k) {
+        const int i = targets[k];
+        const double xi = x[3 * i], yi = x[3 * i + 1], zi = x[3 * i + 2];
+        const double r2nb = rnb2[i];
+        int c = 0;
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            const double dx = x[3 * j] - xi, dy = x[3 * j + 1] - yi,
+                         dz = x[3 * j + 2] - zi;
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (fabs(r2 - r2nb) <= rel * r2nb) {
+                if (c < bcap) blist[(int64_t)k * bcap + c] = j;
+                ++c;
+            }
+        }
+        bcnt[k] = c;
+    }
+    (void)nthreads;
+}
+
+/*
+ * Second-order j prediction used by the reference caller before upload:
+ *   x_p = ((adot*dt/3 + a)*dt/2 + v)*dt + x,  v_p = (adot*dt/2 + a)*dt + v
+ * (src_mpi/Particle/UpdateParticle.cpp:37-59; dt == 0 returns x, v).
+ */
+void oracle_predict2(int n, const double *x, const double *v, const double *a,
+                     const double *adot, const double *dt, double *xp,
+                     double *vp)
+{
+    for (int i = 0; i < n; ++i) {
+        const double h = dt[i];
+        for (int d = 0; d < 3; ++d) {
+            const int q = 3 * i + d;
+            if (h == 0) {
+                xp[q] = x[q];
+                vp[q] = v[q];
+            } else {
+                xp[q] = ((adot[q] * h / 3 + a[q]) * h / 2 + v[q]) * h + x[q];
+                vp[q] = (adot[q] * h / 2 + a[q]) * h + v[q];
+            }
+        }
+    }
+}
+
+int oracle_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

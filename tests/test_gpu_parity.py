@@ -1,0 +1,267 @@
+"""GPU parity: the HIP library (through its C ABI) against the FP64 oracle.
+
+Bars (north-star accuracy contract, split precision unless stated):
+  * neighbour lists: ascending, identical to the oracle's except pairs with
+    |r^2 - r_nb^2| <= 1e-6 r_nb^2, which are counted (hard_mismatches == 0);
+  * a and jdot: relative error per particle |d a| / |a| <= 1e-6 at the
+    99th percentile and normalised error |d a| / sum_j |term_j| <= 1e-6 at
+    the maximum (the plain relative error of a particle whose sum nearly
+    cancels is unbounded for any finite-precision evaluation);
+  * FP64 kernel: relative error <= 1e-12 and lists equal outside a 1e-12 band.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle.compare import compare_lists, force_errors
+from abyss_amd import regf, workloads
+
+pytestmark = pytest.mark.gpu
+
+TOL_REL_P99 = 1e-6
+TOL_NORM_MAX = 1e-6
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = regf.Context(device=0, cap=256)
+    yield c
+    c.close()
+
+
+def check(js, targets, res, band=1e-6, prec=regf.PREC_SPLIT, label=""):
+    acc, adot, nnb, lst = res
+    ra, rj, rn, rl, sa, sj = oracle.regf(js.x, js.v, js.m, js.r2, targets, js.cap, scales=True)
+    err = force_errors(acc, adot, ra, rj, sa, sj)
+    lists = compare_lists(nnb, lst, rn, rl, js.x, js.r2, targets, band=band)
+    say(f"[{label}] n={js.n} nt={len(targets)} a_rel={err['a_rel']} jdot_rel={err['jdot_rel']} "
+        f"a_norm_max={err['a_norm']['max']:.3e} jdot_norm_max={err['jdot_norm']['max']:.3e} "
+        f"lists={lists}")
+    assert lists["hard_mismatches"] == 0
+    assert lists["not_ascending"] == 0
+    assert lists["overflow_mismatch"] == 0
+    if prec == regf.PREC_FP64:
+        assert err["a_rel"]["max"] <= 1e-12
+        assert err["jdot_rel"]["max"] <= 1e-12
+    else:
+        assert err["a_rel"]["p99"] <= TOL_REL_P99
+        assert err["jdot_rel"]["p99"] <= TOL_REL_P99
+        assert err["a_norm"]["max"] <= TOL_NORM_MAX
+        assert err["jdot_norm"]["max"] <= TOL_NORM_MAX
+    return err, lists
+
+
+def test_two_body(ctx):
+    say("case: two_body")
+    x = np.array([[0.0, 0.0, 0.0], [3.0, 4.0, 0.0]])
+    v = np.array([[0.1, -0.2, 0.3], [0.4, 0.1, -0.5]])
+    m = np.array([1.5, 2.5])
+    ctx.upload(m, x, v, np.zeros(2))
+    acc, adot, nnb, _ = ctx.query([0, 1])
+    np.testing.assert_allclose(acc[0], [0.06, 0.08, 0.0], rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(adot[0], [-0.00912, -0.01416, -0.016], rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(acc[1], [-0.036, -0.048, 0.0], rtol=1e-6, atol=1e-9)
+    assert list(nnb) == [0, 0]
+
+
+def test_plummer_1024_full(ctx):
+    say("case: plummer_equal N=1024 full")
+    js = workloads.plummer_equal(1024)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    t = np.arange(js.n, dtype=np.int32)
+    check(js, t, ctx.query(t), label="plummer1024")
+
+
+@pytest.mark.parametrize("size", [1, 3, 64, 1000, 8192, 65536])
+def test_plummer_kroupa_65536_blocks(ctx, size):
+    say(f"case: plummer_kroupa N=65536 block={size}")
+    js = workloads.plummer_kroupa(65536)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    rng = np.random.default_rng(size)
+    t = rng.permutation(js.n)[:size].astype(np.int32)
+    err, _ = check(js, t, ctx.query(t), label=f"kroupa65536/{size}")
+
+
+def test_fp64_kernel(ctx):
+    say("case: fp64 kernel plummer_kroupa N=65536 block=4096")
+    js = workloads.plummer_kroupa(65536)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    t = np.random.default_rng(5).permutation(js.n)[:4096].astype(np.int32)
+    ctx.set_precision(regf.PREC_FP64)
+    try:
+        check(js, t, ctx.query(t), band=1e-12, prec=regf.PREC_FP64, label="fp64")
+    finally:
+        ctx.set_precision(regf.PREC_SPLIT)
+
+
+def test_tuning_variants_agree(ctx):
+    say("case: tuning variants")
+    js = workloads.plummer_kroupa(65536)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    t = np.random.default_rng(9).permutation(js.n)[:3000].astype(np.int32)
+    base = None
+    try:
+        for ipl in (1, 2, 4):
+            for splits in (1, 5, 0):
+                ctx.set_tuning(ipl, splits)
+                r = ctx.query(t)
+                say(f"  ipl={ipl} splits={splits} plan={ctx.plan()}")
+                if base is None:
+                    base = r
+                    continue
+                # same FP32 partial groups (folds are aligned to absolute j),
+                # so only the FP64 summation order can differ
+                np.testing.assert_allclose(r[0], base[0], rtol=1e-10, atol=0)
+                np.testing.assert_allclose(r[1], base[1], rtol=1e-10, atol=0)
+                assert np.array_equal(r[2], base[2])
+                assert np.array_equal(r[3], base[3])
+    finally:
+        ctx.set_tuning(0, 0)
+
+
+def test_deterministic(ctx):
+    say("case: determinism")
+    js = workloads.plummer_kroupa(65536)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    t = np.random.default_rng(3).permutation(js.n)[:20000].astype(np.int32)
+    r1 = ctx.query(t)
+    r2 = ctx.query(t)
+    for a, b in zip(r1, r2):
+        assert np.array_equal(a, b)
+
+
+def test_overflow(ctx):
+    say("case: overflow")
+    js = workloads.plummer_equal(1024)
+    small = regf.Context(device=0, cap=16)
+    try:
+        small.upload(js.m, js.x, js.v, js.r2)
+        t = np.arange(0, js.n, 7, dtype=np.int32)
+        acc, adot, nnb, lst = small.query(t)
+        assert small.last_status == regf.ERR_NEIGHBOR_OVERFLOW
+        ra, rj, rn, rl = oracle.regf(js.x, js.v, js.m, js.r2, t, 16)
+        assert np.array_equal(nnb, rn)
+        assert np.all(nnb < 0)
+        assert np.array_equal(lst, rl)
+    finally:
+        small.close()
+
+
+def test_edge_cases(ctx):
+    say("case: edge cases")
+    # single particle: zero force, no neighbours
+    ctx.upload([1.0], [[0.1, 0.2, 0.3]], [[0.0, 0.0, 0.0]], [0.5])
+    acc, adot, nnb, _ = ctx.query([0])
+    assert np.all(acc == 0) and np.all(adot == 0) and nnb[0] == 0
+    # coincident particles become neighbours, no NaN
+    x = np.array([[1.0, 2.0, 3.0], [1.0, 2.0, 3.0], [4.0, 2.0, 3.0]])
+    v = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    m = np.array([1.0, 2.0, 3.0])
+    ctx.upload(m, x, v, np.zeros(3))
+    acc, adot, nnb, lst = ctx.query([0, 1, 1, 0])   # duplicates allowed
+    assert np.all(np.isfinite(acc)) and np.all(np.isfinite(adot))
+    assert list(nnb) == [1, 1, 1, 1]
+    assert list(lst[:, 0]) == [1, 0, 0, 1]
+    np.testing.assert_allclose(acc[0], [1.0 / 3.0, 0, 0], rtol=1e-6)
+    # r_nb = 0 -> empty lists; r_nb = inf -> all neighbours, zero force
+    rng = np.random.default_rng(1)
+    for n in (257, 1000, 1023):
+        x = rng.normal(size=(n, 3))
+        v = rng.normal(size=(n, 3))
+        m = rng.uniform(0.5, 1.5, n) / n
+        t = rng.permutation(n)[:100].astype(np.int32)
+        ctx.upload(m, x, v, np.zeros(n))
+        acc, adot, nnb, _ = ctx.query(t)
+        ra, rj, _, _ = oracle.regf(x, v, m, np.zeros(n), t, 1)
+        assert np.all(nnb == 0)
+        np.testing.assert_allclose(acc, ra, rtol=2e-5)
+        big = regf.Context(device=0, cap=n)
+        try:
+            big.upload(m, x, v, np.full(n, np.inf))
+            acc, adot, nnb, lst = big.query(t)
+            assert np.all(acc == 0) and np.all(adot == 0)
+            assert np.all(nnb == n - 1)
+            for q, i in enumerate(t):
+                assert list(lst[q, :n - 1]) == [j for j in range(n) if j != i]
+        finally:
+            big.close()
+
+
+def test_invalid_inputs(ctx):
+    say("case: invalid inputs")
+    js = workloads.plummer_equal(1024)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    with pytest.raises(regf.AbyssError) as e:
+        ctx.query([0, js.n])
+    assert e.value.code == regf.ERR_TARGET_RANGE
+    x = js.x.copy()
+    x[5, 1] = np.nan
+    with pytest.raises(regf.AbyssError) as e:
+        ctx.upload(js.m, x, js.v, js.r2)
+    assert e.value.code == regf.ERR_INVALID_INPUT
+    r2 = js.r2.copy()
+    r2[3] = -1.0
+    with pytest.raises(regf.AbyssError) as e:
+        ctx.upload(js.m, js.x, js.v, r2)
+    assert e.value.code == regf.ERR_INVALID_INPUT
+    # the previous j-set stays usable
+    acc, _, _, _ = ctx.query([1, 2])
+    assert np.all(np.isfinite(acc))
+
+
+def test_legacy_entry_points():
+    say("case: legacy entry points")
+    js = workloads.plummer_equal(1024)
+    regf.OpenDevice(0)
+    try:
+        assert regf.num_neighbor_max() == 1000
+        regf.SendToDevice(js.m, js.x, js.v, js.r2)
+        t = np.arange(0, js.n, 3, dtype=np.int32)
+        acc, adot, nnb, lst = regf.CalculateAccelerationOnDevice(t)
+        ra, rj, rn, rl = oracle.regf(js.x, js.v, js.m, js.r2, t, 1000)
+        res = compare_lists(nnb, lst, rn, rl, js.x, js.r2, t)
+        assert res["hard_mismatches"] == 0
+        np.testing.assert_allclose(acc, ra, rtol=1e-5, atol=1e-9)
+        regf.ProfileDevice(0)
+        # a second open only reports it
+        regf.OpenDevice(0)
+    finally:
+        regf.CloseDevice()
+    with pytest.raises(regf.AbyssError) as e:
+        regf.CalculateAccelerationOnDevice([0])
+    assert e.value.code == regf.ERR_NOT_OPEN
+
+
+def test_king_262144_full_properties(ctx):
+    """Full-size King W0=6 + hard binaries: split kernel vs the FP64 kernel on
+    every target, and both against the oracle on a sampled subset."""
+    say("case: king_w6_binaries N=262144 full")
+    js = workloads.king_binaries(262144)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    t = np.arange(js.n, dtype=np.int32)
+    split = ctx.query(t)
+    ctx.set_precision(regf.PREC_FP64)
+    try:
+        f64 = ctx.query(t)
+    finally:
+        ctx.set_precision(regf.PREC_SPLIT)
+    lists = compare_lists(split[2], split[3], f64[2], f64[3], js.x, js.r2, t)
+    err = force_errors(split[0], split[1], f64[0], f64[1])
+    say(f"[king262144 split vs fp64] a_rel={err['a_rel']} jdot_rel={err['jdot_rel']} lists={lists}")
+    assert lists["hard_mismatches"] == 0 and lists["not_ascending"] == 0
+    assert err["a_rel"]["p99"] <= TOL_REL_P99 and err["jdot_rel"]["p99"] <= TOL_REL_P99
+    # every binary member sees its companion as a neighbour
+    pairs = js.meta["pairs"]
+    for p, s in pairs[:200]:
+        assert s in set(split[3][p, :split[2][p]].tolist())
+    # oracle on a sample
+    sample = np.random.default_rng(4).permutation(js.n)[:1024].astype(np.int32)
+    sub = tuple(a[sample] for a in split)
+    check(js, sample, sub, label="king262144/sample")

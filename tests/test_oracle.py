@@ -1,0 +1,215 @@
+"""Pins the FP64 oracle against closed forms, calculus identities, invariants
+and an independent brute force.  Each check is chosen so that a plausible
+mistake in the restated formula (sign, power of r, the factor 3 in the jerk,
+the <= boundary, self-exclusion, list order) fails at least one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+
+
+def _rand_cluster(n, seed, spread=1.0):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(n, 3)) * spread
+    v = rng.normal(size=(n, 3)) * 0.3
+    m = rng.uniform(0.5, 2.0, n) / n
+    return x, v, m
+
+
+def test_two_body_closed_form():
+    # r = x1 - x0 = (3,4,0), |r| = 5; v = v1 - v0 = (0.3,0.3,-0.8), r.v = 2.1
+    x = np.array([[0.0, 0.0, 0.0], [3.0, 4.0, 0.0]])
+    v = np.array([[0.1, -0.2, 0.3], [0.4, 0.1, -0.5]])
+    m = np.array([1.5, 2.5])
+    acc, adot, nnb, _ = oracle.regf(x, v, m, np.zeros(2), [0, 1], cap=4)
+    # hand-evaluated: a0 = 2.5 (3,4,0)/125 ; jdot0 = 2.5[(v)/125 - 3*2.1 (3,4,0)/3125]
+    np.testing.assert_allclose(acc[0], [0.06, 0.08, 0.0], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(adot[0], [-0.00912, -0.01416, -0.016], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(acc[1], [-0.036, -0.048, 0.0], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(adot[1], [0.005472, 0.008496, 0.0096], rtol=0, atol=1e-15)
+    assert list(nnb) == [0, 0]
+
+
+def _potential(x, m, i, xi):
+    d = x - xi
+    r = np.sqrt(np.sum(d * d, axis=1))
+    r[i] = np.inf
+    return -np.sum(m / r)
+
+
+def test_acc_is_minus_grad_potential():
+    x, v, m = _rand_cluster(32, 3)
+    acc, _, _, _ = oracle.regf(x, v, m, np.zeros(32), np.arange(32), cap=32)
+    h = 1e-4
+    for i in range(0, 32, 5):
+        g = np.zeros(3)
+        for d in range(3):
+            e = np.zeros(3)
+            e[d] = 1.0
+            # 4th-order central difference
+            f = [_potential(x, m, i, x[i] + s * h * e) for s in (-2, -1, 1, 2)]
+            g[d] = (f[0] - 8 * f[1] + 8 * f[2] - f[3]) / (12 * h)
+        np.testing.assert_allclose(acc[i], -g, rtol=1e-7, atol=1e-9 * np.abs(acc[i]).max())
+
+
+def test_jerk_is_time_derivative_of_acc():
+    # Straight-line motion x_j(t) = x_j + v_j t: jdot_i = d a_i / dt at t = 0.
+    x, v, m = _rand_cluster(24, 5)
+    n = len(m)
+    t = np.arange(n)
+    _, adot, _, _ = oracle.regf(x, v, m, np.zeros(n), t, cap=n)
+    h = 1e-4
+    a = {}
+    for s in (-2, -1, 1, 2):
+        a[s] = oracle.regf(x + s * h * v, v, m, np.zeros(n), t, cap=n)[0]
+    fd = (a[-2] - 8 * a[-1] + 8 * a[1] - a[2]) / (12 * h)
+    np.testing.assert_allclose(adot, fd, rtol=1e-6, atol=1e-8 * np.abs(adot).max())
+
+
+def test_momentum_conservation_full_force():
+    x, v, m = _rand_cluster(200, 7)
+    n = len(m)
+    acc, adot, _, _ = oracle.regf(x, v, m, np.zeros(n), np.arange(n), cap=n)
+    pa = (m[:, None] * acc).sum(0)
+    pj = (m[:, None] * adot).sum(0)
+    assert np.all(np.abs(pa) < 1e-13 * np.sum(m[:, None] * np.abs(acc)))
+    assert np.all(np.abs(pj) < 1e-13 * np.sum(m[:, None] * np.abs(adot)))
+
+
+def test_homogeneity_scalings():
+    x, v, m = _rand_cluster(64, 11)
+    n = len(m)
+    r2 = np.full(n, 0.3)
+    t = np.arange(n)
+    a0, j0, c0, l0 = oracle.regf(x, v, m, r2, t, cap=n)
+    lam = 3.0
+    a1, j1, c1, l1 = oracle.regf(x, v, lam * m, r2, t, cap=n)
+    np.testing.assert_allclose(a1, lam * a0, rtol=1e-13)
+    np.testing.assert_allclose(j1, lam * j0, rtol=1e-13)
+    assert np.array_equal(c0, c1)
+    # x -> lam x (and r_nb^2 -> lam^2 r_nb^2): a -> a/lam^2, jdot -> jdot/lam^3
+    a2, j2, c2, l2 = oracle.regf(lam * x, v, m, lam * lam * r2, t, cap=n)
+    np.testing.assert_allclose(a2, a0 / lam ** 2, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(j2, j0 / lam ** 3, rtol=1e-12, atol=1e-15)
+    assert np.array_equal(c0, c2) and np.array_equal(l0, l2)
+    # v -> lam v: a unchanged, jdot -> lam jdot
+    a3, j3, _, _ = oracle.regf(x, lam * v, m, r2, t, cap=n)
+    np.testing.assert_allclose(a3, a0, rtol=0, atol=0)
+    np.testing.assert_allclose(j3, lam * j0, rtol=1e-13)
+
+
+def test_radius_limits():
+    x, v, m = _rand_cluster(50, 13)
+    n = len(m)
+    t = np.arange(n)
+    a0, j0, c0, _ = oracle.regf(x, v, m, np.zeros(n), t, cap=n)
+    assert np.all(c0 == 0)
+    ainf, jinf, cinf, linf = oracle.regf(x, v, m, np.full(n, np.inf), t, cap=n)
+    assert np.all(ainf == 0) and np.all(jinf == 0)
+    assert np.all(cinf == n - 1)
+    for k in range(n):
+        assert list(linf[k, :n - 1]) == [j for j in range(n) if j != k]
+    # a_reg(r_nb) + sum over the listed neighbours of the pair force = full force
+    r2 = np.full(n, 0.8)
+    ar, jr, cr, lr = oracle.regf(x, v, m, r2, t, cap=n)
+    for k in range(n):
+        nb = lr[k, :cr[k]]
+        d = x[nb] - x[k]
+        r = np.sqrt(np.sum(d * d, axis=1))
+        pair = (m[nb] / r ** 3)[:, None] * d
+        np.testing.assert_allclose(ar[k] + pair.sum(0), a0[k], rtol=1e-12, atol=1e-14)
+
+
+def test_boundary_is_inclusive_and_self_excluded_by_index():
+    # r^2 = 0.75^2 + 1^2 = 1.5625 exactly representable
+    x = np.array([[0.0, 0.0, 0.0], [0.75, 1.0, 0.0], [5.0, 0.0, 0.0]])
+    v = np.zeros((3, 3))
+    m = np.array([1.0, 1.0, 1.0])
+    r2 = np.array([1.5625, 0.0, 0.0])
+    _, _, c, lst = oracle.regf(x, v, m, r2, [0], cap=3)
+    assert c[0] == 1 and lst[0, 0] == 1          # r^2 == r_nb^2 -> neighbour
+    r2[0] = np.nextafter(1.5625, 0.0)
+    a, _, c, _ = oracle.regf(x, v, m, r2, [0], cap=3)
+    assert c[0] == 0 and a[0, 0] > 0            # r^2 > r_nb^2 -> regular
+
+
+def test_coincident_particle_is_neighbour_without_nan():
+    x = np.array([[1.0, 2.0, 3.0], [1.0, 2.0, 3.0], [4.0, 2.0, 3.0]])
+    v = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    m = np.array([1.0, 2.0, 3.0])
+    a, j, c, lst = oracle.regf(x, v, m, np.zeros(3), [0, 1], cap=3)
+    assert np.all(np.isfinite(a)) and np.all(np.isfinite(j))
+    assert list(c) == [1, 1] and lst[0, 0] == 1 and lst[1, 0] == 0
+    # only particle 2 (at distance 3 along x) contributes
+    np.testing.assert_allclose(a[0], [3.0 / 9.0, 0, 0], rtol=1e-15)
+
+
+def test_overflow_convention():
+    x, v, m = _rand_cluster(40, 17)
+    n = len(m)
+    _, _, cfull, lfull = oracle.regf(x, v, m, np.full(n, np.inf), [3], cap=n)
+    _, _, c, lst = oracle.regf(x, v, m, np.full(n, np.inf), [3], cap=5)
+    assert cfull[0] == n - 1 and c[0] == -(n - 1)
+    assert list(lst[0]) == list(lfull[0, :5])
+
+
+def test_matches_independent_numpy_brute_force():
+    x, v, m = _rand_cluster(150, 19)
+    n = len(m)
+    rng = np.random.default_rng(23)
+    r2 = rng.uniform(0.05, 2.0, n)
+    t = rng.permutation(n)[:97]
+    acc, adot, c, lst = oracle.regf(x, v, m, r2, t, cap=n)
+    D = x[None, :, :] - x[t][:, None, :]                # [nt, n, 3]
+    V = v[None, :, :] - v[t][:, None, :]
+    R2 = np.sum(D * D, axis=2)
+    far = R2 > r2[t][:, None]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = np.where(far, 1.0 / np.sqrt(R2), 0.0)
+    w = m[None, :] * inv ** 3
+    RV = np.sum(D * V, axis=2)
+    a_np = np.sum(w[:, :, None] * D, axis=1)
+    j_np = np.sum(w[:, :, None] * (V - 3.0 * (RV * inv * inv)[:, :, None] * D), axis=1)
+    np.testing.assert_allclose(acc, a_np, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(adot, j_np, rtol=1e-11, atol=1e-13)
+    for q, i in enumerate(t):
+        want = [j for j in range(n) if j != i and not far[q, j]]
+        assert c[q] == len(want)
+        assert list(lst[q, :c[q]]) == want
+
+
+def test_plummer_radial_acceleration_matches_analytic_profile():
+    from abyss_amd import ic
+    x, v, m = ic.plummer(4096, seed=2)
+    n = len(m)
+    t = np.arange(0, n, 4)
+    acc, _, _, _ = oracle.regf(x, v, m, np.zeros(n), t, cap=1, nthreads=0)
+    a_pl = 3 * math.pi / 16
+    r = np.linalg.norm(x[t], axis=1)
+    ar = np.sum(acc * x[t], axis=1) / r          # radial component
+    model = -r / (r * r + a_pl * a_pl) ** 1.5    # G = M = 1
+    sel = (r > 0.3) & (r < 2.0)
+    ratio = ar[sel] / model[sel]
+    assert abs(np.median(ratio) - 1.0) < 0.05
+
+
+def test_band_pairs():
+    x = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    r2 = np.array([1.0 * (1 + 5e-7), 0.0, 0.0])
+    bc, bl = oracle.band_pairs(x, r2, [0], rel=1e-6)
+    assert bc[0] == 2 and list(bl[0, :2]) == [1, 2]
+    r2[0] = 1.0 * (1 + 3e-6)
+    bc, _ = oracle.band_pairs(x, r2, [0], rel=1e-6)
+    assert bc[0] == 0
+
+
+def test_predictor_matches_taylor_series():
+    xp, vp = oracle.predict2([[1.0, 0, 0]], [[2.0, 0, 0]], [[3.0, 0, 0]], [[4.0, 0, 0]], 0.5)
+    # x + v dt + a dt^2/2 + adot dt^3/6 ; v + a dt + adot dt^2/2
+    assert xp[0, 0] == pytest.approx(1 + 1 + 0.375 + 4 * 0.125 / 6, rel=1e-15)
+    assert vp[0, 0] == pytest.approx(2 + 1.5 + 0.5, rel=1e-15)
+    xp, vp = oracle.predict2([[1.0, 2, 3]], [[2.0, 0, 0]], [[3.0, 0, 0]], [[4.0, 0, 0]], 0.0)
+    assert list(xp[0]) == [1.0, 2, 3] and list(vp[0]) == [2.0, 0, 0]
