@@ -32,7 +32,11 @@ struct Plan {
     int jspan = 0;   // j per split (multiple of the tile)
     int grid_x = 0;  // target blocks
     int tile = 256;  // j per LDS tile
+    int unroll = 2;  // j per unrolled inner-loop step
 };
+
+// Whether a (precision, targets-per-lane, unroll) kernel variant exists.
+bool variant_supported(int prec, int ipl, int unroll);
 
 constexpr int kBlock = 256;   // threads per workgroup (4 wave64)
 constexpr int kTile = 256;    // j per LDS tile

@@ -49,11 +49,12 @@ struct Targets {
     int *lst[IPL];            // this (split, target)'s list row
 };
 
-// Append j to target k's list if it is not the target itself.
-template <int IPL, bool F64>
+// Append j to target k's list.  SELF: the tile may contain the target
+// itself, which must be skipped (self-exclusion is by index).
+template <int IPL, bool F64, bool SELF>
 __device__ __forceinline__ void append(Targets<IPL, F64> &T, int k, int jg, int cap)
 {
-    if (jg != T.idx[k]) {
+    if (!SELF || jg != T.idx[k]) {
         if (T.cnt[k] < cap) T.lst[k][T.cnt[k]] = jg;
         T.cnt[k] += 1;
     }
@@ -61,7 +62,7 @@ __device__ __forceinline__ void append(Targets<IPL, F64> &T, int k, int jg, int 
 
 // Evaluate j records [jb, je) of LDS buffer (pos, mass) against all targets.
 // jg0 is the global index of record 0 of the buffer.
-template <int IPL, bool F64, bool FULL>
+template <int IPL, int U, bool F64, bool FULL, bool SELF>
 __device__ __forceinline__ void run_tile(Targets<IPL, F64> &T, const double2 *__restrict__ pos,
                                          const typename MassT<F64>::type *__restrict__ mass,
                                          int count, int jg0, int cap)
@@ -74,7 +75,7 @@ __device__ __forceinline__ void run_tile(Targets<IPL, F64> &T, const double2 *__
 #pragma unroll
                 for (int c = 0; c < 6; ++c) pa[k][c] = 0.0f;
             const int fe = FULL ? kFold : min(kFold, count - f);
-#pragma unroll 4
+#pragma unroll U
             for (int jj = 0; jj < fe; ++jj) {
                 const int r = f + jj;
                 const double2 p0 = pos[3 * r + 0];   // x, y
@@ -90,10 +91,13 @@ __device__ __forceinline__ void run_tile(Targets<IPL, F64> &T, const double2 *__
                     const float dvy = (float)(p2.x - T.vy[k]);
                     const float dvz = (float)(p2.y - T.vz[k]);
                     const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-                    const bool far = r2 > T.rnbf[k];
                     float rinv = __builtin_amdgcn_rsqf(r2);
-                    rinv = far ? rinv : 0.0f;
-                    if (!far) append(T, k, jg0 + r, cap);
+                    // Neighbour (or self, or coincident j): rare branch that
+                    // records j and zeroes rinv, so the pair adds exactly 0.
+                    if (!(r2 > T.rnbf[k])) {
+                        rinv = 0.0f;
+                        append<IPL, F64, SELF>(T, k, jg0 + r, cap);
+                    }
                     const float rinv2 = rinv * rinv;
                     const float mr3 = mj * rinv * rinv2;
                     const float rv = fmaf(dz, dvz, fmaf(dy, dvy, dx * dvx));
@@ -113,7 +117,7 @@ __device__ __forceinline__ void run_tile(Targets<IPL, F64> &T, const double2 *__
         }
     } else {
         const int c_end = FULL ? kTile : count;
-#pragma unroll 2
+#pragma unroll U
         for (int r = 0; r < c_end; ++r) {
             const double2 p0 = pos[3 * r + 0];
             const double2 p1 = pos[3 * r + 1];
@@ -128,9 +132,11 @@ __device__ __forceinline__ void run_tile(Targets<IPL, F64> &T, const double2 *__
                 const double dvy = p2.x - T.vy[k];
                 const double dvz = p2.y - T.vz[k];
                 const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-                const bool far = r2 > T.rnbd[k];
-                double rinv = far ? rsqrt(r2) : 0.0;
-                if (!far) append(T, k, jg0 + r, cap);
+                double rinv = rsqrt(r2);
+                if (!(r2 > T.rnbd[k])) {
+                    rinv = 0.0;
+                    append<IPL, F64, SELF>(T, k, jg0 + r, cap);
+                }
                 const double rinv2 = rinv * rinv;
                 const double mr3 = mj * rinv * rinv2;
                 const double rv = fma(dz, dvz, fma(dy, dvy, dx * dvx));
@@ -146,7 +152,7 @@ __device__ __forceinline__ void run_tile(Targets<IPL, F64> &T, const double2 *__
     }
 }
 
-template <int IPL, bool F64>
+template <int IPL, int U, bool F64>
 __global__ __launch_bounds__(kBlock) void regf_kernel(JState js, const int *__restrict__ tgt,
                                                       int nt, int jspan, QueryOut out, int cap,
                                                       int final_counts)
@@ -215,10 +221,18 @@ __global__ __launch_bounds__(kBlock) void regf_kernel(JState js, const int *__re
         if (t + 1 < ntile) gload(t + 1);
         const int jt = j0 + t * kTile;
         const int count = min(kTile, j1 - jt);
-        if (count == kTile)
-            run_tile<IPL, F64, true>(T, s_pos[b], s_m[b], count, jt, cap);
-        else
-            run_tile<IPL, F64, false>(T, s_pos[b], s_m[b], count, jt, cap);
+        // Wave vote: does this tile hold the target of any lane?  Only then
+        // does the tile need the per-neighbour self test.
+        bool mine = false;
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) mine |= (unsigned)(T.idx[k] - jt) < (unsigned)count;
+        if (__any(mine)) {
+            run_tile<IPL, U, F64, false, true>(T, s_pos[b], s_m[b], count, jt, cap);
+        } else if (count == kTile) {
+            run_tile<IPL, U, F64, true, false>(T, s_pos[b], s_m[b], count, jt, cap);
+        } else {
+            run_tile<IPL, U, F64, false, false>(T, s_pos[b], s_m[b], count, jt, cap);
+        }
         if (t + 1 < ntile) lstore(b ^ 1);
         __syncthreads();
     }
@@ -302,32 +316,39 @@ __global__ __launch_bounds__(1024) void count_stats_kernel(const int *__restrict
     }
 }
 
-template <int IPL, bool F64>
+template <int IPL, int U, bool F64>
 hipError_t launch_t(const JState &js, const int *tgt, int nt, const Plan &p, const QueryOut &out,
                     int cap, hipStream_t st)
 {
     dim3 grid(p.grid_x, p.splits);
-    hipLaunchKernelGGL((regf_kernel<IPL, F64>), grid, dim3(kBlock), 0, st, js, tgt, nt, p.jspan,
-                       out, cap, p.splits == 1 ? 1 : 0);
+    hipLaunchKernelGGL((regf_kernel<IPL, U, F64>), grid, dim3(kBlock), 0, st, js, tgt, nt,
+                       p.jspan, out, cap, p.splits == 1 ? 1 : 0);
     return hipGetLastError();
 }
 
 }  // namespace
 
+bool variant_supported(int prec, int ipl, int unroll)
+{
+    if (prec == 1) return (ipl == 1 || ipl == 2) && unroll == 2;
+    return (ipl == 1 || ipl == 2 || ipl == 4) && (unroll == 2 || unroll == 4);
+}
+
 hipError_t launch_regf(const JState &js, const int *tgt, int nt, const Plan &p,
                        const QueryOut &out, int cap, int prec, hipStream_t st)
 {
+    if (!variant_supported(prec, p.ipl, p.unroll)) return hipErrorInvalidValue;
     if (prec == 1) {
-        switch (p.ipl) {
-            case 1: return launch_t<1, true>(js, tgt, nt, p, out, cap, st);
-            case 2: return launch_t<2, true>(js, tgt, nt, p, out, cap, st);
-            default: return hipErrorInvalidValue;
-        }
+        if (p.ipl == 1) return launch_t<1, 2, true>(js, tgt, nt, p, out, cap, st);
+        return launch_t<2, 2, true>(js, tgt, nt, p, out, cap, st);
     }
-    switch (p.ipl) {
-        case 1: return launch_t<1, false>(js, tgt, nt, p, out, cap, st);
-        case 2: return launch_t<2, false>(js, tgt, nt, p, out, cap, st);
-        case 4: return launch_t<4, false>(js, tgt, nt, p, out, cap, st);
+    switch (p.ipl * 10 + p.unroll) {
+        case 12: return launch_t<1, 2, false>(js, tgt, nt, p, out, cap, st);
+        case 14: return launch_t<1, 4, false>(js, tgt, nt, p, out, cap, st);
+        case 22: return launch_t<2, 2, false>(js, tgt, nt, p, out, cap, st);
+        case 24: return launch_t<2, 4, false>(js, tgt, nt, p, out, cap, st);
+        case 42: return launch_t<4, 2, false>(js, tgt, nt, p, out, cap, st);
+        case 44: return launch_t<4, 4, false>(js, tgt, nt, p, out, cap, st);
         default: return hipErrorInvalidValue;
     }
 }

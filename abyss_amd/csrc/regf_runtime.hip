@@ -29,7 +29,7 @@ struct abyss_ctx_s {
     int cu_count = 0;
     int cap = ABYSS_NUM_NEIGHBOR_MAX;
     int precision = ABYSS_PREC_SPLIT;
-    int force_ipl = 0, force_splits = 0;
+    int force_ipl = 0, force_splits = 0, force_unroll = 0;
     unsigned long long scratch_limit = 1ull << 30;
 
     // j-set: device SoA [x|y|z|vx|vy|vz|m|rnb2] (stride n) + FP32 mass
@@ -157,8 +157,13 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
         // are enough targets to give every CU two workgroups at ipl = 2.
         ipl = (nt >= abyss::kBlock * 2 * cu * 2) ? 2 : 1;
     }
-    if (c->precision == ABYSS_PREC_FP64 && ipl > 2) ipl = 2;
+    int unroll = c->force_unroll ? c->force_unroll : 2;
+    if (c->precision == ABYSS_PREC_FP64) {
+        ipl = std::min(ipl, 2);
+        unroll = 2;
+    }
     p.ipl = ipl;
+    p.unroll = unroll;
     p.grid_x = (nt + abyss::kBlock * ipl - 1) / (abyss::kBlock * ipl);
     int S = c->force_splits;
     if (S == 0) {
@@ -319,14 +324,17 @@ int abyss_ctx_set_precision(abyss_ctx_t c, int mode)
     return ABYSS_OK;
 }
 
-int abyss_ctx_set_tuning(abyss_ctx_t c, int ipl, int splits)
+int abyss_ctx_set_tuning(abyss_ctx_t c, int ipl, int splits, int unroll)
 {
     if (!c) return ABYSS_ERR_INVALID_ARG;
     if (ipl != 0 && ipl != 1 && ipl != 2 && ipl != 4)
         return set_detail(c, ABYSS_ERR_INVALID_ARG, "i_per_lane must be 0, 1, 2 or 4");
+    if (unroll != 0 && unroll != 2 && unroll != 4)
+        return set_detail(c, ABYSS_ERR_INVALID_ARG, "j_unroll must be 0, 2 or 4");
     if (splits < 0) return set_detail(c, ABYSS_ERR_INVALID_ARG, "j_splits must be >= 0");
     c->force_ipl = ipl;
     c->force_splits = splits;
+    c->force_unroll = unroll;
     c->plan_valid = false;
     return ABYSS_OK;
 }
@@ -429,7 +437,7 @@ int abyss_ctx_set_targets(abyss_ctx_t c, int nt, const int *targets)
     return ABYSS_OK;
 }
 
-int abyss_ctx_plan(abyss_ctx_t c, int *ipl, int *splits, int *grid_x, int *tile)
+int abyss_ctx_plan(abyss_ctx_t c, int *ipl, int *splits, int *grid_x, int *tile, int *unroll)
 {
     if (!c) return ABYSS_ERR_INVALID_ARG;
     if (!c->plan_valid) return set_detail(c, ABYSS_ERR_INVALID_ARG, "no targets set");
@@ -437,6 +445,7 @@ int abyss_ctx_plan(abyss_ctx_t c, int *ipl, int *splits, int *grid_x, int *tile)
     if (splits) *splits = c->plan.splits;
     if (grid_x) *grid_x = c->plan.grid_x;
     if (tile) *tile = c->plan.tile;
+    if (unroll) *unroll = c->plan.unroll;
     return ABYSS_OK;
 }
 

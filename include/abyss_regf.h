@@ -133,8 +133,9 @@ typedef struct abyss_ctx_s *abyss_ctx_t;
 int abyss_ctx_create(int device, int cap, abyss_ctx_t *out);
 int abyss_ctx_destroy(abyss_ctx_t ctx);
 int abyss_ctx_set_precision(abyss_ctx_t ctx, int mode);
-/* Forces i-per-lane (1, 2 or 4) and the j-split count; 0 = automatic. */
-int abyss_ctx_set_tuning(abyss_ctx_t ctx, int i_per_lane, int j_splits);
+/* Forces targets per lane (1, 2 or 4), the j-split count (>= 1) and the
+ * inner-loop j unroll (2 or 4); 0 = automatic for each. */
+int abyss_ctx_set_tuning(abyss_ctx_t ctx, int i_per_lane, int j_splits, int j_unroll);
 /* Upper bound on the split-j scratch (bytes; default 1 GiB). */
 int abyss_ctx_set_scratch_limit(abyss_ctx_t ctx, unsigned long long bytes);
 
@@ -155,9 +156,10 @@ int abyss_ctx_fetch(abyss_ctx_t ctx, double *acc, double *adot, int *nnb,
                     int *list);
 /* hipStream_t of the context (for event timing by the caller). */
 void *abyss_ctx_stream(abyss_ctx_t ctx);
-/* Plan chosen for the current targets: i per lane, j splits, grid x, tile. */
+/* Plan chosen for the current targets: targets per lane, j splits, target
+ * blocks, j per LDS tile, j unroll. */
 int abyss_ctx_plan(abyss_ctx_t ctx, int *i_per_lane, int *j_splits,
-                   int *grid_x, int *tile);
+                   int *grid_x, int *tile, int *j_unroll);
 /* Detail string of the context's last error. */
 const char *abyss_ctx_error_detail(abyss_ctx_t ctx);
 

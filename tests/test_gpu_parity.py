@@ -161,11 +161,11 @@ def test_edge_cases(ctx):
     acc, adot, nnb, _ = ctx.query([0])
     assert np.all(acc == 0) and np.all(adot == 0) and nnb[0] == 0
     # coincident particles become neighbours, no NaN
-    x = np.array([[1.0, 2.0, 3.0], [1.0, 2.0, 3.0], [4.0, 2.0, 3.0]])
-    v = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
-    m = np.array([1.0, 2.0, 3.0])
-    ctx.upload(m, x, v, np.zeros(3))
-    acc, adot, nnb, lst = ctx.query([0, 1, 1, 0])   # duplicates allowed
+    x = np.array([[1.0, 2.0, 3.0], [1.0, 2.0, 3.0], [4.0, 2.0, 3.0], [9.0, 9.0, 9.0]])
+    v = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 0.0]])
+    m = np.array([1.0, 2.0, 3.0, 0.0])
+    ctx.upload(m, x, v, np.zeros(4))
+    acc, adot, nnb, lst = ctx.query([0, 1, 1, 0])   # duplicates allowed (NumTarget <= N)
     assert np.all(np.isfinite(acc)) and np.all(np.isfinite(adot))
     assert list(nnb) == [1, 1, 1, 1]
     assert list(lst[:, 0]) == [1, 0, 0, 1]
