@@ -33,14 +33,19 @@ struct Plan {
     int grid_x = 0;  // target blocks
     int tile = 256;  // j per LDS tile
     int unroll = 2;  // j per unrolled inner-loop step
+    int jl = 1;      // lanes per target (1: one target per lane; 4/16/64)
 };
 
-// Whether a (precision, targets-per-lane, unroll) kernel variant exists.
-bool variant_supported(int prec, int ipl, int unroll);
+// Whether a (precision, targets-per-lane, unroll, lanes-per-target) kernel
+// variant exists.
+bool variant_supported(int prec, int ipl, int unroll, int jl);
+// Targets handled by one workgroup under plan p.
+int targets_per_block(const Plan &p);
 
 constexpr int kBlock = 256;   // threads per workgroup (4 wave64)
 constexpr int kTile = 256;    // j per LDS tile
 constexpr int kFold = 64;     // j per FP32 partial before the FP64 fold
+constexpr int kMaxSplits = 2048;  // bound of the finalize kernel's LDS prefix
 
 // Launch the pairwise kernel for nt targets against js (all j), split into
 // plan.splits contiguous j-ranges.  prec: 0 split precision, 1 FP64.

@@ -38,6 +38,34 @@ struct MassT { using type = float; };
 template <>
 struct MassT<true> { using type = double; };
 
+// Loads one j record per thread from the SoA j-set (coalesced) and stores
+// it into an LDS tile as 3 x double2 {x,y} {z,vx} {vy,vz} plus the mass.
+// Records past the end of the j-range are zero-filled so that kernels which
+// read whole tiles never see stale LDS contents.
+template <typename MT>
+struct TileLoader {
+    double rx = 0, ry = 0, rz = 0, rvx = 0, rvy = 0, rvz = 0;
+    MT rm = 0;
+    __device__ __forceinline__ void load(const JState &js, int j, int j1)
+    {
+        if (j < j1) {
+            rx = js.x[j]; ry = js.y[j]; rz = js.z[j];
+            rvx = js.vx[j]; rvy = js.vy[j]; rvz = js.vz[j];
+            if constexpr (sizeof(MT) == 8) rm = js.m[j]; else rm = js.m32[j];
+        } else {
+            rx = ry = rz = rvx = rvy = rvz = 0.0;
+            rm = 0;
+        }
+    }
+    __device__ __forceinline__ void store(double2 *pos, MT *mass, int r) const
+    {
+        pos[3 * r + 0] = make_double2(rx, ry);
+        pos[3 * r + 1] = make_double2(rz, rvx);
+        pos[3 * r + 2] = make_double2(rvy, rvz);
+        mass[r] = rm;
+    }
+};
+
 template <int IPL, bool F64>
 struct Targets {
     double x[IPL], y[IPL], z[IPL], vx[IPL], vy[IPL], vz[IPL];
@@ -193,32 +221,17 @@ __global__ __launch_bounds__(kBlock) void regf_kernel(JState js, const int *__re
     }
 
     const int ntile = (j1 - j0 + kTile - 1) / kTile;
-    double rx = 0, ry = 0, rz = 0, rvx = 0, rvy = 0, rvz = 0;
-    MT rm = 0;
-    auto gload = [&](int t) {
-        const int j = j0 + t * kTile + tid;
-        if (j < j1) {
-            rx = js.x[j]; ry = js.y[j]; rz = js.z[j];
-            rvx = js.vx[j]; rvy = js.vy[j]; rvz = js.vz[j];
-            if constexpr (F64) rm = js.m[j]; else rm = js.m32[j];
-        }
-    };
-    auto lstore = [&](int b) {
-        s_pos[b][3 * tid + 0] = make_double2(rx, ry);
-        s_pos[b][3 * tid + 1] = make_double2(rz, rvx);
-        s_pos[b][3 * tid + 2] = make_double2(rvy, rvz);
-        s_m[b][tid] = rm;
-    };
+    TileLoader<MT> ld;
     static_assert(kTile == kBlock, "one j per thread per tile load");
 
     if (ntile > 0) {
-        gload(0);
-        lstore(0);
+        ld.load(js, j0 + tid, j1);
+        ld.store(s_pos[0], s_m[0], tid);
     }
     __syncthreads();
     for (int t = 0; t < ntile; ++t) {
         const int b = t & 1;
-        if (t + 1 < ntile) gload(t + 1);
+        if (t + 1 < ntile) ld.load(js, j0 + (t + 1) * kTile + tid, j1);
         const int jt = j0 + t * kTile;
         const int count = min(kTile, j1 - jt);
         // Wave vote: does this tile hold the target of any lane?  Only then
@@ -233,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void regf_kernel(JState js, const int *__re
         } else {
             run_tile<IPL, U, F64, false, false>(T, s_pos[b], s_m[b], count, jt, cap);
         }
-        if (t + 1 < ntile) lstore(b ^ 1);
+        if (t + 1 < ntile) ld.store(s_pos[b ^ 1], s_m[b ^ 1], tid);
         __syncthreads();
     }
 
@@ -254,36 +267,208 @@ __global__ __launch_bounds__(kBlock) void regf_kernel(JState js, const int *__re
     }
 }
 
-// One wave per target: sums the S partials in split order (fixed order, so
-// bitwise reproducible) and concatenates the sub-lists in split order, which
-// is ascending j because splits are contiguous ascending j-ranges.
+// Small-block variant: JL lanes share one target (JL = 4, 16 or 64), so a
+// wave serves 64/JL targets and a few active particles still fill the chip.
+// In a tile, step t hands the group's lanes the JL consecutive records
+// t*JL + sub (sub = lane mod JL).  Neighbours of a step are compacted with a
+// wave ballot: a lane's slot is the group's running count plus the number
+// of neighbour lanes below it in the group, which keeps every list in
+// ascending j.  Each lane keeps FP32 partials folded into FP64 every kFold
+// of its own pairs; at the end the JL FP64 sums of a group are combined by a
+// fixed xor-butterfly (bitwise reproducible).
+template <int JL>
+__global__ __launch_bounds__(kBlock) void regf_kernel_jl(JState js, const int *__restrict__ tgt,
+                                                         int nt, int jspan, QueryOut out, int cap,
+                                                         int final_counts)
+{
+    constexpr int TW = 64 / JL;            // targets per wave
+    constexpr int STEPS = kTile / JL;      // steps per full tile
+    __shared__ double2 s_pos[2][kTile * 3];
+    __shared__ float s_m[2][kTile];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int sub = lane % JL;
+    const int g = lane / JL;
+    const int s = blockIdx.y;
+    const int j0 = s * jspan;
+    const int j1 = min(js.n, j0 + jspan);
+    const int q = blockIdx.x * (4 * TW) + (tid >> 6) * TW + g;
+    const unsigned long long gmask =
+        (JL == 64) ? ~0ull : (((1ull << JL) - 1ull) << (g * JL));
+    const unsigned long long below = (1ull << lane) - 1ull;   // lanes < this lane
+
+    double xi = 0, yi = 0, zi = 0, vxi = 0, vyi = 0, vzi = 0;
+    float rnbf = -1.0f;   // inactive target: every j regular, nothing listed
+    int idx = -1;
+    int *lst = nullptr;
+    if (q < nt) {
+        idx = tgt[q];
+        xi = js.x[idx]; yi = js.y[idx]; zi = js.z[idx];
+        vxi = js.vx[idx]; vyi = js.vy[idx]; vzi = js.vz[idx];
+        rnbf = (float)js.rnb2[idx];
+        lst = out.list + ((size_t)s * nt + q) * (size_t)cap;
+    }
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    float pa[6] = {0, 0, 0, 0, 0, 0};
+    int since_fold = 0;
+    int cnt = 0;
+
+    const int ntile = (j1 - j0 + kTile - 1) / kTile;
+    TileLoader<float> ld;
+    if (ntile > 0) {
+        ld.load(js, j0 + tid, j1);
+        ld.store(s_pos[0], s_m[0], tid);
+    }
+    __syncthreads();
+    for (int t = 0; t < ntile; ++t) {
+        const int b = t & 1;
+        if (t + 1 < ntile) ld.load(js, j0 + (t + 1) * kTile + tid, j1);
+        const int jt = j0 + t * kTile;
+        const int count = min(kTile, j1 - jt);
+        const bool self_tile = __any((unsigned)(idx - jt) < (unsigned)count);
+        const bool full = count == kTile;
+        const int steps = full ? STEPS : (count + JL - 1) / JL;
+        const double2 *pos = s_pos[b];
+        const float *mass = s_m[b];
+        for (int st = 0; st < steps; ++st) {
+            const int r = st * JL + sub;
+            const double2 p0 = pos[3 * r + 0];
+            const double2 p1 = pos[3 * r + 1];
+            const double2 p2 = pos[3 * r + 2];
+            const float mj = mass[r];
+            const float dx = (float)(p0.x - xi);
+            const float dy = (float)(p0.y - yi);
+            const float dz = (float)(p1.x - zi);
+            const float dvx = (float)(p1.y - vxi);
+            const float dvy = (float)(p2.x - vyi);
+            const float dvz = (float)(p2.y - vzi);
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            float rinv = __builtin_amdgcn_rsqf(r2);
+            const bool near = !(r2 > rnbf);
+            if (near) rinv = 0.0f;
+            const int jg = jt + r;
+            bool listed = near;
+            if (!full) listed = listed && r < count;          // zero-filled records
+            if (self_tile) listed = listed && jg != idx;      // self-exclusion by index
+            const unsigned long long bal = __ballot(listed);
+            if (bal) {
+                const unsigned long long gb = bal & gmask;
+                if (listed) {
+                    const int p = cnt + __popcll(gb & below);
+                    if (p < cap) lst[p] = jg;
+                }
+                cnt += __popcll(gb);
+            }
+            const float rinv2 = rinv * rinv;
+            const float mr3 = mj * rinv * rinv2;
+            const float rv = fmaf(dz, dvz, fmaf(dy, dvy, dx * dvx));
+            const float al = -3.0f * rv * rinv2;
+            pa[0] = fmaf(mr3, dx, pa[0]);
+            pa[1] = fmaf(mr3, dy, pa[1]);
+            pa[2] = fmaf(mr3, dz, pa[2]);
+            pa[3] = fmaf(mr3, fmaf(al, dx, dvx), pa[3]);
+            pa[4] = fmaf(mr3, fmaf(al, dy, dvy), pa[4]);
+            pa[5] = fmaf(mr3, fmaf(al, dz, dvz), pa[5]);
+            if (++since_fold == kFold) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    acc[c] += (double)pa[c];
+                    pa[c] = 0.0f;
+                }
+                since_fold = 0;
+            }
+        }
+        if (t + 1 < ntile) ld.store(s_pos[b ^ 1], s_m[b ^ 1], tid);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[c] += (double)pa[c];
+#pragma unroll
+    for (int d = JL / 2; d >= 1; d >>= 1)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[c] += __shfl_xor(acc[c], d, 64);
+    if (q < nt && sub == 0) {
+        const size_t o = (size_t)s * nt + q;
+        out.acc[3 * o + 0] = acc[0];
+        out.acc[3 * o + 1] = acc[1];
+        out.acc[3 * o + 2] = acc[2];
+        out.adot[3 * o + 0] = acc[3];
+        out.adot[3 * o + 1] = acc[4];
+        out.adot[3 * o + 2] = acc[5];
+        out.cnt[o] = (final_counts && cnt > cap) ? -cnt : cnt;
+    }
+}
+
+// One wave per target.  Partial sums: lane l adds splits l, l+64, ... in
+// order, then a fixed xor-butterfly combines the 64 lane sums, so the total
+// is bitwise reproducible for a given S.  Neighbour sub-lists: the split
+// counts are loaded lane-parallel and prefix-summed across the wave; every
+// output entry e then finds its split by binary search over the prefix and
+// is copied independently.  Splits are contiguous ascending j-ranges, so the
+// concatenation in split order is ascending in j.
 __global__ __launch_bounds__(256) void finalize_kernel(QueryOut part, int S, int nt, int cap,
                                                        double *__restrict__ acc,
                                                        double *__restrict__ adot,
                                                        int *__restrict__ nnb,
                                                        int *__restrict__ list)
 {
+    extern __shared__ int s_off_all[];   // [4 waves][S + 1]
     const int lane = threadIdx.x & 63;
-    const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (q >= nt) return;
-    if (lane < 6) {
-        const double *src = lane < 3 ? part.acc : part.adot;
-        const int c = lane < 3 ? lane : lane - 3;
-        double sum = 0.0;
-        for (int s = 0; s < S; ++s) sum += src[((size_t)s * nt + q) * 3 + c];
-        (lane < 3 ? acc : adot)[(size_t)q * 3 + c] = sum;
+    const int w = threadIdx.x >> 6;
+    const int q = blockIdx.x * 4 + w;
+    const bool active = q < nt;
+    int *off = s_off_all + w * (S + 1);
+
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    if (active) {
+        for (int s = lane; s < S; s += 64) {
+            const size_t o = (size_t)s * nt + q;
+            a[0] += part.acc[3 * o + 0];
+            a[1] += part.acc[3 * o + 1];
+            a[2] += part.acc[3 * o + 2];
+            a[3] += part.adot[3 * o + 0];
+            a[4] += part.adot[3 * o + 1];
+            a[5] += part.adot[3 * o + 2];
+        }
     }
-    int off = 0;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a[c] += __shfl_xor(a[c], d, 64);
+
+    int running = 0;
+    for (int base = 0; base < S; base += 64) {
+        const int s = base + lane;
+        const int c = (active && s < S) ? part.cnt[(size_t)s * nt + q] : 0;
+        int incl = c;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += t;
+        }
+        if (s < S) off[s] = running + incl - c;
+        running += __shfl(incl, 63, 64);
+    }
+    if (lane == 0) off[S] = running;
+    __syncthreads();
+
+    if (!active) return;
+    if (lane < 3) acc[(size_t)q * 3 + lane] = a[lane];
+    else if (lane < 6) adot[(size_t)q * 3 + lane - 3] = a[lane];
+    const int total = running;
+    const int ncopy = min(total, cap);
     int *dst = list + (size_t)q * cap;
-    for (int s = 0; s < S; ++s) {
-        const size_t o = (size_t)s * nt + q;
-        const int c = part.cnt[o];
-        const int take = min(c, cap - off);
-        const int *src = part.list + o * (size_t)cap;
-        for (int e = lane; e < take; e += 64) dst[off + e] = src[e];
-        off += c;
+    for (int e = lane; e < ncopy; e += 64) {
+        // largest s with off[s] <= e (off is non-decreasing, off[0] = 0)
+        int lo = 0, hi = S - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= e) lo = mid; else hi = mid - 1;
+        }
+        dst[e] = part.list[((size_t)lo * nt + q) * (size_t)cap + (e - off[lo])];
     }
-    if (lane == 0) nnb[q] = off <= cap ? off : -off;
+    if (lane == 0) nnb[q] = total <= cap ? total : -total;
 }
 
 // Single workgroup: d_stats[0] = max over targets of min(|nnb|, cap),
@@ -328,16 +513,35 @@ hipError_t launch_t(const JState &js, const int *tgt, int nt, const Plan &p, con
 
 }  // namespace
 
-bool variant_supported(int prec, int ipl, int unroll)
+template <int JL>
+hipError_t launch_jl(const JState &js, const int *tgt, int nt, const Plan &p, const QueryOut &out,
+                     int cap, hipStream_t st)
 {
+    dim3 grid(p.grid_x, p.splits);
+    hipLaunchKernelGGL((regf_kernel_jl<JL>), grid, dim3(kBlock), 0, st, js, tgt, nt, p.jspan,
+                       out, cap, p.splits == 1 ? 1 : 0);
+    return hipGetLastError();
+}
+
+bool variant_supported(int prec, int ipl, int unroll, int jl)
+{
+    if (jl != 1) return prec == 0 && (jl == 4 || jl == 16 || jl == 64);
     if (prec == 1) return (ipl == 1 || ipl == 2) && unroll == 2;
     return (ipl == 1 || ipl == 2 || ipl == 4) && (unroll == 2 || unroll == 4);
+}
+
+int targets_per_block(const Plan &p)
+{
+    return p.jl > 1 ? 4 * (64 / p.jl) : kBlock * p.ipl;
 }
 
 hipError_t launch_regf(const JState &js, const int *tgt, int nt, const Plan &p,
                        const QueryOut &out, int cap, int prec, hipStream_t st)
 {
-    if (!variant_supported(prec, p.ipl, p.unroll)) return hipErrorInvalidValue;
+    if (!variant_supported(prec, p.ipl, p.unroll, p.jl)) return hipErrorInvalidValue;
+    if (p.jl == 4) return launch_jl<4>(js, tgt, nt, p, out, cap, st);
+    if (p.jl == 16) return launch_jl<16>(js, tgt, nt, p, out, cap, st);
+    if (p.jl == 64) return launch_jl<64>(js, tgt, nt, p, out, cap, st);
     if (prec == 1) {
         if (p.ipl == 1) return launch_t<1, 2, true>(js, tgt, nt, p, out, cap, st);
         return launch_t<2, 2, true>(js, tgt, nt, p, out, cap, st);
@@ -357,7 +561,8 @@ hipError_t launch_finalize(const QueryOut &part, int S, int nt, int cap, double 
                            double *adot, int *nnb, int *list, hipStream_t st)
 {
     dim3 grid((nt + 3) / 4);
-    hipLaunchKernelGGL(finalize_kernel, grid, dim3(256), 0, st, part, S, nt, cap, acc, adot, nnb,
+    const size_t lds = 4 * (size_t)(S + 1) * sizeof(int);
+    hipLaunchKernelGGL(finalize_kernel, grid, dim3(256), lds, st, part, S, nt, cap, acc, adot, nnb,
                        list);
     return hipGetLastError();
 }

@@ -29,7 +29,7 @@ struct abyss_ctx_s {
     int cu_count = 0;
     int cap = ABYSS_NUM_NEIGHBOR_MAX;
     int precision = ABYSS_PREC_SPLIT;
-    int force_ipl = 0, force_splits = 0, force_unroll = 0;
+    int force_ipl = 0, force_splits = 0, force_unroll = 0, force_jl = 0;
     unsigned long long scratch_limit = 1ull << 30;
 
     // j-set: device SoA [x|y|z|vx|vy|vz|m|rnb2] (stride n) + FP32 mass
@@ -151,26 +151,38 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
     Plan p;
     p.tile = abyss::kTile;
     const int cu = c->cu_count > 0 ? c->cu_count : 256;
-    int ipl = c->force_ipl;
-    if (ipl == 0) {
-        // Two targets per lane halves the LDS traffic per pair once there
-        // are enough targets to give every CU two workgroups at ipl = 2.
-        ipl = (nt >= abyss::kBlock * 2 * cu * 2) ? 2 : 1;
+    // Lanes per target: one target per lane once there are enough targets
+    // to occupy the chip; small blocks share a target among 4/16/64 lanes.
+    int jl = c->force_jl;
+    if (jl == 0) {
+        if (c->precision == ABYSS_PREC_FP64 || nt >= 16384) jl = 1;
+        else if (nt >= 2048) jl = 4;
+        else if (nt >= 256) jl = 16;
+        else jl = 64;
     }
+    if (c->precision == ABYSS_PREC_FP64) jl = 1;
+    // One target per lane (ipl = 1, unroll 2) measured fastest at every
+    // block size on MI355X; ipl 2/4 keep more targets in registers but cut
+    // occupancy to 3 and 1 waves per SIMD.
+    int ipl = c->force_ipl ? c->force_ipl : 1;
     int unroll = c->force_unroll ? c->force_unroll : 2;
     if (c->precision == ABYSS_PREC_FP64) {
         ipl = std::min(ipl, 2);
         unroll = 2;
     }
+    if (jl > 1) ipl = 1;
+    p.jl = jl;
     p.ipl = ipl;
     p.unroll = unroll;
-    p.grid_x = (nt + abyss::kBlock * ipl - 1) / (abyss::kBlock * ipl);
+    const int tpb = abyss::targets_per_block(p);
+    p.grid_x = (nt + tpb - 1) / tpb;
     int S = c->force_splits;
     if (S == 0) {
-        const int want_wg = cu * 8;
+        // ~12 workgroups per CU; every split keeps at least one tile.
+        const int want_wg = cu * 12;
         S = (want_wg + p.grid_x - 1) / p.grid_x;
-        const int max_by_n = std::max(1, n / (4 * abyss::kTile));
-        S = std::min(S, max_by_n);
+        const int max_by_n = std::max(1, n / abyss::kTile);
+        S = std::min(S, std::min(max_by_n, abyss::kMaxSplits));
     }
     if (S > 1) {
         const double per = (double)nt * ((double)c->cap * 4.0 + 6 * 8.0 + 4.0);
@@ -324,17 +336,22 @@ int abyss_ctx_set_precision(abyss_ctx_t c, int mode)
     return ABYSS_OK;
 }
 
-int abyss_ctx_set_tuning(abyss_ctx_t c, int ipl, int splits, int unroll)
+int abyss_ctx_set_tuning(abyss_ctx_t c, int ipl, int splits, int unroll, int jl)
 {
     if (!c) return ABYSS_ERR_INVALID_ARG;
     if (ipl != 0 && ipl != 1 && ipl != 2 && ipl != 4)
         return set_detail(c, ABYSS_ERR_INVALID_ARG, "i_per_lane must be 0, 1, 2 or 4");
     if (unroll != 0 && unroll != 2 && unroll != 4)
         return set_detail(c, ABYSS_ERR_INVALID_ARG, "j_unroll must be 0, 2 or 4");
-    if (splits < 0) return set_detail(c, ABYSS_ERR_INVALID_ARG, "j_splits must be >= 0");
+    if (jl != 0 && jl != 1 && jl != 4 && jl != 16 && jl != 64)
+        return set_detail(c, ABYSS_ERR_INVALID_ARG, "lanes_per_target must be 0, 1, 4, 16 or 64");
+    if (splits < 0 || splits > abyss::kMaxSplits)
+        return set_detail(c, ABYSS_ERR_INVALID_ARG, "j_splits must be in [0, %d]",
+                          abyss::kMaxSplits);
     c->force_ipl = ipl;
     c->force_splits = splits;
     c->force_unroll = unroll;
+    c->force_jl = jl;
     c->plan_valid = false;
     return ABYSS_OK;
 }
@@ -437,7 +454,8 @@ int abyss_ctx_set_targets(abyss_ctx_t c, int nt, const int *targets)
     return ABYSS_OK;
 }
 
-int abyss_ctx_plan(abyss_ctx_t c, int *ipl, int *splits, int *grid_x, int *tile, int *unroll)
+int abyss_ctx_plan(abyss_ctx_t c, int *ipl, int *splits, int *grid_x, int *tile, int *unroll,
+                   int *jl)
 {
     if (!c) return ABYSS_ERR_INVALID_ARG;
     if (!c->plan_valid) return set_detail(c, ABYSS_ERR_INVALID_ARG, "no targets set");
@@ -446,6 +464,7 @@ int abyss_ctx_plan(abyss_ctx_t c, int *ipl, int *splits, int *grid_x, int *tile,
     if (grid_x) *grid_x = c->plan.grid_x;
     if (tile) *tile = c->plan.tile;
     if (unroll) *unroll = c->plan.unroll;
+    if (jl) *jl = c->plan.jl;
     return ABYSS_OK;
 }
 

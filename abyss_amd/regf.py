@@ -85,7 +85,8 @@ def load():
         "abyss_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
         "abyss_ctx_destroy": (ctypes.c_int, [_vp]),
         "abyss_ctx_set_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
-        "abyss_ctx_set_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "abyss_ctx_set_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_int]),
         "abyss_ctx_set_scratch_limit": (ctypes.c_int, [_vp, ctypes.c_ulonglong]),
         "abyss_ctx_upload": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp]),
         "abyss_ctx_query": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _ip, _ip]),
@@ -94,7 +95,7 @@ def load():
         "abyss_ctx_sync": (ctypes.c_int, [_vp]),
         "abyss_ctx_fetch": (ctypes.c_int, [_vp, _dp, _dp, _ip, _ip]),
         "abyss_ctx_stream": (_vp, [_vp]),
-        "abyss_ctx_plan": (ctypes.c_int, [_vp, _ip, _ip, _ip, _ip, _ip]),
+        "abyss_ctx_plan": (ctypes.c_int, [_vp, _ip, _ip, _ip, _ip, _ip, _ip]),
         "abyss_ctx_error_detail": (ctypes.c_char_p, [_vp]),
     }
     for name, (res, args) in sig.items():
@@ -251,9 +252,9 @@ class Context:
     def set_precision(self, mode):
         self._chk(self._lib.abyss_ctx_set_precision(self.h, int(mode)))
 
-    def set_tuning(self, i_per_lane=0, j_splits=0, j_unroll=0):
+    def set_tuning(self, i_per_lane=0, j_splits=0, j_unroll=0, lanes_per_target=0):
         self._chk(self._lib.abyss_ctx_set_tuning(self.h, int(i_per_lane), int(j_splits),
-                                                 int(j_unroll)))
+                                                 int(j_unroll), int(lanes_per_target)))
 
     def set_scratch_limit(self, nbytes):
         self._chk(self._lib.abyss_ctx_set_scratch_limit(self.h, int(nbytes)))
@@ -302,11 +303,12 @@ class Context:
         return acc[:nt], adot[:nt], nnb[:nt], lst[:nt]
 
     def plan(self):
-        a, b, c, d, e = (ctypes.c_int() for _ in range(5))
+        a, b, c, d, e, f = (ctypes.c_int() for _ in range(6))
         self._chk(self._lib.abyss_ctx_plan(self.h, ctypes.byref(a), ctypes.byref(b),
-                                           ctypes.byref(c), ctypes.byref(d), ctypes.byref(e)))
+                                           ctypes.byref(c), ctypes.byref(d), ctypes.byref(e),
+                                           ctypes.byref(f)))
         return {"i_per_lane": a.value, "j_splits": b.value, "grid_x": c.value, "tile": d.value,
-                "j_unroll": e.value}
+                "j_unroll": e.value, "lanes_per_target": f.value}
 
     @property
     def stream(self):

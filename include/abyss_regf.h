@@ -133,9 +133,11 @@ typedef struct abyss_ctx_s *abyss_ctx_t;
 int abyss_ctx_create(int device, int cap, abyss_ctx_t *out);
 int abyss_ctx_destroy(abyss_ctx_t ctx);
 int abyss_ctx_set_precision(abyss_ctx_t ctx, int mode);
-/* Forces targets per lane (1, 2 or 4), the j-split count (>= 1) and the
- * inner-loop j unroll (2 or 4); 0 = automatic for each. */
-int abyss_ctx_set_tuning(abyss_ctx_t ctx, int i_per_lane, int j_splits, int j_unroll);
+/* Forces targets per lane (1, 2 or 4), the j-split count (1..2048), the
+ * inner-loop j unroll (2 or 4) and the lanes sharing one target (1, 4, 16
+ * or 64; > 1 implies one target per lane group); 0 = automatic for each. */
+int abyss_ctx_set_tuning(abyss_ctx_t ctx, int i_per_lane, int j_splits, int j_unroll,
+                         int lanes_per_target);
 /* Upper bound on the split-j scratch (bytes; default 1 GiB). */
 int abyss_ctx_set_scratch_limit(abyss_ctx_t ctx, unsigned long long bytes);
 
@@ -157,9 +159,9 @@ int abyss_ctx_fetch(abyss_ctx_t ctx, double *acc, double *adot, int *nnb,
 /* hipStream_t of the context (for event timing by the caller). */
 void *abyss_ctx_stream(abyss_ctx_t ctx);
 /* Plan chosen for the current targets: targets per lane, j splits, target
- * blocks, j per LDS tile, j unroll. */
+ * blocks, j per LDS tile, j unroll, lanes per target. */
 int abyss_ctx_plan(abyss_ctx_t ctx, int *i_per_lane, int *j_splits,
-                   int *grid_x, int *tile, int *j_unroll);
+                   int *grid_x, int *tile, int *j_unroll, int *lanes_per_target);
 /* Detail string of the context's last error. */
 const char *abyss_ctx_error_detail(abyss_ctx_t ctx);
 

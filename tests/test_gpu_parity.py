@@ -102,28 +102,32 @@ def test_fp64_kernel(ctx):
 
 
 def test_tuning_variants_agree(ctx):
+    """Every kernel variant classifies every pair identically (same FP32 r^2),
+    so lists are equal; one-target-per-lane variants share FP32 partial groups
+    (folds aligned to absolute j) and differ only in FP64 summation order;
+    lanes-per-target variants group the FP32 partials differently."""
     say("case: tuning variants")
     js = workloads.plummer_kroupa(65536)
     ctx.upload(js.m, js.x, js.v, js.r2)
     t = np.random.default_rng(9).permutation(js.n)[:3000].astype(np.int32)
     base = None
     try:
-        for ipl in (1, 2, 4):
-            for splits in (1, 5, 0):
-                ctx.set_tuning(ipl, splits)
-                r = ctx.query(t)
-                say(f"  ipl={ipl} splits={splits} plan={ctx.plan()}")
-                if base is None:
-                    base = r
-                    continue
-                # same FP32 partial groups (folds are aligned to absolute j),
-                # so only the FP64 summation order can differ
-                np.testing.assert_allclose(r[0], base[0], rtol=1e-10, atol=0)
-                np.testing.assert_allclose(r[1], base[1], rtol=1e-10, atol=0)
-                assert np.array_equal(r[2], base[2])
-                assert np.array_equal(r[3], base[3])
+        variants = [(ipl, s_, un, 1) for ipl in (1, 2, 4) for s_ in (1, 5, 0) for un in (2, 4)]
+        variants += [(1, s_, 2, jl) for jl in (4, 16, 64) for s_ in (1, 7, 0)]
+        for ipl, splits, un, jl in variants:
+            ctx.set_tuning(ipl, splits, un, jl)
+            r = ctx.query(t)
+            say(f"  ipl={ipl} splits={splits} unroll={un} jl={jl} plan={ctx.plan()}")
+            if base is None:
+                base = r
+                continue
+            rtol = 1e-10 if jl == 1 else 1e-6
+            np.testing.assert_allclose(r[0], base[0], rtol=rtol, atol=1e-12 * np.abs(base[0]).max())
+            np.testing.assert_allclose(r[1], base[1], rtol=rtol, atol=1e-12 * np.abs(base[1]).max())
+            assert np.array_equal(r[2], base[2])
+            assert np.array_equal(r[3], base[3])
     finally:
-        ctx.set_tuning(0, 0)
+        ctx.set_tuning(0, 0, 0, 0)
 
 
 def test_deterministic(ctx):
