@@ -155,9 +155,9 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
     // to occupy the chip; small blocks share a target among 4/16/64 lanes.
     int jl = c->force_jl;
     if (jl == 0) {
-        if (c->precision == ABYSS_PREC_FP64 || nt >= 16384) jl = 1;
-        else if (nt >= 2048) jl = 4;
-        else if (nt >= 256) jl = 16;
+        if (c->precision == ABYSS_PREC_FP64 || nt >= 4096) jl = 1;
+        else if (nt >= 1024) jl = 4;
+        else if (nt >= 64) jl = 16;
         else jl = 64;
     }
     if (c->precision == ABYSS_PREC_FP64) jl = 1;
@@ -178,8 +178,8 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
     p.grid_x = (nt + tpb - 1) / tpb;
     int S = c->force_splits;
     if (S == 0) {
-        // ~12 workgroups per CU; every split keeps at least one tile.
-        const int want_wg = cu * 12;
+        // ~16 workgroups per CU; every split keeps at least one tile.
+        const int want_wg = cu * 16;
         S = (want_wg + p.grid_x - 1) / p.grid_x;
         const int max_by_n = std::max(1, n / abyss::kTile);
         S = std::min(S, std::min(max_by_n, abyss::kMaxSplits));
