@@ -67,6 +67,39 @@ class HipEvents:
         return float(ms.value)
 
 
+def copy_calibration(nbytes, reps=4):
+    """Device-to-device copies of a known size (a blit kernel): the reference
+    point that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
+    lib = HipEvents().lib
+    lib.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    lib.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    lib.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
+    lib.hipFree.argtypes = [ctypes.c_void_p]
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert lib.hipMalloc(ctypes.byref(a), nbytes) == 0
+    assert lib.hipMalloc(ctypes.byref(b), nbytes) == 0
+    assert lib.hipMemset(a, 1, nbytes) == 0
+    for _ in range(reps):
+        assert lib.hipMemcpy(b, a, nbytes, 3) == 0      # hipMemcpyDeviceToDevice
+    assert lib.hipDeviceSynchronize() == 0
+    lib.hipFree(a)
+    lib.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
+
+
+def profiled_traffic(name="regf_kernel_king262144_fullN_traffic.md"):
+    """HBM bytes per launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated against a copy of known size)."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(js, target_seconds=10.0):
     """Time the oracle's FP64 loop (all host cores) on a bounded target sample."""
     import numpy as np
@@ -120,7 +153,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the sampled accuracy check")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run device-to-device copies of BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the regular-force path runs on a single MI355X "
@@ -194,7 +232,8 @@ def main():
                    "precision": args.precision, "plan": plan},
         "pct_fp32_peak": 100.0 * achieved / FP32_PEAK_TFLOPS,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": profiled_traffic() if args.n == 262144 else None,
                      "flop_per_interaction": FLOP_PER_INTERACTION,
                      "interactions_per_launch": inter},
         "cpu_baseline": cpu,

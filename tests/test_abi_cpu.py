@@ -1,0 +1,104 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+declared in include/abyss_regf.h, and validates before touching the device."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from abyss_amd import regf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "abyss_regf.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", src)
+    return sorted(set(n for n in names if n not in ("if", "while", "return")))
+
+
+def test_header_lists_match_python_binding():
+    assert sorted(declared_functions()) == sorted(regf.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = regf.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", regf.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(line.split()[-1] for line in out.splitlines() if line.strip())
+    for name in declared_functions():
+        assert name in exported, name
+
+
+def test_header_compiles_as_c_and_cpp(tmp_path):
+    c = tmp_path / "t.c"
+    c.write_text('#include "abyss_regf.h"\nint main(void){return abyss_device_count()<0;}\n')
+    for comp in (["gcc", "-std=c99"], ["g++", "-x", "c++"]):
+        subprocess.run(comp + ["-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(c)],
+                       check=True)
+
+
+def test_error_strings():
+    assert regf.error_string(regf.OK) == "ok"
+    assert "overflow" in regf.error_string(regf.ERR_NEIGHBOR_OVERFLOW)
+
+
+@pytest.mark.skipif(regf.device_count() > 0, reason="checks the no-device path")
+def test_calls_without_device_fail_cleanly():
+    with pytest.raises(regf.AbyssError) as e:
+        regf.SendToDevice(np.ones(2), np.zeros((2, 3)), np.zeros((2, 3)), np.ones(2))
+    assert e.value.code == regf.ERR_NOT_OPEN
+    with pytest.raises(regf.AbyssError) as e:
+        regf.CalculateAccelerationOnDevice([0])
+    assert e.value.code == regf.ERR_NOT_OPEN
+    with pytest.raises(regf.AbyssError) as e:
+        regf.OpenDevice(0)
+    assert e.value.code == regf.ERR_NO_DEVICE
+    with pytest.raises(regf.AbyssError) as e:
+        regf.Context()
+    assert e.value.code == regf.ERR_NO_DEVICE
+
+
+def test_neighbor_stride_setter_validates():
+    assert regf.num_neighbor_max() == 1000
+    with pytest.raises(regf.AbyssError):
+        regf.set_num_neighbor_max(0)
+    regf.set_num_neighbor_max(200)
+    assert regf.num_neighbor_max() == 200
+    regf.set_num_neighbor_max(1000)
+
+
+def test_c_caller_links_and_reports_no_device(tmp_path):
+    """A plain C caller in the style of ABYSS's host code links against the
+    library and gets a clean status (no GPU in this container)."""
+    src = tmp_path / "caller.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include "abyss_regf.h"
+int main(void) {
+    int rank = 0, n = 2, nt = 1, idx[1] = {0}, nnb[1];
+    double m[2] = {1, 1}, x[2][3] = {{0,0,0},{1,0,0}}, v[2][3] = {{0}}, r2[2] = {0, 0};
+    double acc[1][3], adot[1][3];
+    int list[ABYSS_NUM_NEIGHBOR_MAX];
+    if (abyss_device_count() > 0) return 0;
+    OpenDevice(&rank);
+    if (abyss_last_error() != ABYSS_ERR_NO_DEVICE) return 1;
+    SendToDevice(&n, m, x, v, r2, NULL);
+    if (abyss_last_error() != ABYSS_ERR_NOT_OPEN) return 2;
+    CalculateAccelerationOnDevice(&nt, idx, acc, adot, nnb, list);
+    if (abyss_last_error() != ABYSS_ERR_NOT_OPEN) return 3;
+    printf("ok\n");
+    return 0;
+}
+''')
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(regf.LIB_PATH)
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-L", libdir,
+                    "-labyss_regf", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)

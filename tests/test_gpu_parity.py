@@ -272,3 +272,22 @@ def test_king_262144_full_properties(ctx):
     sample = np.random.default_rng(4).permutation(js.n)[:1024].astype(np.int32)
     sub = tuple(a[sample] for a in split)
     check(js, sample, sub, label="king262144/sample")
+
+
+@pytest.mark.parametrize("name", ["plummer_n64_seed1.npz", "plummer_n1024_seed1.npz"])
+def test_golden_fixtures(name):
+    """The HIP path against the committed golden vectors (no oracle call)."""
+    import os
+    say(f"case: golden {name}")
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name))
+    cap = int(g["cap"])
+    with regf.Context(device=0, cap=cap) as c:
+        c.upload(g["m"], g["x"], g["v"], g["r2"])
+        acc, adot, nnb, lst = c.query(g["targets"])
+    assert np.array_equal(nnb, g["nnb"])
+    for k in range(len(nnb)):
+        assert np.array_equal(lst[k, :nnb[k]], g["lists"][k, :nnb[k]])
+    err = force_errors(acc, adot, g["acc"], g["adot"])
+    say(f"[golden {name}] {err}")
+    assert err["a_rel"]["max"] <= 1e-6
+    assert err["jdot_rel"]["p99"] <= 1e-6
