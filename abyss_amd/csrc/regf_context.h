@@ -1,0 +1,62 @@
+// Internal: the per-cluster context behind abyss_ctx_t, shared by the
+// single-cluster runtime (regf_runtime.hip) and the ensemble / serving
+// runtime (regf_ensemble.hip).  Not part of the public ABI.
+#pragma once
+#include "../../include/abyss_regf.h"
+#include "regf_internal.h"
+
+struct abyss_ctx_s {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    bool own_stream = true;   // false: the stream belongs to an ensemble
+    int cu_count = 0;
+    int cap = ABYSS_NUM_NEIGHBOR_MAX;
+    int precision = ABYSS_PREC_SPLIT;
+    int force_ipl = 0, force_splits = 0, force_unroll = 0, force_jl = 0;
+    unsigned long long scratch_limit = 1ull << 30;
+
+    // j-set: device SoA [x|y|z|vx|vy|vz|m|rnb2] (stride n) + FP32 mass
+    int n = 0;
+    size_t j_bytes = 0;
+    char *d_j = nullptr;
+    char *h_j = nullptr;  // pinned staging, same layout
+    abyss::JState js{};
+
+    // targets and outputs
+    int nt = 0;
+    size_t t_capacity = 0;  // targets the output buffers can hold
+    int *d_tgt = nullptr, *h_tgt = nullptr;
+    double *d_acc = nullptr, *d_adot = nullptr;
+    int *d_nnb = nullptr, *d_list = nullptr;
+    int *d_stats = nullptr;
+    double *h_res = nullptr;  // pinned [nt][6] (acc then adot)
+    int *h_nnb = nullptr;
+    int *h_list = nullptr;    // pinned, rows of width <= cap
+    size_t h_list_bytes = 0;
+    int *h_stats = nullptr;
+
+    // split-j scratch
+    size_t scratch_bytes = 0;
+    char *d_scratch = nullptr;
+    abyss::QueryOut part{};
+
+    abyss::Plan plan{};
+    bool plan_valid = false;
+
+    // timing / counters (ProfileDevice)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timing_pending = false;
+    long long n_send = 0, n_grav = 0, sum_ni = 0;
+    double interactions = 0.0, t_grav_ms = 0.0;
+
+    char detail[512] = {0};
+};
+
+
+namespace abyss {
+// Creates a context that issues all its work on `stream` (owned by the
+// caller) instead of a stream of its own.
+int ctx_create_on_stream(int device, int cap, hipStream_t stream, abyss_ctx_t *out);
+// Device bytes a context holds (j-set, targets, outputs, scratch).
+unsigned long long ctx_device_bytes(abyss_ctx_t c);
+}  // namespace abyss

@@ -1,0 +1,624 @@
+// Batched ensembles and the served ensemble (include/abyss_regf.h, Part 3).
+//
+// Ensemble: every cluster is a context (regf_context.h) whose work is issued
+// on the ensemble's stream.  A launch builds a work list of (cluster, target
+// block) items, copies it with the cluster descriptors to the device, and
+// replays a captured HIP graph of two kernels: a counter reset and the
+// persistent batch_kernel, which drains the queue.  Clusters run unsplit, so
+// no per-cluster scratch or merge pass is needed; the many clusters of a
+// batch fill the chip instead.  Items are queued longest-first (cost = the
+// item's targets x N) so the long items of big clusters start early and the
+// short ones fill in behind them.
+//
+// Served ensemble: a worker thread takes every pending request (at most one
+// per cluster per batch, FIFO), sets the clusters' targets, launches one
+// batch, fetches the results and completes the requests; requests arriving
+// meanwhile form the next batch (continuous batching).  Admission compares a
+// cluster's estimated device footprint with the remaining HBM budget.
+#include "regf_context.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+using abyss::BatchHeader;
+using abyss::ClusterDesc;
+using abyss::WorkItem;
+
+struct abyss_ens_s {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int cap = 0;
+    int cu_count = 0;
+    int max_clusters = 0;
+    std::vector<abyss_ctx_t> cl;     // nullptr: free slot
+    std::vector<char> active;        // has targets for the next launch
+
+    ClusterDesc *d_cd = nullptr, *h_cd = nullptr;
+    WorkItem *d_items = nullptr, *h_items = nullptr;
+    size_t items_cap = 0;
+    BatchHeader *d_hdr = nullptr, *h_hdr = nullptr;
+    int *d_counter = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int grid = 0;
+
+    char detail[512] = {0};
+};
+
+namespace {
+
+int ens_detail(abyss_ens_t e, int code, const char *fmt, ...)
+{
+    if (e) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(e->detail, sizeof(e->detail), fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define ENSCHK(e, call)                                                                \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ens_detail(e,                                                       \
+                              e_ == hipErrorOutOfMemory ? ABYSS_ERR_OUT_OF_MEMORY      \
+                                                        : ABYSS_ERR_HIP,               \
+                              "%s failed: %s", #call, hipGetErrorString(e_));          \
+    } while (0)
+
+// Copies a context's error into the ensemble's detail string.
+int from_ctx(abyss_ens_t e, abyss_ctx_t c, int rc)
+{
+    if (rc != ABYSS_OK && rc != ABYSS_ERR_NEIGHBOR_OVERFLOW && c)
+        ens_detail(e, rc, "%s", abyss_ctx_error_detail(c));
+    return rc;
+}
+
+void drop_graph(abyss_ens_t e)
+{
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+}
+
+int ensure_items(abyss_ens_t e, size_t n)
+{
+    if (n <= e->items_cap) return ABYSS_OK;
+    const size_t cap = std::max<size_t>(n + n / 2, 4096);
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    if (e->d_items) (void)hipFree(e->d_items);
+    if (e->h_items) (void)hipHostFree(e->h_items);
+    e->d_items = nullptr;
+    e->h_items = nullptr;
+    e->items_cap = 0;
+    drop_graph(e);   // the graph's kernel arguments point at the old buffer
+    ENSCHK(e, hipMalloc(&e->d_items, cap * sizeof(WorkItem)));
+    ENSCHK(e, hipHostMalloc(&e->h_items, cap * sizeof(WorkItem), hipHostMallocDefault));
+    e->items_cap = cap;
+    return ABYSS_OK;
+}
+
+int capture_graph(abyss_ens_t e)
+{
+    ENSCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t r1 = abyss::launch_batch_reset(e->d_counter, e->stream);
+    hipError_t r2 = abyss::launch_batch(e->d_cd, e->d_items, e->d_hdr, e->d_counter, e->cap,
+                                        e->grid, e->stream);
+    hipGraph_t g = nullptr;
+    hipError_t r3 = hipStreamEndCapture(e->stream, &g);
+    ENSCHK(e, r1);
+    ENSCHK(e, r2);
+    ENSCHK(e, r3);
+    e->graph = g;
+    ENSCHK(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    return ABYSS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int abyss_ens_create(int device, int cap, int max_clusters, abyss_ens_t *out)
+{
+    if (!out || cap <= 0 || max_clusters <= 0) return ABYSS_ERR_INVALID_ARG;
+    *out = nullptr;
+    const int ndev = abyss_device_count();
+    if (ndev <= 0) return ABYSS_ERR_NO_DEVICE;
+    if (device < 0 || device >= ndev) return ABYSS_ERR_INVALID_ARG;
+    abyss_ens_t e = new (std::nothrow) abyss_ens_s();
+    if (!e) return ABYSS_ERR_OUT_OF_MEMORY;
+    e->device = device;
+    e->cap = cap;
+    e->max_clusters = max_clusters;
+    e->cl.assign(max_clusters, nullptr);
+    e->active.assign(max_clusters, 0);
+    int rc = ABYSS_OK;
+    do {
+        if (hipSetDevice(device) != hipSuccess) { rc = ABYSS_ERR_NO_DEVICE; break; }
+        if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+            rc = ABYSS_ERR_HIP; break;
+        }
+        if (hipDeviceGetAttribute(&e->cu_count, hipDeviceAttributeMultiprocessorCount, device) !=
+            hipSuccess) { rc = ABYSS_ERR_HIP; break; }
+        // four 256-thread workgroups per CU: the batch kernel's occupancy
+        e->grid = std::max(1, e->cu_count) * 4;
+        if (hipMalloc(&e->d_cd, (size_t)max_clusters * sizeof(ClusterDesc)) != hipSuccess ||
+            hipHostMalloc(&e->h_cd, (size_t)max_clusters * sizeof(ClusterDesc),
+                          hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&e->d_hdr, sizeof(BatchHeader)) != hipSuccess ||
+            hipHostMalloc(&e->h_hdr, sizeof(BatchHeader), hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&e->d_counter, 4 * sizeof(int)) != hipSuccess) {
+            rc = ABYSS_ERR_OUT_OF_MEMORY; break;
+        }
+        memset(e->h_hdr, 0, sizeof(BatchHeader));
+    } while (0);
+    if (rc != ABYSS_OK) {
+        abyss_ens_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return ABYSS_OK;
+}
+
+int abyss_ens_destroy(abyss_ens_t e)
+{
+    if (!e) return ABYSS_ERR_INVALID_ARG;
+    if (e->device >= 0) (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (auto &c : e->cl)
+        if (c) abyss_ctx_destroy(c);
+    drop_graph(e);
+    if (e->d_cd) (void)hipFree(e->d_cd);
+    if (e->h_cd) (void)hipHostFree(e->h_cd);
+    if (e->d_items) (void)hipFree(e->d_items);
+    if (e->h_items) (void)hipHostFree(e->h_items);
+    if (e->d_hdr) (void)hipFree(e->d_hdr);
+    if (e->h_hdr) (void)hipHostFree(e->h_hdr);
+    if (e->d_counter) (void)hipFree(e->d_counter);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return ABYSS_OK;
+}
+
+const char *abyss_ens_error_detail(abyss_ens_t e) { return e ? e->detail : ""; }
+void *abyss_ens_stream(abyss_ens_t e) { return e ? (void *)e->stream : nullptr; }
+
+int abyss_ens_upload(abyss_ens_t e, int cluster, int n, const double *m, const double *x,
+                     const double *v, const double *r2)
+{
+    if (!e) return ABYSS_ERR_INVALID_ARG;
+    if (cluster < 0 || cluster >= e->max_clusters)
+        return ens_detail(e, ABYSS_ERR_INVALID_ARG, "cluster %d outside [0,%d)", cluster,
+                          e->max_clusters);
+    abyss_ctx_t &c = e->cl[cluster];
+    if (!c) {
+        int rc = abyss::ctx_create_on_stream(e->device, e->cap, e->stream, &c);
+        if (rc != ABYSS_OK) {
+            c = nullptr;
+            return ens_detail(e, rc, "context for cluster %d: %s", cluster,
+                              abyss_error_string(rc));
+        }
+        // batched clusters run unsplit with one target per lane or a lane
+        // group per target (the batch kernel's bodies)
+        abyss_ctx_set_tuning(c, 1, 1, 2, 0);
+    }
+    e->active[cluster] = 0;
+    return from_ctx(e, c, abyss_ctx_upload(c, n, m, x, v, r2));
+}
+
+int abyss_ens_release(abyss_ens_t e, int cluster)
+{
+    if (!e || cluster < 0 || cluster >= e->max_clusters) return ABYSS_ERR_INVALID_ARG;
+    if (e->cl[cluster]) {
+        ENSCHK(e, hipSetDevice(e->device));
+        ENSCHK(e, hipStreamSynchronize(e->stream));
+        abyss_ctx_destroy(e->cl[cluster]);
+    }
+    e->cl[cluster] = nullptr;
+    e->active[cluster] = 0;
+    return ABYSS_OK;
+}
+
+int abyss_ens_set_targets(abyss_ens_t e, int cluster, int nt, const int *targets)
+{
+    if (!e || cluster < 0 || cluster >= e->max_clusters) return ABYSS_ERR_INVALID_ARG;
+    abyss_ctx_t c = e->cl[cluster];
+    if (!c) return ens_detail(e, ABYSS_ERR_NO_JSET, "cluster %d is empty", cluster);
+    if (nt == 0) {
+        e->active[cluster] = 0;
+        return ABYSS_OK;
+    }
+    int rc = abyss_ctx_set_targets(c, nt, targets);
+    e->active[cluster] = rc == ABYSS_OK ? 1 : 0;
+    return from_ctx(e, c, rc);
+}
+
+int abyss_ens_launch(abyss_ens_t e)
+{
+    if (!e) return ABYSS_ERR_INVALID_ARG;
+    ENSCHK(e, hipSetDevice(e->device));
+    // active clusters, longest item first
+    struct Entry {
+        int cluster, nblocks;
+        double item_cost;
+    };
+    std::vector<Entry> act;
+    size_t nitems = 0;
+    for (int k = 0; k < e->max_clusters; ++k) {
+        abyss_ctx_t c = e->cl[k];
+        if (!c || !e->active[k] || c->nt <= 0 || !c->plan_valid) continue;
+        const int tpb = abyss::targets_per_block(c->plan);
+        act.push_back({k, c->plan.grid_x, (double)std::min(tpb, c->nt) * c->n});
+        nitems += c->plan.grid_x;
+    }
+    if (act.empty()) return ABYSS_OK;
+    std::sort(act.begin(), act.end(), [](const Entry &a, const Entry &b) {
+        return a.item_cost != b.item_cost ? a.item_cost > b.item_cost : a.cluster < b.cluster;
+    });
+    int rc = ensure_items(e, nitems);
+    if (rc != ABYSS_OK) return rc;
+    // the previous launch may still read the pinned descriptor buffers
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    size_t it = 0;
+    for (size_t a = 0; a < act.size(); ++a) {
+        abyss_ctx_t c = e->cl[act[a].cluster];
+        ClusterDesc &d = e->h_cd[a];
+        d.js = c->js;
+        d.tgt = c->d_tgt;
+        d.nt = c->nt;
+        d.jl = c->plan.jl;
+        d.out = abyss::QueryOut{c->d_acc, c->d_adot, c->d_nnb, c->d_list};
+        for (int b = 0; b < act[a].nblocks; ++b) e->h_items[it++] = WorkItem{(int)a, b};
+        c->n_grav += 1;
+        c->sum_ni += c->nt;
+        c->interactions += (double)c->nt * (double)(c->n - 1);
+    }
+    e->h_hdr->nitems = (int)nitems;
+    ENSCHK(e, hipMemcpyAsync(e->d_cd, e->h_cd, act.size() * sizeof(ClusterDesc),
+                             hipMemcpyHostToDevice, e->stream));
+    ENSCHK(e, hipMemcpyAsync(e->d_items, e->h_items, nitems * sizeof(WorkItem),
+                             hipMemcpyHostToDevice, e->stream));
+    ENSCHK(e, hipMemcpyAsync(e->d_hdr, e->h_hdr, sizeof(BatchHeader), hipMemcpyHostToDevice,
+                             e->stream));
+    if (!e->exec) {
+        rc = capture_graph(e);
+        if (rc != ABYSS_OK) return rc;
+    }
+    ENSCHK(e, hipGraphLaunch(e->exec, e->stream));
+    return ABYSS_OK;
+}
+
+int abyss_ens_sync(abyss_ens_t e)
+{
+    if (!e) return ABYSS_ERR_INVALID_ARG;
+    ENSCHK(e, hipSetDevice(e->device));
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    return ABYSS_OK;
+}
+
+int abyss_ens_fetch(abyss_ens_t e, int cluster, double *acc, double *adot, int *nnb, int *list)
+{
+    if (!e || cluster < 0 || cluster >= e->max_clusters) return ABYSS_ERR_INVALID_ARG;
+    abyss_ctx_t c = e->cl[cluster];
+    if (!c) return ens_detail(e, ABYSS_ERR_NO_JSET, "cluster %d is empty", cluster);
+    return from_ctx(e, c, abyss_ctx_fetch(c, acc, adot, nnb, list));
+}
+
+unsigned long long abyss_ens_device_bytes(abyss_ens_t e)
+{
+    if (!e) return 0;
+    unsigned long long b = (unsigned long long)e->max_clusters * sizeof(ClusterDesc) +
+                           e->items_cap * sizeof(WorkItem);
+    for (auto c : e->cl) b += abyss::ctx_device_bytes(c);
+    return b;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Served ensemble
+// ---------------------------------------------------------------------------
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    long long id = 0;
+    int cluster = 0;
+    std::vector<int> targets;
+    Clock::time_point t_submit;
+    bool done = false;
+    int status = ABYSS_OK;
+    std::vector<double> acc, adot;
+    std::vector<int> nnb, list;
+    double latency_ms = 0.0;
+};
+
+}  // namespace
+
+struct abyss_srv_s {
+    abyss_ens_t ens = nullptr;
+    int cap = 0;
+    int max_clusters = 0;
+    unsigned long long budget = 0, used = 0;
+    std::vector<unsigned long long> footprint;   // 0: free slot
+    std::vector<int> nparticles;
+
+    std::mutex mu;        // queue, request map, statistics, cluster table
+    std::mutex ens_mu;    // every call into the ensemble
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::shared_ptr<Request>> pending;
+    std::unordered_map<long long, std::shared_ptr<Request>> reqs;
+    std::thread worker;
+    bool stop = false;
+    long long next_id = 1;
+
+    std::vector<double> latencies;
+    double interactions = 0.0;
+    long long batches = 0, completed = 0;
+};
+
+namespace {
+
+void serve_loop(abyss_srv_t s)
+{
+    for (;;) {
+        std::vector<std::shared_ptr<Request>> batch;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+            if (s->stop && s->pending.empty()) return;
+            std::vector<char> taken(s->max_clusters, 0);
+            for (auto it = s->pending.begin(); it != s->pending.end();) {
+                const int c = (*it)->cluster;
+                if (!taken[c]) {
+                    taken[c] = 1;
+                    batch.push_back(*it);
+                    it = s->pending.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+        }
+        double inter = 0.0;
+        {
+            std::lock_guard<std::mutex> elk(s->ens_mu);
+            for (auto &r : batch) {
+                r->status = abyss_ens_set_targets(s->ens, r->cluster, (int)r->targets.size(),
+                                                  r->targets.data());
+            }
+            int rc = abyss_ens_launch(s->ens);
+            if (rc == ABYSS_OK) rc = abyss_ens_sync(s->ens);
+            for (auto &r : batch) {
+                if (r->status == ABYSS_OK && rc != ABYSS_OK) r->status = rc;
+                if (r->status == ABYSS_OK) {
+                    const size_t nt = r->targets.size();
+                    r->acc.resize(3 * nt);
+                    r->adot.resize(3 * nt);
+                    r->nnb.resize(nt);
+                    r->list.resize(nt * (size_t)s->cap);
+                    r->status = abyss_ens_fetch(s->ens, r->cluster, r->acc.data(),
+                                                r->adot.data(), r->nnb.data(), r->list.data());
+                    inter += (double)nt * (double)(s->nparticles[r->cluster] - 1);
+                }
+                abyss_ens_set_targets(s->ens, r->cluster, 0, nullptr);
+            }
+        }
+        const auto now = Clock::now();
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (auto &r : batch) {
+                r->latency_ms =
+                    std::chrono::duration<double, std::milli>(now - r->t_submit).count();
+                r->done = true;
+                s->latencies.push_back(r->latency_ms);
+                s->completed += 1;
+            }
+            s->interactions += inter;
+            s->batches += 1;
+        }
+        s->cv_done.notify_all();
+    }
+}
+
+double percentile(std::vector<double> v, double p)
+{
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const double pos = p * (double)(v.size() - 1);
+    const size_t i = (size_t)pos;
+    const double f = pos - (double)i;
+    return i + 1 < v.size() ? v[i] * (1.0 - f) + v[i + 1] * f : v[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+int abyss_srv_create(int device, int cap, int max_clusters, unsigned long long hbm_budget,
+                     abyss_srv_t *out)
+{
+    if (!out || hbm_budget == 0) return ABYSS_ERR_INVALID_ARG;
+    *out = nullptr;
+    abyss_ens_t e = nullptr;
+    int rc = abyss_ens_create(device, cap, max_clusters, &e);
+    if (rc != ABYSS_OK) return rc;
+    abyss_srv_t s = new (std::nothrow) abyss_srv_s();
+    if (!s) {
+        abyss_ens_destroy(e);
+        return ABYSS_ERR_OUT_OF_MEMORY;
+    }
+    s->ens = e;
+    s->cap = cap;
+    s->max_clusters = max_clusters;
+    s->budget = hbm_budget;
+    s->footprint.assign(max_clusters, 0);
+    s->nparticles.assign(max_clusters, 0);
+    s->worker = std::thread(serve_loop, s);
+    *out = s;
+    return ABYSS_OK;
+}
+
+int abyss_srv_destroy(abyss_srv_t s)
+{
+    if (!s) return ABYSS_ERR_INVALID_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    abyss_ens_destroy(s->ens);
+    delete s;
+    return ABYSS_OK;
+}
+
+unsigned long long abyss_srv_cluster_footprint(abyss_srv_t s, int n)
+{
+    if (!s || n <= 0) return 0;
+    // j-set (68 B/particle) + targets and results for up to n targets, with
+    // the 1.5x growth headroom of the context's buffers
+    const double per = 68.0 + 1.5 * (4.0 + 48.0 + 4.0 + 4.0 * s->cap);
+    return (unsigned long long)(per * n) + (1ull << 20);
+}
+
+int abyss_srv_add_cluster(abyss_srv_t s, int n, const double *m, const double *x,
+                          const double *v, const double *r2, int *cluster)
+{
+    if (!s || !cluster || n <= 0) return ABYSS_ERR_INVALID_ARG;
+    const unsigned long long need = abyss_srv_cluster_footprint(s, n);
+    int slot = -1;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        for (int k = 0; k < s->max_clusters; ++k)
+            if (s->footprint[k] == 0) {
+                slot = k;
+                break;
+            }
+        if (slot < 0) return ABYSS_ERR_BUSY;
+        if (s->used + need > s->budget) return ABYSS_ERR_ADMISSION;
+        s->footprint[slot] = need;   // reserve before the upload
+        s->used += need;
+        s->nparticles[slot] = n;
+    }
+    int rc;
+    {
+        std::lock_guard<std::mutex> elk(s->ens_mu);
+        rc = abyss_ens_upload(s->ens, slot, n, m, x, v, r2);
+        if (rc != ABYSS_OK) abyss_ens_release(s->ens, slot);
+    }
+    if (rc != ABYSS_OK) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->used -= s->footprint[slot];
+        s->footprint[slot] = 0;
+        return rc;
+    }
+    *cluster = slot;
+    return ABYSS_OK;
+}
+
+int abyss_srv_remove_cluster(abyss_srv_t s, int cluster)
+{
+    if (!s || cluster < 0 || cluster >= s->max_clusters) return ABYSS_ERR_INVALID_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->footprint[cluster] == 0) return ABYSS_ERR_INVALID_ARG;
+        for (auto &r : s->pending)
+            if (r->cluster == cluster) return ABYSS_ERR_BUSY;
+    }
+    int rc;
+    {
+        std::lock_guard<std::mutex> elk(s->ens_mu);
+        rc = abyss_ens_release(s->ens, cluster);
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->used -= s->footprint[cluster];
+    s->footprint[cluster] = 0;
+    s->nparticles[cluster] = 0;
+    return rc;
+}
+
+int abyss_srv_submit(abyss_srv_t s, int cluster, int nt, const int *targets, long long *ticket)
+{
+    if (!s || !ticket || !targets || nt <= 0 || cluster < 0 || cluster >= s->max_clusters)
+        return ABYSS_ERR_INVALID_ARG;
+    auto r = std::make_shared<Request>();
+    r->cluster = cluster;
+    r->targets.assign(targets, targets + nt);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->footprint[cluster] == 0) return ABYSS_ERR_NO_JSET;
+        const int n = s->nparticles[cluster];
+        if (nt > n) return ABYSS_ERR_INVALID_ARG;
+        for (int k = 0; k < nt; ++k)
+            if (targets[k] < 0 || targets[k] >= n) return ABYSS_ERR_TARGET_RANGE;
+        r->id = s->next_id++;
+        r->t_submit = Clock::now();
+        s->reqs[r->id] = r;
+        s->pending.push_back(r);
+        *ticket = r->id;
+    }
+    s->cv_work.notify_one();
+    return ABYSS_OK;
+}
+
+int abyss_srv_wait(abyss_srv_t s, long long ticket, double *acc, double *adot, int *nnb,
+                   int *list, double *latency_ms)
+{
+    if (!s) return ABYSS_ERR_INVALID_ARG;
+    std::shared_ptr<Request> r;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(ticket);
+        if (it == s->reqs.end()) return ABYSS_ERR_INVALID_ARG;
+        r = it->second;
+        s->cv_done.wait(lk, [&] { return r->done; });
+        s->reqs.erase(ticket);
+    }
+    if (latency_ms) *latency_ms = r->latency_ms;
+    if (r->status != ABYSS_OK && r->status != ABYSS_ERR_NEIGHBOR_OVERFLOW) return r->status;
+    if (acc) memcpy(acc, r->acc.data(), r->acc.size() * sizeof(double));
+    if (adot) memcpy(adot, r->adot.data(), r->adot.size() * sizeof(double));
+    if (nnb) memcpy(nnb, r->nnb.data(), r->nnb.size() * sizeof(int));
+    if (list) memcpy(list, r->list.data(), r->list.size() * sizeof(int));
+    return r->status;
+}
+
+int abyss_srv_stats(abyss_srv_t s, long long *completed, long long *batches,
+                    double *interactions, double *p50_ms, double *p99_ms)
+{
+    if (!s) return ABYSS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (completed) *completed = s->completed;
+    if (batches) *batches = s->batches;
+    if (interactions) *interactions = s->interactions;
+    if (p50_ms) *p50_ms = percentile(s->latencies, 0.50);
+    if (p99_ms) *p99_ms = percentile(s->latencies, 0.99);
+    return ABYSS_OK;
+}
+
+int abyss_srv_reset_stats(abyss_srv_t s)
+{
+    if (!s) return ABYSS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->latencies.clear();
+    s->interactions = 0.0;
+    s->batches = s->completed = 0;
+    return ABYSS_OK;
+}
+
+}  // extern "C"

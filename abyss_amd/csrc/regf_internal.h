@@ -47,6 +47,36 @@ constexpr int kTile = 256;    // j per LDS tile
 constexpr int kFold = 64;     // j per FP32 partial before the FP64 fold
 constexpr int kMaxSplits = 2048;  // bound of the finalize kernel's LDS prefix
 
+// ---- batched ensembles ---------------------------------------------------
+// One cluster of a batched launch: its j-set, targets, plan and outputs.
+// Batched clusters run unsplit (one j-range), so `out` holds the final
+// acc/adot/nnb/list arrays.
+struct ClusterDesc {
+    JState js;
+    const int *tgt;
+    int nt, jl;
+    QueryOut out;
+};
+
+// (cluster, target block): one workgroup's worth of work.
+struct WorkItem {
+    int cluster, tblock;
+};
+
+// Per-launch sizes, read by the persistent kernel from device memory so a
+// captured graph can replay with a different mix of clusters.
+struct BatchHeader {
+    int nitems;    // work items in `items`
+    int pad[3];
+};
+
+// counters[0..1] := 0 (a kernel, so that graph replays need no memset node)
+hipError_t launch_batch_reset(int *counters, hipStream_t st);
+// Persistent pairwise kernel over the work queue; exits when the queue index
+// passes hdr->nitems.
+hipError_t launch_batch(const ClusterDesc *cd, const WorkItem *items, const BatchHeader *hdr,
+                        int *counter, int cap, int grid, hipStream_t st);
+
 // Launch the pairwise kernel for nt targets against js (all j), split into
 // plan.splits contiguous j-ranges.  prec: 0 split precision, 1 FP64.
 hipError_t launch_regf(const JState &js, const int *tgt, int nt, const Plan &plan,

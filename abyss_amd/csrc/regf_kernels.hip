@@ -180,24 +180,26 @@ __device__ __forceinline__ void run_tile(Targets<IPL, F64> &T, const double2 *__
     }
 }
 
+// One workgroup's share of a query: target block bx against j-split s.
+// Shared by regf_kernel (one launch per query) and batch_kernel (work items
+// of many clusters in one persistent launch), so both evaluate every pair
+// with the same instructions in the same order.
 template <int IPL, int U, bool F64>
-__global__ __launch_bounds__(kBlock) void regf_kernel(JState js, const int *__restrict__ tgt,
-                                                      int nt, int jspan, QueryOut out, int cap,
-                                                      int final_counts)
+__device__ __forceinline__ void regf_body(const JState &js, const int *__restrict__ tgt, int nt,
+                                          int jspan, const QueryOut &out, int cap,
+                                          int final_counts, int bx, int s,
+                                          double2 (*s_pos)[kTile * 3],
+                                          typename MassT<F64>::type (*s_m)[kTile])
 {
     using MT = typename MassT<F64>::type;
-    __shared__ double2 s_pos[2][kTile * 3];
-    __shared__ MT s_m[2][kTile];
-
     const int tid = threadIdx.x;
-    const int s = blockIdx.y;
     const int j0 = s * jspan;
     const int j1 = min(js.n, j0 + jspan);
 
     Targets<IPL, F64> T;
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
-        const int q = blockIdx.x * (kBlock * IPL) + k * kBlock + tid;
+        const int q = bx * (kBlock * IPL) + k * kBlock + tid;
         if (q < nt) {
             const int i = tgt[q];
             T.idx[k] = i;
@@ -252,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void regf_kernel(JState js, const int *__re
 
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
-        const int q = blockIdx.x * (kBlock * IPL) + k * kBlock + tid;
+        const int q = bx * (kBlock * IPL) + k * kBlock + tid;
         if (q < nt) {
             const size_t o = (size_t)s * nt + q;
             out.acc[3 * o + 0] = T.acc[k][0];
@@ -267,6 +269,17 @@ __global__ __launch_bounds__(kBlock) void regf_kernel(JState js, const int *__re
     }
 }
 
+template <int IPL, int U, bool F64>
+__global__ __launch_bounds__(kBlock) void regf_kernel(JState js, const int *__restrict__ tgt,
+                                                      int nt, int jspan, QueryOut out, int cap,
+                                                      int final_counts)
+{
+    __shared__ double2 s_pos[2][kTile * 3];
+    __shared__ typename MassT<F64>::type s_m[2][kTile];
+    regf_body<IPL, U, F64>(js, tgt, nt, jspan, out, cap, final_counts, blockIdx.x, blockIdx.y,
+                           s_pos, s_m);
+}
+
 // Small-block variant: JL lanes share one target (JL = 4, 16 or 64), so a
 // wave serves 64/JL targets and a few active particles still fill the chip.
 // In a tile, step t hands the group's lanes the JL consecutive records
@@ -277,23 +290,20 @@ __global__ __launch_bounds__(kBlock) void regf_kernel(JState js, const int *__re
 // of its own pairs; at the end the JL FP64 sums of a group are combined by a
 // fixed xor-butterfly (bitwise reproducible).
 template <int JL>
-__global__ __launch_bounds__(kBlock) void regf_kernel_jl(JState js, const int *__restrict__ tgt,
-                                                         int nt, int jspan, QueryOut out, int cap,
-                                                         int final_counts)
+__device__ __forceinline__ void regf_body_jl(const JState &js, const int *__restrict__ tgt, int nt,
+                                             int jspan, const QueryOut &out, int cap,
+                                             int final_counts, int bx, int s,
+                                             double2 (*s_pos)[kTile * 3], float (*s_m)[kTile])
 {
     constexpr int TW = 64 / JL;            // targets per wave
     constexpr int STEPS = kTile / JL;      // steps per full tile
-    __shared__ double2 s_pos[2][kTile * 3];
-    __shared__ float s_m[2][kTile];
-
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int sub = lane % JL;
     const int g = lane / JL;
-    const int s = blockIdx.y;
     const int j0 = s * jspan;
     const int j1 = min(js.n, j0 + jspan);
-    const int q = blockIdx.x * (4 * TW) + (tid >> 6) * TW + g;
+    const int q = bx * (4 * TW) + (tid >> 6) * TW + g;
     const unsigned long long gmask =
         (JL == 64) ? ~0ull : (((1ull << JL) - 1ull) << (g * JL));
     const unsigned long long below = (1ull << lane) - 1ull;   // lanes < this lane
@@ -400,6 +410,17 @@ __global__ __launch_bounds__(kBlock) void regf_kernel_jl(JState js, const int *_
     }
 }
 
+template <int JL>
+__global__ __launch_bounds__(kBlock) void regf_kernel_jl(JState js, const int *__restrict__ tgt,
+                                                         int nt, int jspan, QueryOut out, int cap,
+                                                         int final_counts)
+{
+    __shared__ double2 s_pos[2][kTile * 3];
+    __shared__ float s_m[2][kTile];
+    regf_body_jl<JL>(js, tgt, nt, jspan, out, cap, final_counts, blockIdx.x, blockIdx.y, s_pos,
+                     s_m);
+}
+
 // One wave per target.  Partial sums: lane l adds splits l, l+64, ... in
 // order, then a fixed xor-butterfly combines the 64 lane sums, so the total
 // is bitwise reproducible for a given S.  Neighbour sub-lists: the split
@@ -407,18 +428,15 @@ __global__ __launch_bounds__(kBlock) void regf_kernel_jl(JState js, const int *_
 // output entry e then finds its split by binary search over the prefix and
 // is copied independently.  Splits are contiguous ascending j-ranges, so the
 // concatenation in split order is ascending in j.
-__global__ __launch_bounds__(256) void finalize_kernel(QueryOut part, int S, int nt, int cap,
-                                                       double *__restrict__ acc,
-                                                       double *__restrict__ adot,
-                                                       int *__restrict__ nnb,
-                                                       int *__restrict__ list)
+// Called by every thread of the workgroup (it contains a barrier); waves with
+// active == false only take part in the barrier.  `off` is this wave's LDS
+// prefix array of S + 1 ints.
+__device__ __forceinline__ void finalize_body(const QueryOut &part, int S, int nt, int cap, int q,
+                                              bool active, int *off, double *__restrict__ acc,
+                                              double *__restrict__ adot,
+                                              int *__restrict__ nnb, int *__restrict__ list)
 {
-    extern __shared__ int s_off_all[];   // [4 waves][S + 1]
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int q = blockIdx.x * 4 + w;
-    const bool active = q < nt;
-    int *off = s_off_all + w * (S + 1);
 
     double a[6] = {0, 0, 0, 0, 0, 0};
     if (active) {
@@ -471,6 +489,18 @@ __global__ __launch_bounds__(256) void finalize_kernel(QueryOut part, int S, int
     if (lane == 0) nnb[q] = total <= cap ? total : -total;
 }
 
+__global__ __launch_bounds__(256) void finalize_kernel(QueryOut part, int S, int nt, int cap,
+                                                       double *__restrict__ acc,
+                                                       double *__restrict__ adot,
+                                                       int *__restrict__ nnb,
+                                                       int *__restrict__ list)
+{
+    extern __shared__ int s_off_all[];   // [4 waves][S + 1]
+    const int w = threadIdx.x >> 6;
+    const int q = blockIdx.x * 4 + w;
+    finalize_body(part, S, nt, cap, q, q < nt, s_off_all + w * (S + 1), acc, adot, nnb, list);
+}
+
 // Single workgroup: d_stats[0] = max over targets of min(|nnb|, cap),
 // d_stats[1] = number of overflowed targets (nnb < 0).
 __global__ __launch_bounds__(1024) void count_stats_kernel(const int *__restrict__ nnb, int nt,
@@ -498,6 +528,56 @@ __global__ __launch_bounds__(1024) void count_stats_kernel(const int *__restrict
     if (threadIdx.x == 0) {
         d_stats[0] = s_max[0];
         d_stats[1] = s_ovf[0];
+    }
+}
+
+// ---- batched ensembles ---------------------------------------------------
+
+__global__ void batch_reset_kernel(int *counters)
+{
+    if (threadIdx.x < 2) counters[threadIdx.x] = 0;
+}
+
+// Persistent: each workgroup takes work items from a device queue until it
+// is empty, so small and large clusters share one launch and the chip stays
+// busy.  A cluster's items use the same plan and body as a single-cluster
+// query, so its results do not depend on what else is in the batch.
+// Every workgroup leaves the loop once the queue index passes nitems.
+__global__ __launch_bounds__(kBlock) void batch_kernel(const ClusterDesc *__restrict__ cd,
+                                                       const WorkItem *__restrict__ items,
+                                                       const BatchHeader *__restrict__ hdr,
+                                                       int *__restrict__ counter, int cap)
+{
+    __shared__ double2 s_pos[2][kTile * 3];
+    __shared__ float s_m[2][kTile];
+    __shared__ int s_item;
+    const int nitems = hdr->nitems;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= nitems) break;
+        const WorkItem w = items[it];
+        const ClusterDesc &d = cd[w.cluster];
+        const int jspan = d.js.n;   // unsplit: one j-range per cluster
+        switch (d.jl) {
+            case 1:
+                regf_body<1, 2, false>(d.js, d.tgt, d.nt, jspan, d.out, cap, 1, w.tblock, 0,
+                                       s_pos, s_m);
+                break;
+            case 4:
+                regf_body_jl<4>(d.js, d.tgt, d.nt, jspan, d.out, cap, 1, w.tblock, 0, s_pos, s_m);
+                break;
+            case 16:
+                regf_body_jl<16>(d.js, d.tgt, d.nt, jspan, d.out, cap, 1, w.tblock, 0, s_pos,
+                                 s_m);
+                break;
+            default:
+                regf_body_jl<64>(d.js, d.tgt, d.nt, jspan, d.out, cap, 1, w.tblock, 0, s_pos,
+                                 s_m);
+                break;
+        }
     }
 }
 
@@ -564,6 +644,20 @@ hipError_t launch_finalize(const QueryOut &part, int S, int nt, int cap, double 
     const size_t lds = 4 * (size_t)(S + 1) * sizeof(int);
     hipLaunchKernelGGL(finalize_kernel, grid, dim3(256), lds, st, part, S, nt, cap, acc, adot, nnb,
                        list);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_reset(int *counters, hipStream_t st)
+{
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, st, counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch(const ClusterDesc *cd, const WorkItem *items, const BatchHeader *hdr,
+                        int *counter, int cap, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(kBlock), 0, st, cd, items, hdr, counter,
+                       cap);
     return hipGetLastError();
 }
 

@@ -9,8 +9,7 @@
 // status codes; device buffers are sized from the actual N and targets
 // (grow-only) instead of next-pow2 staging of GridDimY*NNB_per_block*N ints;
 // only the populated columns of the neighbour lists cross PCIe.
-#include "../../include/abyss_regf.h"
-#include "regf_internal.h"
+#include "regf_context.h"
 
 #include <math.h>
 #include <stdarg.h>
@@ -22,52 +21,6 @@
 using abyss::JState;
 using abyss::Plan;
 using abyss::QueryOut;
-
-struct abyss_ctx_s {
-    int device = -1;
-    hipStream_t stream = nullptr;
-    int cu_count = 0;
-    int cap = ABYSS_NUM_NEIGHBOR_MAX;
-    int precision = ABYSS_PREC_SPLIT;
-    int force_ipl = 0, force_splits = 0, force_unroll = 0, force_jl = 0;
-    unsigned long long scratch_limit = 1ull << 30;
-
-    // j-set: device SoA [x|y|z|vx|vy|vz|m|rnb2] (stride n) + FP32 mass
-    int n = 0;
-    size_t j_bytes = 0;
-    char *d_j = nullptr;
-    char *h_j = nullptr;  // pinned staging, same layout
-    JState js{};
-
-    // targets and outputs
-    int nt = 0;
-    size_t t_capacity = 0;  // targets the output buffers can hold
-    int *d_tgt = nullptr, *h_tgt = nullptr;
-    double *d_acc = nullptr, *d_adot = nullptr;
-    int *d_nnb = nullptr, *d_list = nullptr;
-    int *d_stats = nullptr;
-    double *h_res = nullptr;  // pinned [nt][6] (acc then adot)
-    int *h_nnb = nullptr;
-    int *h_list = nullptr;    // pinned, rows of width <= cap
-    size_t h_list_bytes = 0;
-    int *h_stats = nullptr;
-
-    // split-j scratch
-    size_t scratch_bytes = 0;
-    char *d_scratch = nullptr;
-    QueryOut part{};
-
-    Plan plan{};
-    bool plan_valid = false;
-
-    // timing / counters (ProfileDevice)
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool timing_pending = false;
-    long long n_send = 0, n_grav = 0, sum_ni = 0;
-    double interactions = 0.0, t_grav_ms = 0.0;
-
-    char detail[512] = {0};
-};
 
 namespace {
 
@@ -246,7 +199,7 @@ void release_all(abyss_ctx_t c)
     hfree(c->h_tgt); hfree(c->h_res); hfree(c->h_nnb); hfree(c->h_list); hfree(c->h_stats);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     c->ev0 = c->ev1 = nullptr;
     c->stream = nullptr;
 }
@@ -284,7 +237,11 @@ const char *abyss_error_string(int code)
     }
 }
 
-int abyss_ctx_create(int device, int cap, abyss_ctx_t *out)
+}  // extern "C"
+
+namespace abyss {
+
+int ctx_create_on_stream(int device, int cap, hipStream_t stream, abyss_ctx_t *out)
 {
     if (!out || cap <= 0) return ABYSS_ERR_INVALID_ARG;
     *out = nullptr;
@@ -298,7 +255,10 @@ int abyss_ctx_create(int device, int cap, abyss_ctx_t *out)
     int rc = ABYSS_OK;
     do {
         if (hipSetDevice(device) != hipSuccess) { rc = ABYSS_ERR_NO_DEVICE; break; }
-        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        if (stream) {
+            c->stream = stream;
+            c->own_stream = false;
+        } else if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
             rc = ABYSS_ERR_HIP; break;
         }
         if (hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, device) !=
@@ -318,6 +278,23 @@ int abyss_ctx_create(int device, int cap, abyss_ctx_t *out)
     }
     *out = c;
     return ABYSS_OK;
+}
+
+unsigned long long ctx_device_bytes(abyss_ctx_t c)
+{
+    if (!c) return 0;
+    return (unsigned long long)c->j_bytes + c->scratch_bytes +
+           (unsigned long long)c->t_capacity * (sizeof(int) * 2 + 6 * sizeof(double) +
+                                                (size_t)c->cap * sizeof(int));
+}
+
+}  // namespace abyss
+
+extern "C" {
+
+int abyss_ctx_create(int device, int cap, abyss_ctx_t *out)
+{
+    return abyss::ctx_create_on_stream(device, cap, nullptr, out);
 }
 
 int abyss_ctx_destroy(abyss_ctx_t c)

@@ -45,6 +45,12 @@ EXPORTS = [
     "abyss_ctx_set_scratch_limit", "abyss_ctx_upload", "abyss_ctx_query",
     "abyss_ctx_set_targets", "abyss_ctx_launch", "abyss_ctx_sync", "abyss_ctx_fetch",
     "abyss_ctx_stream", "abyss_ctx_plan", "abyss_ctx_error_detail",
+    "abyss_ens_create", "abyss_ens_destroy", "abyss_ens_upload", "abyss_ens_release",
+    "abyss_ens_set_targets", "abyss_ens_launch", "abyss_ens_sync", "abyss_ens_fetch",
+    "abyss_ens_stream", "abyss_ens_device_bytes", "abyss_ens_error_detail",
+    "abyss_srv_create", "abyss_srv_destroy", "abyss_srv_add_cluster", "abyss_srv_remove_cluster",
+    "abyss_srv_submit", "abyss_srv_wait", "abyss_srv_stats", "abyss_srv_reset_stats",
+    "abyss_srv_cluster_footprint",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -97,6 +103,34 @@ def load():
         "abyss_ctx_stream": (_vp, [_vp]),
         "abyss_ctx_plan": (ctypes.c_int, [_vp, _ip, _ip, _ip, _ip, _ip, _ip]),
         "abyss_ctx_error_detail": (ctypes.c_char_p, [_vp]),
+        "abyss_ens_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(_vp)]),
+        "abyss_ens_destroy": (ctypes.c_int, [_vp]),
+        "abyss_ens_upload": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]),
+        "abyss_ens_release": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "abyss_ens_set_targets": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip]),
+        "abyss_ens_launch": (ctypes.c_int, [_vp]),
+        "abyss_ens_sync": (ctypes.c_int, [_vp]),
+        "abyss_ens_fetch": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _ip, _ip]),
+        "abyss_ens_stream": (_vp, [_vp]),
+        "abyss_ens_device_bytes": (ctypes.c_ulonglong, [_vp]),
+        "abyss_ens_error_detail": (ctypes.c_char_p, [_vp]),
+        "abyss_srv_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_ulonglong, ctypes.POINTER(_vp)]),
+        "abyss_srv_destroy": (ctypes.c_int, [_vp]),
+        "abyss_srv_add_cluster": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]),
+        "abyss_srv_remove_cluster": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "abyss_srv_submit": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _ip,
+                                            ctypes.POINTER(ctypes.c_longlong)]),
+        "abyss_srv_wait": (ctypes.c_int, [_vp, ctypes.c_longlong, _dp, _dp, _ip, _ip,
+                                          ctypes.POINTER(ctypes.c_double)]),
+        "abyss_srv_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong),
+                                           ctypes.POINTER(ctypes.c_longlong),
+                                           ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_double)]),
+        "abyss_srv_reset_stats": (ctypes.c_int, [_vp]),
+        "abyss_srv_cluster_footprint": (ctypes.c_ulonglong, [_vp, ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -313,3 +347,168 @@ class Context:
     @property
     def stream(self):
         return self._lib.abyss_ctx_stream(self.h)
+
+
+# --------------------------------------------------------------------------
+# Batched ensemble and served ensemble
+# --------------------------------------------------------------------------
+class Ensemble:
+    """Up to max_clusters device-resident clusters evaluated by one batched
+    launch (include/abyss_regf.h, Part 3)."""
+
+    def __init__(self, device=0, cap=256, max_clusters=256):
+        self._lib = load()
+        h = _vp()
+        rc = self._lib.abyss_ens_create(device, cap, max_clusters, ctypes.byref(h))
+        if rc != OK:
+            raise AbyssError(rc)
+        self.h = h
+        self.cap = cap
+        self.max_clusters = max_clusters
+        self.nt = {}
+
+    def _chk(self, rc, allow_overflow=True):
+        if rc == OK or (allow_overflow and rc == ERR_NEIGHBOR_OVERFLOW):
+            return rc
+        raise AbyssError(rc, self._lib.abyss_ens_error_detail(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self._lib.abyss_ens_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, cluster, m, x, v, r2):
+        x = _f64(x, (-1, 3))
+        n = x.shape[0]
+        self._chk(self._lib.abyss_ens_upload(self.h, cluster, n, _d(_f64(m, (n,))), _d(x),
+                                             _d(_f64(v, (n, 3))), _d(_f64(r2, (n,)))))
+
+    def release(self, cluster):
+        self._chk(self._lib.abyss_ens_release(self.h, cluster))
+        self.nt.pop(cluster, None)
+
+    def set_targets(self, cluster, targets):
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        self._chk(self._lib.abyss_ens_set_targets(self.h, cluster, t.size,
+                                                  _i(t) if t.size else None))
+        if t.size:
+            self.nt[cluster] = t.size
+        else:
+            self.nt.pop(cluster, None)
+
+    def launch(self):
+        self._chk(self._lib.abyss_ens_launch(self.h))
+
+    def sync(self):
+        self._chk(self._lib.abyss_ens_sync(self.h))
+
+    def fetch(self, cluster):
+        nt = self.nt[cluster]
+        acc = np.zeros((nt, 3))
+        adot = np.zeros((nt, 3))
+        nnb = np.zeros(nt, dtype=np.int32)
+        lst = np.full((nt, self.cap), -1, dtype=np.int32)
+        self._chk(self._lib.abyss_ens_fetch(self.h, cluster, _d(acc), _d(adot), _i(nnb),
+                                            _i(lst)))
+        return acc, adot, nnb, lst
+
+    @property
+    def stream(self):
+        return self._lib.abyss_ens_stream(self.h)
+
+    def device_bytes(self):
+        return self._lib.abyss_ens_device_bytes(self.h)
+
+
+class Server:
+    """Served ensemble: resident clusters, requests from any thread,
+    continuous batching by a native worker thread, admission by HBM budget."""
+
+    def __init__(self, device=0, cap=256, max_clusters=64, hbm_budget=64 << 30):
+        self._lib = load()
+        h = _vp()
+        rc = self._lib.abyss_srv_create(device, cap, max_clusters, int(hbm_budget),
+                                        ctypes.byref(h))
+        if rc != OK:
+            raise AbyssError(rc)
+        self.h = h
+        self.cap = cap
+
+    def close(self):
+        if self.h:
+            self._lib.abyss_srv_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def footprint(self, n):
+        return self._lib.abyss_srv_cluster_footprint(self.h, n)
+
+    def add_cluster(self, m, x, v, r2):
+        x = _f64(x, (-1, 3))
+        n = x.shape[0]
+        cid = ctypes.c_int(-1)
+        rc = self._lib.abyss_srv_add_cluster(self.h, n, _d(_f64(m, (n,))), _d(x),
+                                             _d(_f64(v, (n, 3))), _d(_f64(r2, (n,))),
+                                             ctypes.byref(cid))
+        if rc != OK:
+            raise AbyssError(rc)
+        return cid.value
+
+    def remove_cluster(self, cluster):
+        rc = self._lib.abyss_srv_remove_cluster(self.h, cluster)
+        if rc != OK:
+            raise AbyssError(rc)
+
+    def submit(self, cluster, targets):
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        tk = ctypes.c_longlong()
+        rc = self._lib.abyss_srv_submit(self.h, cluster, t.size, _i(t), ctypes.byref(tk))
+        if rc != OK:
+            raise AbyssError(rc)
+        return tk.value, t.size
+
+    def wait(self, ticket, nt):
+        acc = np.zeros((nt, 3))
+        adot = np.zeros((nt, 3))
+        nnb = np.zeros(nt, dtype=np.int32)
+        lst = np.full((nt, self.cap), -1, dtype=np.int32)
+        lat = ctypes.c_double()
+        rc = self._lib.abyss_srv_wait(self.h, ticket, _d(acc), _d(adot), _i(nnb), _i(lst),
+                                      ctypes.byref(lat))
+        if rc not in (OK, ERR_NEIGHBOR_OVERFLOW):
+            raise AbyssError(rc)
+        return acc, adot, nnb, lst, lat.value
+
+    def stats(self):
+        c, b = ctypes.c_longlong(), ctypes.c_longlong()
+        i, p50, p99 = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self._lib.abyss_srv_stats(self.h, ctypes.byref(c), ctypes.byref(b), ctypes.byref(i),
+                                  ctypes.byref(p50), ctypes.byref(p99))
+        return {"completed": c.value, "batches": b.value, "interactions": i.value,
+                "p50_ms": p50.value, "p99_ms": p99.value}
+
+    def reset_stats(self):
+        self._lib.abyss_srv_reset_stats(self.h)

@@ -141,6 +141,121 @@ def accuracy_sample(js, res, nsample=256):
             "list_band_pairs_flipped": lists["band_pairs_flipped"]}
 
 
+def ensemble_workload(count=256, lo=1024, hi=65536, seed=1, k_nb=64):
+    """256 Plummer clusters, N log-uniform in [lo, hi], distinct seeds, active
+    fraction uniform in [0.01, 1]; Kroupa masses; r_nb for ~k_nb neighbours."""
+    import numpy as np
+    from abyss_amd import ic
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(int)
+    fr = rng.uniform(0.01, 1.0, count)
+    out = []
+    for k in range(count):
+        n = int(ns[k])
+        r = np.random.default_rng(seed * 1000 + k)
+        x, v, m = ic.plummer(n, seed=seed * 1000 + k, masses=ic.kroupa_masses(n, r))
+        r2 = ic.neighbour_radius2(x, k_nb)
+        t = r.permutation(n)[:max(1, int(round(fr[k] * n)))].astype(np.int32)
+        out.append((x, v, m, r2, t))
+    return out
+
+
+def bench_ensemble(args):
+    """Batched ensemble vs per-cluster launches (device-resident and legacy)."""
+    import numpy as np
+    from abyss_amd import regf
+    t0 = time.perf_counter()
+    cl = ensemble_workload()
+    log(f"ensemble workload: {len(cl)} clusters, sum N = {sum(len(c[2]) for c in cl)}, "
+        f"built in {time.perf_counter() - t0:.1f} s")
+    inter = float(sum(len(c[4]) * (len(c[2]) - 1) for c in cl))
+    cap = 256
+    ev = HipEvents()
+    # (1) batched: one persistent graph launch per step
+    ens = regf.Ensemble(device=0, cap=cap, max_clusters=len(cl))
+    for k, (x, v, m, r2, t) in enumerate(cl):
+        ens.upload(k, m, x, v, r2)
+        ens.set_targets(k, t)
+    for _ in range(args.warmup):
+        ens.launch()
+    ens.sync()
+    e0, e1 = ev.create(), ev.create()
+    ev.record(e0, ens.stream)
+    for _ in range(args.steps):
+        ens.launch()
+    ev.record(e1, ens.stream)
+    ms_batched = ev.elapsed_ms(e0, e1) / args.steps
+    ens.sync()
+    w0 = time.perf_counter()
+    ens.launch()
+    ens.sync()
+    res_b = [ens.fetch(k) for k in range(len(cl))]
+    ms_batched_fetch = (time.perf_counter() - w0) * 1e3
+    log(f"batched: {ms_batched:.3f} ms/step on device, {ms_batched_fetch:.1f} ms incl. fetch")
+    ens.close()
+    # (2) per-cluster device-resident launches, one after another
+    ctxs = []
+    for (x, v, m, r2, t) in cl:
+        c = regf.Context(device=0, cap=cap)
+        c.upload(m, x, v, r2)
+        c.set_targets(t)
+        ctxs.append(c)
+    for c in ctxs:
+        c.launch()
+        c.sync()
+    times = []
+    for _ in range(max(1, args.steps // 2)):
+        w0 = time.perf_counter()
+        for c in ctxs:
+            c.launch()
+            c.sync()
+        times.append((time.perf_counter() - w0) * 1e3)
+    ms_plain = float(np.median(times))
+    log(f"per-cluster device-resident launches: {ms_plain:.3f} ms/step")
+    for c in ctxs:
+        c.close()
+    # (3) per-cluster legacy ABI calls (upload + query + results on the host)
+    regf.set_num_neighbor_max(cap)
+    regf.OpenDevice(0)
+    w0 = time.perf_counter()
+    for (x, v, m, r2, t) in cl:
+        regf.SendToDevice(m, x, v, r2)
+        regf.CalculateAccelerationOnDevice(t)
+    ms_legacy = (time.perf_counter() - w0) * 1e3
+    regf.CloseDevice()
+    regf.set_num_neighbor_max(1000)
+    log(f"per-cluster legacy calls: {ms_legacy:.1f} ms/step")
+    # accuracy spot check on a few clusters
+    import oracle
+    from oracle.compare import compare_lists, force_errors
+    worst = 0.0
+    hard = 0
+    for k in range(0, len(cl), 32):
+        x, v, m, r2, t = cl[k]
+        s = t[:128]
+        ra, rj, rn, rl, sa, sj = oracle.regf(x, v, m, r2, s, cap, scales=True)
+        sel = np.arange(len(s))
+        err = force_errors(res_b[k][0][sel], res_b[k][1][sel], ra, rj, sa, sj)
+        worst = max(worst, err["a_norm"]["max"], err["jdot_norm"]["max"])
+        hard += compare_lists(res_b[k][2][sel], res_b[k][3][sel], rn, rl, x, r2, s)["hard_mismatches"]
+    line = {
+        "metric": "ensemble regular-force pair-interactions/s",
+        "value": inter / (ms_batched * 1e-3), "unit": "interactions/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_batched,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (f64 i-j differences and per-particle sums)", "data": "synthetic",
+        "config": {"workload": "ensemble_256_plummer_n1024-65536_active0.01-1",
+                   "clusters": len(cl), "interactions_per_step": inter, "cap": cap},
+        "per_cluster_device_launches_ms": ms_plain,
+        "per_cluster_legacy_calls_ms": ms_legacy,
+        "batched_with_fetch_ms": ms_batched_fetch,
+        "speedup_vs_device_launches": ms_plain / ms_batched,
+        "speedup_vs_legacy_calls": ms_legacy / ms_batched_fetch,
+        "accuracy": {"worst_normalised_error": worst, "list_hard_mismatches": hard},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,9 +270,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration)")
+    ap.add_argument("--workload", choices=["king262144", "ensemble"], default="king262144")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.workload == "ensemble":
+        bench_ensemble(args)
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

@@ -165,6 +165,69 @@ int abyss_ctx_plan(abyss_ctx_t ctx, int *i_per_lane, int *j_splits,
 /* Detail string of the context's last error. */
 const char *abyss_ctx_error_detail(abyss_ctx_t ctx);
 
+/* ===================================================================== */
+/* Part 3: batched ensembles and the served ensemble.                   */
+/* No counterpart in the reference (one cluster per process there).     */
+/* ===================================================================== */
+
+/* An ensemble holds up to max_clusters independent clusters, each with its
+ * own device-resident FP64 j-set.  abyss_ens_launch evaluates every cluster
+ * that has targets set, in ONE persistent kernel launch replayed from a
+ * captured HIP graph (work items of all clusters share a device queue).  A
+ * cluster's decomposition depends only on its own N and target count, so
+ * its results are bitwise identical alone and in any mix. */
+typedef struct abyss_ens_s *abyss_ens_t;
+
+int abyss_ens_create(int device, int cap, int max_clusters, abyss_ens_t *out);
+int abyss_ens_destroy(abyss_ens_t ens);
+/* (Re)load cluster slot `cluster` (0 <= cluster < max_clusters). */
+int abyss_ens_upload(abyss_ens_t ens, int cluster, int n, const double *m, const double *x,
+                     const double *v, const double *r2);
+/* Frees a cluster slot and its device memory. */
+int abyss_ens_release(abyss_ens_t ens, int cluster);
+/* Targets of a cluster for the next launch; nt = 0 leaves it out. */
+int abyss_ens_set_targets(abyss_ens_t ens, int cluster, int nt, const int *targets);
+/* One batched step over all clusters with targets (asynchronous). */
+int abyss_ens_launch(abyss_ens_t ens);
+int abyss_ens_sync(abyss_ens_t ens);
+/* Results of a cluster from the last launch (list row stride = cap). */
+int abyss_ens_fetch(abyss_ens_t ens, int cluster, double *acc, double *adot, int *nnb,
+                    int *list);
+void *abyss_ens_stream(abyss_ens_t ens);
+/* Device bytes held by the ensemble (all clusters + descriptors). */
+unsigned long long abyss_ens_device_bytes(abyss_ens_t ens);
+const char *abyss_ens_error_detail(abyss_ens_t ens);
+
+/* Served ensemble: clusters stay resident; requests (one regular step of
+ * one cluster) are submitted from any host thread and completed by a worker
+ * thread that continuously batches every pending request (at most one per
+ * cluster per batch) into the next ensemble launch.  New clusters are
+ * admitted while their estimated HBM footprint fits the budget. */
+typedef struct abyss_srv_s *abyss_srv_t;
+
+int abyss_srv_create(int device, int cap, int max_clusters, unsigned long long hbm_budget,
+                     abyss_srv_t *out);
+int abyss_srv_destroy(abyss_srv_t srv);
+/* Admits a cluster (ABYSS_ERR_ADMISSION if it does not fit the budget,
+ * ABYSS_ERR_BUSY if no slot is free); returns its id in *cluster. */
+int abyss_srv_add_cluster(abyss_srv_t srv, int n, const double *m, const double *x,
+                          const double *v, const double *r2, int *cluster);
+int abyss_srv_remove_cluster(abyss_srv_t srv, int cluster);
+/* Queues one regular step of `cluster` for `nt` targets (copied). */
+int abyss_srv_submit(abyss_srv_t srv, int cluster, int nt, const int *targets,
+                     long long *ticket);
+/* Blocks until the request is done; copies its results (list row stride =
+ * cap) and its latency (submit -> results on the host) in milliseconds. */
+int abyss_srv_wait(abyss_srv_t srv, long long ticket, double *acc, double *adot, int *nnb,
+                   int *list, double *latency_ms);
+/* Completed requests, batches, summed interactions and latency percentiles
+ * (p50, p99; ms) since creation or the last reset. */
+int abyss_srv_stats(abyss_srv_t srv, long long *completed, long long *batches,
+                    double *interactions, double *p50_ms, double *p99_ms);
+int abyss_srv_reset_stats(abyss_srv_t srv);
+/* Estimated device bytes a cluster of n particles needs (admission rule). */
+unsigned long long abyss_srv_cluster_footprint(abyss_srv_t srv, int n);
+
 #ifdef __cplusplus
 }
 #endif

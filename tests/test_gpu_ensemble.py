@@ -1,0 +1,170 @@
+"""GPU tests of the batched ensemble and the served ensemble.
+
+* every cluster's result in a batch is bitwise identical to the same cluster
+  run alone (its decomposition depends only on its own N and target count),
+  and meets the oracle bars of test_gpu_parity;
+* the captured graph replays correctly with eager work between replays and a
+  different mix each time;
+* the server completes concurrently submitted requests with the same bits,
+  refuses admission beyond its HBM budget, and reports latency percentiles.
+"""
+import sys
+import threading
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle.compare import compare_lists, force_errors
+from abyss_amd import ic, regf
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_clusters(sizes, seed0=100, k_nb=32):
+    out = []
+    for k, n in enumerate(sizes):
+        rng = np.random.default_rng(seed0 + k)
+        if k % 3 == 2:
+            x, v, m = ic.king(n, W0=3.0 + 6.0 * rng.uniform(), seed=seed0 + k,
+                              masses=ic.kroupa_masses(n, rng))
+        else:
+            x, v, m = ic.plummer(n, seed=seed0 + k, masses=ic.kroupa_masses(n, rng))
+        r2 = ic.neighbour_radius2(x, min(k_nb, n - 2))
+        out.append((x, v, m, r2))
+    return out
+
+
+def random_targets(n, seed):
+    rng = np.random.default_rng(seed)
+    size = int(np.exp(rng.uniform(0.0, np.log(n + 1))))
+    size = max(1, min(n, size))
+    return rng.permutation(n)[:size].astype(np.int32)
+
+
+SIZES = [300, 1024, 2000, 257, 5000, 9000, 20000, 700, 3000, 12000, 65536, 1500]
+
+
+@pytest.fixture(scope="module")
+def clusters():
+    return make_clusters(SIZES)
+
+
+def check_vs_oracle(cl, t, res, label):
+    x, v, m, r2 = cl
+    acc, adot, nnb, lst = res
+    ra, rj, rn, rl, sa, sj = oracle.regf(x, v, m, r2, t, lst.shape[1], scales=True)
+    err = force_errors(acc, adot, ra, rj, sa, sj)
+    lists = compare_lists(nnb, lst, rn, rl, x, r2, t)
+    assert lists["hard_mismatches"] == 0 and lists["not_ascending"] == 0, (label, lists)
+    assert err["a_norm"]["max"] <= 1e-6 and err["jdot_norm"]["max"] <= 1e-6, (label, err)
+    assert err["a_rel"]["p99"] <= 1e-6, (label, err)
+
+
+def test_ensemble_matches_alone_bitwise(clusters):
+    say("case: ensemble mix vs alone")
+    with regf.Ensemble(device=0, cap=128, max_clusters=len(clusters)) as ens:
+        for c, (x, v, m, r2) in enumerate(clusters):
+            ens.upload(c, m, x, v, r2)
+        targets = [random_targets(len(cl[2]), 7 + c) for c, cl in enumerate(clusters)]
+        for c, t in enumerate(targets):
+            ens.set_targets(c, t)
+        ens.launch()
+        ens.sync()
+        mixed = [ens.fetch(c) for c in range(len(clusters))]
+        for c, t in enumerate(targets):
+            for k in range(len(clusters)):
+                ens.set_targets(k, t if k == c else [])
+            ens.launch()
+            ens.sync()
+            alone = ens.fetch(c)
+            for a, b in zip(alone, mixed[c]):
+                assert np.array_equal(a, b), f"cluster {c} differs alone vs mixed"
+            check_vs_oracle(clusters[c], t, mixed[c], f"cluster{c}")
+            say(f"  cluster {c}: n={len(clusters[c][2])} nt={len(t)} identical alone/mixed")
+
+
+def test_ensemble_graph_replays_with_changing_mix(clusters):
+    say("case: ensemble graph replays")
+    rng = np.random.default_rng(3)
+    with regf.Ensemble(device=0, cap=128, max_clusters=len(clusters)) as ens:
+        for c, (x, v, m, r2) in enumerate(clusters):
+            ens.upload(c, m, x, v, r2)
+        for rep in range(4):
+            chosen = sorted(rng.permutation(len(clusters))[:5].tolist())
+            tg = {}
+            for c in range(len(clusters)):
+                if c in chosen:
+                    tg[c] = random_targets(len(clusters[c][2]), 1000 * rep + c)
+                    ens.set_targets(c, tg[c])
+                else:
+                    ens.set_targets(c, [])
+            ens.launch()
+            ens.sync()
+            # eager single-cluster work between replays
+            with regf.Context(device=0, cap=128) as ctx:
+                x, v, m, r2 = clusters[0]
+                ctx.upload(m, x, v, r2)
+                ctx.query(np.arange(64, dtype=np.int32))
+            for c in chosen:
+                check_vs_oracle(clusters[c], tg[c], ens.fetch(c), f"rep{rep}/c{c}")
+            say(f"  replay {rep}: clusters {chosen} ok")
+
+
+def test_server_concurrent_requests_match_alone(clusters):
+    say("case: server concurrent requests")
+    sub = clusters[:8]
+    with regf.Server(device=0, cap=128, max_clusters=8, hbm_budget=8 << 30) as srv:
+        ids = [srv.add_cluster(m, x, v, r2) for (x, v, m, r2) in sub]
+        reqs = [(ids[c % 8], random_targets(len(sub[c % 8][2]), 500 + c)) for c in range(40)]
+        results = [None] * len(reqs)
+
+        def client(lo, hi):
+            for q in range(lo, hi):
+                tk, nt = srv.submit(*reqs[q])
+                results[q] = srv.wait(tk, nt)
+
+        th = [threading.Thread(target=client, args=(k * 10, k * 10 + 10)) for k in range(4)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        st = srv.stats()
+        say(f"  stats {st}")
+        assert st["completed"] == len(reqs)
+        assert st["p50_ms"] > 0 and st["p99_ms"] >= st["p50_ms"]
+    with regf.Ensemble(device=0, cap=128, max_clusters=8) as ens:
+        for c, (x, v, m, r2) in enumerate(sub):
+            ens.upload(c, m, x, v, r2)
+        for q, (cid, t) in enumerate(reqs):
+            c = ids.index(cid)
+            for k in range(8):
+                ens.set_targets(k, t if k == c else [])
+            ens.launch()
+            ens.sync()
+            alone = ens.fetch(c)
+            for a, b in zip(alone, results[q][:4]):
+                assert np.array_equal(a, b), f"request {q} differs from the cluster alone"
+
+
+def test_server_admission_by_budget(clusters):
+    say("case: server admission")
+    x, v, m, r2 = clusters[6]             # n = 20000
+    probe = regf.Server(device=0, cap=128, max_clusters=4, hbm_budget=1 << 40)
+    need = probe.footprint(len(m))
+    probe.close()
+    with regf.Server(device=0, cap=128, max_clusters=4, hbm_budget=int(need * 1.5)) as srv:
+        a = srv.add_cluster(m, x, v, r2)
+        with pytest.raises(regf.AbyssError) as e:
+            srv.add_cluster(m, x, v, r2)
+        assert e.value.code == regf.ERR_ADMISSION
+        srv.remove_cluster(a)
+        b = srv.add_cluster(m, x, v, r2)
+        tk, nt = srv.submit(b, np.arange(10, dtype=np.int32))
+        res = srv.wait(tk, nt)
+        check_vs_oracle(clusters[6], np.arange(10, dtype=np.int32), res[:4], "admitted")
