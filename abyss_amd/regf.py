@@ -414,12 +414,15 @@ class Ensemble:
     def sync(self):
         self._chk(self._lib.abyss_ens_sync(self.h))
 
-    def fetch(self, cluster):
+    def fetch(self, cluster, fill=True):
+        """Results of `cluster`; list entries past each count are -1 with
+        fill=True and left uninitialised (faster) with fill=False."""
         nt = self.nt[cluster]
-        acc = np.zeros((nt, 3))
-        adot = np.zeros((nt, 3))
-        nnb = np.zeros(nt, dtype=np.int32)
-        lst = np.full((nt, self.cap), -1, dtype=np.int32)
+        acc = np.empty((nt, 3))
+        adot = np.empty((nt, 3))
+        nnb = np.empty(nt, dtype=np.int32)
+        lst = (np.full((nt, self.cap), -1, dtype=np.int32) if fill
+               else np.empty((nt, self.cap), dtype=np.int32))
         self._chk(self._lib.abyss_ens_fetch(self.h, cluster, _d(acc), _d(adot), _i(nnb),
                                             _i(lst)))
         return acc, adot, nnb, lst
@@ -490,12 +493,20 @@ class Server:
             raise AbyssError(rc)
         return tk.value, t.size
 
-    def wait(self, ticket, nt):
+    def wait(self, ticket, nt, copy=True):
+        """Blocks until the request is done.  Returns (acc, adot, nnb, lists,
+        latency_ms); with copy=False only the latency (results discarded)."""
+        lat = ctypes.c_double()
+        if not copy:
+            rc = self._lib.abyss_srv_wait(self.h, ticket, None, None, None, None,
+                                          ctypes.byref(lat))
+            if rc not in (OK, ERR_NEIGHBOR_OVERFLOW):
+                raise AbyssError(rc)
+            return lat.value
         acc = np.zeros((nt, 3))
         adot = np.zeros((nt, 3))
         nnb = np.zeros(nt, dtype=np.int32)
         lst = np.full((nt, self.cap), -1, dtype=np.int32)
-        lat = ctypes.c_double()
         rc = self._lib.abyss_srv_wait(self.h, ticket, _d(acc), _d(adot), _i(nnb), _i(lst),
                                       ctypes.byref(lat))
         if rc not in (OK, ERR_NEIGHBOR_OVERFLOW):

@@ -189,7 +189,7 @@ def bench_ensemble(args):
     w0 = time.perf_counter()
     ens.launch()
     ens.sync()
-    res_b = [ens.fetch(k) for k in range(len(cl))]
+    res_b = [ens.fetch(k, fill=False) for k in range(len(cl))]
     ms_batched_fetch = (time.perf_counter() - w0) * 1e3
     log(f"batched: {ms_batched:.3f} ms/step on device, {ms_batched_fetch:.1f} ms incl. fetch")
     ens.close()
@@ -256,6 +256,98 @@ def bench_ensemble(args):
     print(json.dumps(line), flush=True)
 
 
+def served_workload(count=64, lo=4096, hi=1048576, seed=5, k_nb=32):
+    """64 clusters, N log-uniform in [lo, hi], Plummer or King (W0 uniform in
+    [3, 9]), Kroupa masses, r_nb for ~k_nb neighbours."""
+    import numpy as np
+    from abyss_amd import ic
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(int)
+    out = []
+    for k in range(count):
+        n = int(ns[k])
+        r = np.random.default_rng(seed * 1000 + k)
+        masses = ic.kroupa_masses(n, r)
+        if k % 2:
+            x, v, m = ic.king(n, W0=float(r.uniform(3.0, 9.0)), seed=seed * 1000 + k,
+                              masses=masses)
+        else:
+            x, v, m = ic.plummer(n, seed=seed * 1000 + k, masses=masses)
+        out.append((x, v, m, ic.neighbour_radius2(x, k_nb)))
+    return out
+
+
+def serve_stream(srv, ids, sizes, rate, window_s, seed=9):
+    """Poisson request stream at `rate` req/s for `window_s` seconds; each
+    request is one regular step of a random cluster with an active block
+    log-uniform in [1, N].  Returns (latencies_ms, interactions, wall_s)."""
+    import threading
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    t_arr = np.cumsum(rng.exponential(1.0 / rate, int(rate * window_s * 1.5) + 16))
+    t_arr = t_arr[t_arr < window_s]
+    reqs = []
+    for _ in t_arr:
+        c = int(rng.integers(len(ids)))
+        n = sizes[c]
+        mblk = int(min(n, max(1, np.exp(rng.uniform(0.0, np.log(n + 1))))))
+        reqs.append((ids[c], rng.choice(n, mblk, replace=False).astype(np.int32), n))
+    tickets = [None] * len(reqs)
+    lat = [None] * len(reqs)
+    done_sub = threading.Event()
+
+    def waiter():
+        q = 0
+        while q < len(reqs):
+            while tickets[q] is None:
+                time.sleep(0.0002)
+            lat[q] = srv.wait(tickets[q][0], tickets[q][1], copy=False)
+            q += 1
+
+    wt = threading.Thread(target=waiter)
+    wt.start()
+    t0 = time.perf_counter()
+    for q, (cid, t, n) in enumerate(reqs):
+        delay = t_arr[q] - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        tickets[q] = srv.submit(cid, t)
+    done_sub.set()
+    wt.join()
+    wall = time.perf_counter() - t0
+    inter = float(sum(len(t) * (n - 1) for (_, t, n) in reqs))
+    return np.array(lat), inter, wall
+
+
+def bench_serve(args, rate=2000.0, window_s=1.0):
+    import numpy as np
+    from abyss_amd import regf
+    t0 = time.perf_counter()
+    cl = served_workload()
+    sizes = [len(c[2]) for c in cl]
+    log(f"served workload: {len(cl)} clusters, sum N = {sum(sizes)}, built in "
+        f"{time.perf_counter() - t0:.1f} s")
+    cap = 128
+    srv = regf.Server(device=0, cap=cap, max_clusters=len(cl), hbm_budget=160 << 30)
+    ids = [srv.add_cluster(m, x, v, r2) for (x, v, m, r2) in cl]
+    # warm-up: one small request per cluster
+    for c in ids:
+        tk, nt = srv.submit(c, np.arange(8, dtype=np.int32))
+        srv.wait(tk, nt, copy=False)
+    out = {}
+    for name, r, w in (("nominal", rate, window_s), ("sustained", args.serve_rate, 4.0)):
+        srv.reset_stats()
+        lat, inter, wall = serve_stream(srv, ids, sizes, r, w)
+        st = srv.stats()
+        out[name] = {"rate_req_s": r, "window_s": w, "requests": int(len(lat)),
+                     "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                     "interactions_per_s": inter / wall, "wall_s": wall, "batches": st["batches"],
+                     "mean_batch_size": st["completed"] / max(1, st["batches"])}
+        log(f"serve {name}: {out[name]}")
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,13 +362,20 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration)")
-    ap.add_argument("--workload", choices=["king262144", "ensemble"], default="king262144")
+    ap.add_argument("--workload", choices=["king262144", "ensemble", "serve"],
+                    default="king262144")
+    ap.add_argument("--serve-rate", type=float, default=100.0,
+                    help="sustained request rate of the served-ensemble leg (req/s)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
     if args.workload == "ensemble":
         bench_ensemble(args)
+        return
+    if args.workload == "serve":
+        print(json.dumps({"metric": "served ensemble regular-step latency",
+                          "served_ensemble": bench_serve(args)}), flush=True)
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
