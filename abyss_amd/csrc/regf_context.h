@@ -57,6 +57,10 @@ namespace abyss {
 // Creates a context that issues all its work on `stream` (owned by the
 // caller) instead of a stream of its own.
 int ctx_create_on_stream(int device, int cap, hipStream_t stream, abyss_ctx_t *out);
+// abyss_ctx_fetch with the device->host work issued on `st` (which must not
+// race with a launch writing this context's outputs).
+int ctx_fetch_on_stream(abyss_ctx_t c, hipStream_t st, double *acc, double *adot, int *nnb,
+                        int *list);
 // Device bytes a context holds (j-set, targets, outputs, scratch).
 unsigned long long ctx_device_bytes(abyss_ctx_t c);
 }  // namespace abyss

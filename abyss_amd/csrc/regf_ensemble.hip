@@ -4,17 +4,23 @@
 // on the ensemble's stream.  A launch builds a work list of (cluster, target
 // block) items, copies it with the cluster descriptors to the device, and
 // replays a captured HIP graph of two kernels: a counter reset and the
-// persistent batch_kernel, which drains the queue.  Clusters run unsplit, so
-// no per-cluster scratch or merge pass is needed; the many clusters of a
-// batch fill the chip instead.  Items are queued longest-first (cost = the
-// item's targets x N) so the long items of big clusters start early and the
-// short ones fill in behind them.
+// persistent batch_kernel, which drains the queue, and a persistent finalize
+// that merges the j-split partials.  Each cluster is split in j so that one
+// item is ~4M pairs whatever N is (large clusters would otherwise give items
+// of 1e8+ pairs that no scheduler can balance); split partials live in a
+// scratch pool sized by the batch, not by the clusters.  Items are queued
+// longest-first so the long ones start early and short ones fill in.
 //
-// Served ensemble: a worker thread takes every pending request (at most one
-// per cluster per batch, FIFO), sets the clusters' targets, launches one
-// batch, fetches the results and completes the requests; requests arriving
-// meanwhile form the next batch (continuous batching).  Admission compares a
-// cluster's estimated device footprint with the remaining HBM budget.
+// Served ensemble: time-sliced continuous batching.  A worker thread starts
+// pending requests (one per cluster at a time), and every batch carries at
+// least one target block of each running request -- smallest remaining work
+// first -- plus more blocks until a pair budget (~25 ms of work) is used.  So
+// a small request that arrives while a huge one runs waits at most about one
+// slice instead of the whole huge step.  Finished requests go to a completion
+// thread that copies results to the host on its own stream while the next
+// batch runs.  Splitting a request's target blocks over batches does not
+// change its bits.  Admission compares a cluster's estimated device
+// footprint with the remaining HBM budget.
 #include "regf_context.h"
 
 #include <stdarg.h>
@@ -43,15 +49,20 @@ struct abyss_ens_s {
     int max_clusters = 0;
     std::vector<abyss_ctx_t> cl;     // nullptr: free slot
     std::vector<char> active;        // has targets for the next launch
+    std::vector<int> blk_first, blk_count;   // target-block range (count < 0: all)
 
     ClusterDesc *d_cd = nullptr, *h_cd = nullptr;
     WorkItem *d_items = nullptr, *h_items = nullptr;
     size_t items_cap = 0;
     BatchHeader *d_hdr = nullptr, *h_hdr = nullptr;
     int *d_counter = nullptr;
+    int *d_fin_cl = nullptr, *h_fin_cl = nullptr;     // [max_clusters]
+    int *d_fin_off = nullptr, *h_fin_off = nullptr;   // [max_clusters + 1]
+    char *d_pool = nullptr;                           // split scratch of one batch
+    size_t pool_bytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int grid = 0;
+    int grid = 0, fin_grid = 0;
 
     char detail[512] = {0};
 };
@@ -112,16 +123,32 @@ int ensure_items(abyss_ens_t e, size_t n)
     return ABYSS_OK;
 }
 
+int ensure_pool(abyss_ens_t e, size_t bytes)
+{
+    if (bytes <= e->pool_bytes) return ABYSS_OK;
+    const size_t cap = bytes + bytes / 2;
+    ENSCHK(e, hipStreamSynchronize(e->stream));
+    if (e->d_pool) (void)hipFree(e->d_pool);
+    e->d_pool = nullptr;
+    e->pool_bytes = 0;
+    ENSCHK(e, hipMalloc(&e->d_pool, cap));
+    e->pool_bytes = cap;
+    return ABYSS_OK;
+}
+
 int capture_graph(abyss_ens_t e)
 {
     ENSCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     hipError_t r1 = abyss::launch_batch_reset(e->d_counter, e->stream);
     hipError_t r2 = abyss::launch_batch(e->d_cd, e->d_items, e->d_hdr, e->d_counter, e->cap,
                                         e->grid, e->stream);
+    hipError_t r4 = abyss::launch_batch_finalize(e->d_cd, e->d_fin_cl, e->d_fin_off, e->d_hdr,
+                                                 e->cap, e->fin_grid, e->stream);
     hipGraph_t g = nullptr;
     hipError_t r3 = hipStreamEndCapture(e->stream, &g);
     ENSCHK(e, r1);
     ENSCHK(e, r2);
+    ENSCHK(e, r4);
     ENSCHK(e, r3);
     e->graph = g;
     ENSCHK(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
@@ -146,6 +173,8 @@ int abyss_ens_create(int device, int cap, int max_clusters, abyss_ens_t *out)
     e->max_clusters = max_clusters;
     e->cl.assign(max_clusters, nullptr);
     e->active.assign(max_clusters, 0);
+    e->blk_first.assign(max_clusters, 0);
+    e->blk_count.assign(max_clusters, -1);
     int rc = ABYSS_OK;
     do {
         if (hipSetDevice(device) != hipSuccess) { rc = ABYSS_ERR_NO_DEVICE; break; }
@@ -156,12 +185,19 @@ int abyss_ens_create(int device, int cap, int max_clusters, abyss_ens_t *out)
             hipSuccess) { rc = ABYSS_ERR_HIP; break; }
         // four 256-thread workgroups per CU: the batch kernel's occupancy
         e->grid = std::max(1, e->cu_count) * 4;
+        e->fin_grid = std::max(1, e->cu_count) * 8;
         if (hipMalloc(&e->d_cd, (size_t)max_clusters * sizeof(ClusterDesc)) != hipSuccess ||
             hipHostMalloc(&e->h_cd, (size_t)max_clusters * sizeof(ClusterDesc),
                           hipHostMallocDefault) != hipSuccess ||
             hipMalloc(&e->d_hdr, sizeof(BatchHeader)) != hipSuccess ||
             hipHostMalloc(&e->h_hdr, sizeof(BatchHeader), hipHostMallocDefault) != hipSuccess ||
-            hipMalloc(&e->d_counter, 4 * sizeof(int)) != hipSuccess) {
+            hipMalloc(&e->d_counter, 4 * sizeof(int)) != hipSuccess ||
+            hipMalloc(&e->d_fin_cl, (size_t)max_clusters * sizeof(int)) != hipSuccess ||
+            hipHostMalloc(&e->h_fin_cl, (size_t)max_clusters * sizeof(int),
+                          hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&e->d_fin_off, (size_t)(max_clusters + 1) * sizeof(int)) != hipSuccess ||
+            hipHostMalloc(&e->h_fin_off, (size_t)(max_clusters + 1) * sizeof(int),
+                          hipHostMallocDefault) != hipSuccess) {
             rc = ABYSS_ERR_OUT_OF_MEMORY; break;
         }
         memset(e->h_hdr, 0, sizeof(BatchHeader));
@@ -189,6 +225,11 @@ int abyss_ens_destroy(abyss_ens_t e)
     if (e->d_hdr) (void)hipFree(e->d_hdr);
     if (e->h_hdr) (void)hipHostFree(e->h_hdr);
     if (e->d_counter) (void)hipFree(e->d_counter);
+    if (e->d_fin_cl) (void)hipFree(e->d_fin_cl);
+    if (e->h_fin_cl) (void)hipHostFree(e->h_fin_cl);
+    if (e->d_fin_off) (void)hipFree(e->d_fin_off);
+    if (e->h_fin_off) (void)hipHostFree(e->h_fin_off);
+    if (e->d_pool) (void)hipFree(e->d_pool);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return ABYSS_OK;
@@ -244,26 +285,69 @@ int abyss_ens_set_targets(abyss_ens_t e, int cluster, int nt, const int *targets
     }
     int rc = abyss_ctx_set_targets(c, nt, targets);
     e->active[cluster] = rc == ABYSS_OK ? 1 : 0;
+    e->blk_first[cluster] = 0;
+    e->blk_count[cluster] = -1;
     return from_ctx(e, c, rc);
+}
+
+int abyss_ens_blocks(abyss_ens_t e, int cluster, int *nblocks, int *targets_per_block)
+{
+    if (!e || cluster < 0 || cluster >= e->max_clusters) return ABYSS_ERR_INVALID_ARG;
+    abyss_ctx_t c = e->cl[cluster];
+    if (!c || !e->active[cluster] || !c->plan_valid)
+        return ens_detail(e, ABYSS_ERR_INVALID_ARG, "cluster %d has no targets", cluster);
+    if (nblocks) *nblocks = c->plan.grid_x;
+    if (targets_per_block) *targets_per_block = abyss::targets_per_block(c->plan);
+    return ABYSS_OK;
+}
+
+int abyss_ens_set_block_range(abyss_ens_t e, int cluster, int first, int count)
+{
+    if (!e || cluster < 0 || cluster >= e->max_clusters || first < 0)
+        return ABYSS_ERR_INVALID_ARG;
+    e->blk_first[cluster] = first;
+    e->blk_count[cluster] = count;
+    return ABYSS_OK;
 }
 
 int abyss_ens_launch(abyss_ens_t e)
 {
     if (!e) return ABYSS_ERR_INVALID_ARG;
     ENSCHK(e, hipSetDevice(e->device));
-    // active clusters, longest item first
+    // Per active cluster: its target-block range and a j-split so that one
+    // work item is about kItemPairs pairs whatever N is.  The split depends
+    // only on the cluster's own N and targets per block.
+    constexpr double kItemPairs = 4194304.0;
     struct Entry {
-        int cluster, nblocks;
+        int cluster, first, nblocks, splits, jspan, nt_slice;
         double item_cost;
     };
     std::vector<Entry> act;
-    size_t nitems = 0;
+    size_t nitems = 0, pool = 0;
+    const size_t per_target = (size_t)e->cap * sizeof(int) + 6 * sizeof(double) + sizeof(int);
     for (int k = 0; k < e->max_clusters; ++k) {
         abyss_ctx_t c = e->cl[k];
         if (!c || !e->active[k] || c->nt <= 0 || !c->plan_valid) continue;
         const int tpb = abyss::targets_per_block(c->plan);
-        act.push_back({k, c->plan.grid_x, (double)std::min(tpb, c->nt) * c->n});
-        nitems += c->plan.grid_x;
+        const int first = std::min(e->blk_first[k], c->plan.grid_x);
+        const int avail = c->plan.grid_x - first;
+        const int nb = e->blk_count[k] < 0 ? avail : std::min(e->blk_count[k], avail);
+        if (nb <= 0) continue;
+        const int eff = std::min(tpb, c->nt);
+        const int tile = abyss::kTile;
+        const int nround = ((c->n + tile - 1) / tile) * tile;
+        int jspan = (int)std::ceil(kItemPairs / eff / tile) * tile;
+        jspan = std::max(tile, std::min(jspan, nround));
+        int S = (c->n + jspan - 1) / jspan;
+        if (S > abyss::kMaxSplitsBatch) {
+            S = abyss::kMaxSplitsBatch;
+            jspan = (((c->n + S - 1) / S + tile - 1) / tile) * tile;
+            S = (c->n + jspan - 1) / jspan;
+        }
+        const int nt_slice = std::min(c->nt, (first + nb) * tpb) - first * tpb;
+        act.push_back({k, first, nb, S, jspan, nt_slice, (double)eff * jspan});
+        nitems += (size_t)nb * S;
+        if (S > 1) pool += (size_t)S * nt_slice * per_target + 4 * 256;
     }
     if (act.empty()) return ABYSS_OK;
     std::sort(act.begin(), act.end(), [](const Entry &a, const Entry &b) {
@@ -271,26 +355,66 @@ int abyss_ens_launch(abyss_ens_t e)
     });
     int rc = ensure_items(e, nitems);
     if (rc != ABYSS_OK) return rc;
+    rc = ensure_pool(e, pool);
+    if (rc != ABYSS_OK) return rc;
     // the previous launch may still read the pinned descriptor buffers
     ENSCHK(e, hipStreamSynchronize(e->stream));
     size_t it = 0;
+    char *bump = e->d_pool;
+    auto take = [&](size_t bytes) {
+        char *r = bump;
+        bump += (bytes + 255) & ~size_t(255);
+        return r;
+    };
+    int nfc = 0, nfin = 0;
     for (size_t a = 0; a < act.size(); ++a) {
-        abyss_ctx_t c = e->cl[act[a].cluster];
+        const Entry &en = act[a];
+        abyss_ctx_t c = e->cl[en.cluster];
+        const int tpb = abyss::targets_per_block(c->plan);
         ClusterDesc &d = e->h_cd[a];
         d.js = c->js;
         d.tgt = c->d_tgt;
         d.nt = c->nt;
         d.jl = c->plan.jl;
-        d.out = abyss::QueryOut{c->d_acc, c->d_adot, c->d_nnb, c->d_list};
-        for (int b = 0; b < act[a].nblocks; ++b) e->h_items[it++] = WorkItem{(int)a, b};
+        d.splits = en.splits;
+        d.jspan = en.jspan;
+        d.acc = c->d_acc;
+        d.adot = c->d_adot;
+        d.nnb = c->d_nnb;
+        d.list = c->d_list;
+        if (en.splits > 1) {
+            const size_t m = (size_t)en.splits * en.nt_slice;
+            d.q_base = en.first * tpb;
+            d.nt_stride = en.nt_slice;
+            d.part.acc = reinterpret_cast<double *>(take(m * 3 * sizeof(double)));
+            d.part.adot = reinterpret_cast<double *>(take(m * 3 * sizeof(double)));
+            d.part.cnt = reinterpret_cast<int *>(take(m * sizeof(int)));
+            d.part.list = reinterpret_cast<int *>(take(m * (size_t)e->cap * sizeof(int)));
+            e->h_fin_cl[nfc] = (int)a;
+            e->h_fin_off[nfc] = nfin;
+            ++nfc;
+            nfin += en.nt_slice;
+        } else {
+            d.q_base = 0;
+            d.nt_stride = c->nt;
+            d.part = abyss::QueryOut{nullptr, nullptr, nullptr, nullptr};
+        }
+        for (int b = 0; b < en.nblocks; ++b)
+            for (int sp = 0; sp < en.splits; ++sp)
+                e->h_items[it++] = WorkItem{(int)a, en.first + b, sp, 0};
         c->n_grav += 1;
-        c->sum_ni += c->nt;
-        c->interactions += (double)c->nt * (double)(c->n - 1);
     }
+    e->h_fin_off[nfc] = nfin;
     e->h_hdr->nitems = (int)nitems;
+    e->h_hdr->nfin = nfin;
+    e->h_hdr->nfc = nfc;
     ENSCHK(e, hipMemcpyAsync(e->d_cd, e->h_cd, act.size() * sizeof(ClusterDesc),
                              hipMemcpyHostToDevice, e->stream));
     ENSCHK(e, hipMemcpyAsync(e->d_items, e->h_items, nitems * sizeof(WorkItem),
+                             hipMemcpyHostToDevice, e->stream));
+    ENSCHK(e, hipMemcpyAsync(e->d_fin_cl, e->h_fin_cl, std::max(nfc, 1) * sizeof(int),
+                             hipMemcpyHostToDevice, e->stream));
+    ENSCHK(e, hipMemcpyAsync(e->d_fin_off, e->h_fin_off, (nfc + 1) * sizeof(int),
                              hipMemcpyHostToDevice, e->stream));
     ENSCHK(e, hipMemcpyAsync(e->d_hdr, e->h_hdr, sizeof(BatchHeader), hipMemcpyHostToDevice,
                              e->stream));
@@ -322,15 +446,16 @@ unsigned long long abyss_ens_device_bytes(abyss_ens_t e)
 {
     if (!e) return 0;
     unsigned long long b = (unsigned long long)e->max_clusters * sizeof(ClusterDesc) +
-                           e->items_cap * sizeof(WorkItem);
+                           e->items_cap * sizeof(WorkItem) + e->pool_bytes;
     for (auto c : e->cl) b += abyss::ctx_device_bytes(c);
     return b;
 }
 
 }  // extern "C"
 
+
 // ---------------------------------------------------------------------------
-// Served ensemble
+// Served ensemble: time-sliced continuous batching
 // ---------------------------------------------------------------------------
 namespace {
 
@@ -341,6 +466,9 @@ struct Request {
     int cluster = 0;
     std::vector<int> targets;
     Clock::time_point t_submit;
+    // progress through the cluster's target blocks (worker thread only)
+    int nblocks = 0, next_block = 0;
+    double block_cost = 0.0;   // pairs per block
     bool done = false;
     int status = ABYSS_OK;
     std::vector<double> acc, adot;
@@ -348,23 +476,30 @@ struct Request {
     double latency_ms = 0.0;
 };
 
+using ReqPtr = std::shared_ptr<Request>;
+
 }  // namespace
 
 struct abyss_srv_s {
     abyss_ens_t ens = nullptr;
+    int device = 0;
     int cap = 0;
     int max_clusters = 0;
     unsigned long long budget = 0, used = 0;
+    double batch_pairs = 3.0e10;   // work budget of one batch (pairs)
     std::vector<unsigned long long> footprint;   // 0: free slot
     std::vector<int> nparticles;
+    std::vector<char> busy;        // a request of the cluster is running / being fetched
 
-    std::mutex mu;        // queue, request map, statistics, cluster table
+    std::mutex mu;        // queues, request map, statistics, cluster table
     std::mutex ens_mu;    // every call into the ensemble
-    std::condition_variable cv_work, cv_done;
-    std::deque<std::shared_ptr<Request>> pending;
-    std::unordered_map<long long, std::shared_ptr<Request>> reqs;
-    std::thread worker;
+    std::condition_variable cv_work, cv_done, cv_fetch;
+    std::deque<ReqPtr> pending;    // submitted, not started
+    std::deque<ReqPtr> to_fetch;   // all blocks done (or failed), results not on the host
+    std::unordered_map<long long, ReqPtr> reqs;
+    std::thread worker, completer;
     bool stop = false;
+    bool worker_done = false;   // set by the worker when it has drained everything
     long long next_id = 1;
 
     std::vector<double> latencies;
@@ -374,65 +509,147 @@ struct abyss_srv_s {
 
 namespace {
 
+// Worker: forms batches.  Every running request gets at least one target
+// block per batch; requests with the least remaining work go first, then
+// blocks are handed out until the batch's pair budget is used.  A request
+// whose blocks are all done goes to the completion thread.
 void serve_loop(abyss_srv_t s)
 {
+    std::vector<ReqPtr> running;
     for (;;) {
-        std::vector<std::shared_ptr<Request>> batch;
+        std::vector<ReqPtr> starting;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
-            if (s->stop && s->pending.empty()) return;
-            std::vector<char> taken(s->max_clusters, 0);
+            s->cv_work.wait(lk, [&] {
+                if (!running.empty()) return true;
+                for (auto &r : s->pending)
+                    if (!s->busy[r->cluster]) return true;
+                return s->stop && s->pending.empty();
+            });
+            if (s->stop && s->pending.empty() && running.empty()) {
+                s->worker_done = true;
+                s->cv_fetch.notify_all();
+                return;
+            }
             for (auto it = s->pending.begin(); it != s->pending.end();) {
                 const int c = (*it)->cluster;
-                if (!taken[c]) {
-                    taken[c] = 1;
-                    batch.push_back(*it);
+                if (!s->busy[c]) {
+                    s->busy[c] = 1;
+                    starting.push_back(*it);
                     it = s->pending.erase(it);
                 } else {
                     ++it;
                 }
             }
         }
-        double inter = 0.0;
+        std::vector<ReqPtr> failed;
         {
             std::lock_guard<std::mutex> elk(s->ens_mu);
-            for (auto &r : batch) {
+            for (auto &r : starting) {
                 r->status = abyss_ens_set_targets(s->ens, r->cluster, (int)r->targets.size(),
                                                   r->targets.data());
-            }
-            int rc = abyss_ens_launch(s->ens);
-            if (rc == ABYSS_OK) rc = abyss_ens_sync(s->ens);
-            for (auto &r : batch) {
-                if (r->status == ABYSS_OK && rc != ABYSS_OK) r->status = rc;
-                if (r->status == ABYSS_OK) {
-                    const size_t nt = r->targets.size();
-                    r->acc.resize(3 * nt);
-                    r->adot.resize(3 * nt);
-                    r->nnb.resize(nt);
-                    r->list.resize(nt * (size_t)s->cap);
-                    r->status = abyss_ens_fetch(s->ens, r->cluster, r->acc.data(),
-                                                r->adot.data(), r->nnb.data(), r->list.data());
-                    inter += (double)nt * (double)(s->nparticles[r->cluster] - 1);
+                int tpb = 0;
+                if (r->status == ABYSS_OK)
+                    r->status = abyss_ens_blocks(s->ens, r->cluster, &r->nblocks, &tpb);
+                if (r->status != ABYSS_OK) {
+                    failed.push_back(r);
+                    continue;
                 }
-                abyss_ens_set_targets(s->ens, r->cluster, 0, nullptr);
+                r->block_cost = (double)std::min<size_t>(tpb, r->targets.size()) *
+                                (double)s->nparticles[r->cluster];
+                running.push_back(r);
             }
+            if (!running.empty()) {
+                std::sort(running.begin(), running.end(), [](const ReqPtr &a, const ReqPtr &b) {
+                    const double ra = (a->nblocks - a->next_block) * a->block_cost;
+                    const double rb = (b->nblocks - b->next_block) * b->block_cost;
+                    return ra != rb ? ra < rb : a->id < b->id;
+                });
+                double left = s->batch_pairs;
+                std::vector<int> take(running.size());
+                for (size_t k = 0; k < running.size(); ++k) {
+                    const ReqPtr &r = running[k];
+                    const int remain = r->nblocks - r->next_block;
+                    int nb = 1;
+                    if (left > r->block_cost)
+                        nb = (int)std::min<double>(remain, std::floor(left / r->block_cost));
+                    nb = std::max(1, std::min(nb, remain));
+                    left -= nb * r->block_cost;
+                    take[k] = nb;
+                    abyss_ens_set_block_range(s->ens, r->cluster, r->next_block, nb);
+                }
+                int rc = abyss_ens_launch(s->ens);
+                if (rc == ABYSS_OK) rc = abyss_ens_sync(s->ens);
+                double inter = 0.0;
+                std::vector<ReqPtr> still;
+                for (size_t k = 0; k < running.size(); ++k) {
+                    const ReqPtr &r = running[k];
+                    if (rc != ABYSS_OK) r->status = rc;
+                    r->next_block += take[k];
+                    inter += take[k] * r->block_cost;
+                    if (r->status != ABYSS_OK || r->next_block >= r->nblocks) {
+                        abyss_ens_set_targets(s->ens, r->cluster, 0, nullptr);
+                        failed.push_back(r);   // to the completion thread
+                    } else {
+                        still.push_back(r);
+                    }
+                }
+                running.swap(still);
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->batches += 1;
+                (void)inter;
+            }
+        }
+        if (!failed.empty()) {
+            std::lock_guard<std::mutex> lk(s->mu);
+            for (auto &r : failed) s->to_fetch.push_back(r);
+            s->cv_fetch.notify_one();
+        }
+    }
+}
+
+// Completion thread: copies finished requests' results to the host on its
+// own stream while the worker's next batch runs, then frees the cluster.
+void complete_loop(abyss_srv_t s)
+{
+    (void)hipSetDevice(s->device);
+    hipStream_t st = nullptr;
+    (void)hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    for (;;) {
+        ReqPtr r;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_fetch.wait(lk, [&] { return !s->to_fetch.empty() || s->worker_done; });
+            if (s->to_fetch.empty()) break;   // the worker is done and nothing is left
+            r = s->to_fetch.front();
+            s->to_fetch.pop_front();
+        }
+        double inter = 0.0;
+        if (r->status == ABYSS_OK) {
+            const size_t nt = r->targets.size();
+            r->acc.resize(3 * nt);
+            r->adot.resize(3 * nt);
+            r->nnb.resize(nt);
+            r->list.resize(nt * (size_t)s->cap);
+            abyss_ctx_t c = s->ens->cl[r->cluster];
+            r->status = abyss::ctx_fetch_on_stream(c, st ? st : c->stream, r->acc.data(),
+                                                   r->adot.data(), r->nnb.data(), r->list.data());
+            inter = (double)nt * (double)(s->nparticles[r->cluster] - 1);
         }
         const auto now = Clock::now();
         {
             std::lock_guard<std::mutex> lk(s->mu);
-            for (auto &r : batch) {
-                r->latency_ms =
-                    std::chrono::duration<double, std::milli>(now - r->t_submit).count();
-                r->done = true;
-                s->latencies.push_back(r->latency_ms);
-                s->completed += 1;
-            }
+            r->latency_ms = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
+            r->done = true;
+            s->busy[r->cluster] = 0;
+            s->latencies.push_back(r->latency_ms);
+            s->completed += 1;
             s->interactions += inter;
-            s->batches += 1;
         }
         s->cv_done.notify_all();
+        s->cv_work.notify_one();
     }
+    if (st) (void)hipStreamDestroy(st);
 }
 
 double percentile(std::vector<double> v, double p)
@@ -463,12 +680,15 @@ int abyss_srv_create(int device, int cap, int max_clusters, unsigned long long h
         return ABYSS_ERR_OUT_OF_MEMORY;
     }
     s->ens = e;
+    s->device = device;
     s->cap = cap;
     s->max_clusters = max_clusters;
     s->budget = hbm_budget;
     s->footprint.assign(max_clusters, 0);
     s->nparticles.assign(max_clusters, 0);
+    s->busy.assign(max_clusters, 0);
     s->worker = std::thread(serve_loop, s);
+    s->completer = std::thread(complete_loop, s);
     *out = s;
     return ABYSS_OK;
 }
@@ -481,9 +701,18 @@ int abyss_srv_destroy(abyss_srv_t s)
         s->stop = true;
     }
     s->cv_work.notify_all();
-    if (s->worker.joinable()) s->worker.join();
+    if (s->worker.joinable()) s->worker.join();   // drains every submitted request
+    if (s->completer.joinable()) s->completer.join();
     abyss_ens_destroy(s->ens);
     delete s;
+    return ABYSS_OK;
+}
+
+int abyss_srv_set_batch_budget(abyss_srv_t s, double pairs)
+{
+    if (!s || !(pairs > 0.0)) return ABYSS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> elk(s->ens_mu);
+    s->batch_pairs = pairs;
     return ABYSS_OK;
 }
 
@@ -537,8 +766,10 @@ int abyss_srv_remove_cluster(abyss_srv_t s, int cluster)
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->footprint[cluster] == 0) return ABYSS_ERR_INVALID_ARG;
+        if (s->busy[cluster]) return ABYSS_ERR_BUSY;
         for (auto &r : s->pending)
             if (r->cluster == cluster) return ABYSS_ERR_BUSY;
+        s->footprint[cluster] = ~0ull;   // keep the slot while it is released
     }
     int rc;
     {
@@ -546,7 +777,7 @@ int abyss_srv_remove_cluster(abyss_srv_t s, int cluster)
         rc = abyss_ens_release(s->ens, cluster);
     }
     std::lock_guard<std::mutex> lk(s->mu);
-    s->used -= s->footprint[cluster];
+    s->used -= abyss_srv_cluster_footprint(s, s->nparticles[cluster]);
     s->footprint[cluster] = 0;
     s->nparticles[cluster] = 0;
     return rc;
@@ -561,7 +792,8 @@ int abyss_srv_submit(abyss_srv_t s, int cluster, int nt, const int *targets, lon
     r->targets.assign(targets, targets + nt);
     {
         std::lock_guard<std::mutex> lk(s->mu);
-        if (s->footprint[cluster] == 0) return ABYSS_ERR_NO_JSET;
+        if (s->footprint[cluster] == 0 || s->footprint[cluster] == ~0ull)
+            return ABYSS_ERR_NO_JSET;
         const int n = s->nparticles[cluster];
         if (nt > n) return ABYSS_ERR_INVALID_ARG;
         for (int k = 0; k < nt; ++k)
@@ -580,7 +812,7 @@ int abyss_srv_wait(abyss_srv_t s, long long ticket, double *acc, double *adot, i
                    int *list, double *latency_ms)
 {
     if (!s) return ABYSS_ERR_INVALID_ARG;
-    std::shared_ptr<Request> r;
+    ReqPtr r;
     {
         std::unique_lock<std::mutex> lk(s->mu);
         auto it = s->reqs.find(ticket);

@@ -48,27 +48,37 @@ constexpr int kFold = 64;     // j per FP32 partial before the FP64 fold
 constexpr int kMaxSplits = 2048;  // bound of the finalize kernel's LDS prefix
 
 // ---- batched ensembles ---------------------------------------------------
-// One cluster of a batched launch: its j-set, targets, plan and outputs.
-// Batched clusters run unsplit (one j-range), so `out` holds the final
-// acc/adot/nnb/list arrays.
+// One cluster of a batched launch.  `splits` contiguous j-ranges of
+// `jspan`; with splits == 1 the pairwise bodies write the final outputs
+// (acc/adot/nnb/list) directly, otherwise `part` is this slice's region of
+// the batch scratch, indexed [s][q - q_base] with stride nt_stride, and the
+// batched finalize merges it into the final outputs.
 struct ClusterDesc {
     JState js;
     const int *tgt;
     int nt, jl;
-    QueryOut out;
+    int splits, jspan;
+    int q_base, nt_stride;
+    QueryOut part;
+    double *acc, *adot;
+    int *nnb, *list;
 };
 
-// (cluster, target block): one workgroup's worth of work.
+// (cluster, target block, j-split): one workgroup's worth of work.
 struct WorkItem {
-    int cluster, tblock;
+    int cluster, tblock, split, pad;
 };
 
-// Per-launch sizes, read by the persistent kernel from device memory so a
+// Per-launch sizes, read by the persistent kernels from device memory so a
 // captured graph can replay with a different mix of clusters.
 struct BatchHeader {
     int nitems;    // work items in `items`
-    int pad[3];
+    int nfin;      // targets of split cluster slices (finalize work)
+    int nfc;       // number of split cluster slices
+    int pad;
 };
+
+constexpr int kMaxSplitsBatch = 512;   // j-splits of one cluster in a batch
 
 // counters[0..1] := 0 (a kernel, so that graph replays need no memset node)
 hipError_t launch_batch_reset(int *counters, hipStream_t st);
@@ -76,6 +86,10 @@ hipError_t launch_batch_reset(int *counters, hipStream_t st);
 // passes hdr->nitems.
 hipError_t launch_batch(const ClusterDesc *cd, const WorkItem *items, const BatchHeader *hdr,
                         int *counter, int cap, int grid, hipStream_t st);
+// Persistent finalize over the targets of split slices: fin_off[k] (k <=
+// nfc) is the prefix of slice target counts, fin_cl[k] the descriptor index.
+hipError_t launch_batch_finalize(const ClusterDesc *cd, const int *fin_cl, const int *fin_off,
+                                 const BatchHeader *hdr, int cap, int grid, hipStream_t st);
 
 // Launch the pairwise kernel for nt targets against js (all j), split into
 // plan.splits contiguous j-ranges.  prec: 0 split precision, 1 FP64.

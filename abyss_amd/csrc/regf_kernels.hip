@@ -185,9 +185,13 @@ __device__ __forceinline__ void run_tile(Targets<IPL, F64> &T, const double2 *__
 // of many clusters in one persistent launch), so both evaluate every pair
 // with the same instructions in the same order.
 template <int IPL, int U, bool F64>
+// Partial outputs of target q in split s live at index s*nt_stride + q -
+// q_base (q_base = 0, nt_stride = nt for a whole query; a batch slice of a
+// split cluster uses its first target and its target count).
 __device__ __forceinline__ void regf_body(const JState &js, const int *__restrict__ tgt, int nt,
                                           int jspan, const QueryOut &out, int cap,
-                                          int final_counts, int bx, int s,
+                                          int final_counts, int bx, int s, int q_base,
+                                          int nt_stride,
                                           double2 (*s_pos)[kTile * 3],
                                           typename MassT<F64>::type (*s_m)[kTile])
 {
@@ -206,7 +210,7 @@ __device__ __forceinline__ void regf_body(const JState &js, const int *__restric
             T.x[k] = js.x[i]; T.y[k] = js.y[i]; T.z[k] = js.z[i];
             T.vx[k] = js.vx[i]; T.vy[k] = js.vy[i]; T.vz[k] = js.vz[i];
             T.rnbd[k] = js.rnb2[i];
-            T.lst[k] = out.list + ((size_t)s * nt + q) * (size_t)cap;
+            T.lst[k] = out.list + ((size_t)s * nt_stride + (q - q_base)) * (size_t)cap;
         } else {
             // Inactive lane: r_nb^2 = -1 makes every j regular, so it never
             // appends; its sums are discarded.
@@ -256,7 +260,7 @@ __device__ __forceinline__ void regf_body(const JState &js, const int *__restric
     for (int k = 0; k < IPL; ++k) {
         const int q = bx * (kBlock * IPL) + k * kBlock + tid;
         if (q < nt) {
-            const size_t o = (size_t)s * nt + q;
+            const size_t o = (size_t)s * nt_stride + (q - q_base);
             out.acc[3 * o + 0] = T.acc[k][0];
             out.acc[3 * o + 1] = T.acc[k][1];
             out.acc[3 * o + 2] = T.acc[k][2];
@@ -277,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void regf_kernel(JState js, const int *__re
     __shared__ double2 s_pos[2][kTile * 3];
     __shared__ typename MassT<F64>::type s_m[2][kTile];
     regf_body<IPL, U, F64>(js, tgt, nt, jspan, out, cap, final_counts, blockIdx.x, blockIdx.y,
-                           s_pos, s_m);
+                           0, nt, s_pos, s_m);
 }
 
 // Small-block variant: JL lanes share one target (JL = 4, 16 or 64), so a
@@ -292,7 +296,8 @@ __global__ __launch_bounds__(kBlock) void regf_kernel(JState js, const int *__re
 template <int JL>
 __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__restrict__ tgt, int nt,
                                              int jspan, const QueryOut &out, int cap,
-                                             int final_counts, int bx, int s,
+                                             int final_counts, int bx, int s, int q_base,
+                                             int nt_stride,
                                              double2 (*s_pos)[kTile * 3], float (*s_m)[kTile])
 {
     constexpr int TW = 64 / JL;            // targets per wave
@@ -317,7 +322,7 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
         xi = js.x[idx]; yi = js.y[idx]; zi = js.z[idx];
         vxi = js.vx[idx]; vyi = js.vy[idx]; vzi = js.vz[idx];
         rnbf = (float)js.rnb2[idx];
-        lst = out.list + ((size_t)s * nt + q) * (size_t)cap;
+        lst = out.list + ((size_t)s * nt_stride + (q - q_base)) * (size_t)cap;
     }
     double acc[6] = {0, 0, 0, 0, 0, 0};
     float pa[6] = {0, 0, 0, 0, 0, 0};
@@ -399,7 +404,7 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
 #pragma unroll
         for (int c = 0; c < 6; ++c) acc[c] += __shfl_xor(acc[c], d, 64);
     if (q < nt && sub == 0) {
-        const size_t o = (size_t)s * nt + q;
+        const size_t o = (size_t)s * nt_stride + (q - q_base);
         out.acc[3 * o + 0] = acc[0];
         out.acc[3 * o + 1] = acc[1];
         out.acc[3 * o + 2] = acc[2];
@@ -417,8 +422,8 @@ __global__ __launch_bounds__(kBlock) void regf_kernel_jl(JState js, const int *_
 {
     __shared__ double2 s_pos[2][kTile * 3];
     __shared__ float s_m[2][kTile];
-    regf_body_jl<JL>(js, tgt, nt, jspan, out, cap, final_counts, blockIdx.x, blockIdx.y, s_pos,
-                     s_m);
+    regf_body_jl<JL>(js, tgt, nt, jspan, out, cap, final_counts, blockIdx.x, blockIdx.y, 0, nt,
+                     s_pos, s_m);
 }
 
 // One wave per target.  Partial sums: lane l adds splits l, l+64, ... in
@@ -431,8 +436,11 @@ __global__ __launch_bounds__(kBlock) void regf_kernel_jl(JState js, const int *_
 // Called by every thread of the workgroup (it contains a barrier); waves with
 // active == false only take part in the barrier.  `off` is this wave's LDS
 // prefix array of S + 1 ints.
-__device__ __forceinline__ void finalize_body(const QueryOut &part, int S, int nt, int cap, int q,
-                                              bool active, int *off, double *__restrict__ acc,
+// Partials of this target are read at s*nt_stride + q_rel; the final
+// outputs are written at q (absolute target position).
+__device__ __forceinline__ void finalize_body(const QueryOut &part, int S, int nt_stride, int cap,
+                                              int q_rel, int q, bool active, int *off,
+                                              double *__restrict__ acc,
                                               double *__restrict__ adot,
                                               int *__restrict__ nnb, int *__restrict__ list)
 {
@@ -441,7 +449,7 @@ __device__ __forceinline__ void finalize_body(const QueryOut &part, int S, int n
     double a[6] = {0, 0, 0, 0, 0, 0};
     if (active) {
         for (int s = lane; s < S; s += 64) {
-            const size_t o = (size_t)s * nt + q;
+            const size_t o = (size_t)s * nt_stride + q_rel;
             a[0] += part.acc[3 * o + 0];
             a[1] += part.acc[3 * o + 1];
             a[2] += part.acc[3 * o + 2];
@@ -458,7 +466,7 @@ __device__ __forceinline__ void finalize_body(const QueryOut &part, int S, int n
     int running = 0;
     for (int base = 0; base < S; base += 64) {
         const int s = base + lane;
-        const int c = (active && s < S) ? part.cnt[(size_t)s * nt + q] : 0;
+        const int c = (active && s < S) ? part.cnt[(size_t)s * nt_stride + q_rel] : 0;
         int incl = c;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -484,7 +492,7 @@ __device__ __forceinline__ void finalize_body(const QueryOut &part, int S, int n
             const int mid = (lo + hi + 1) >> 1;
             if (off[mid] <= e) lo = mid; else hi = mid - 1;
         }
-        dst[e] = part.list[((size_t)lo * nt + q) * (size_t)cap + (e - off[lo])];
+        dst[e] = part.list[((size_t)lo * nt_stride + q_rel) * (size_t)cap + (e - off[lo])];
     }
     if (lane == 0) nnb[q] = total <= cap ? total : -total;
 }
@@ -498,7 +506,7 @@ __global__ __launch_bounds__(256) void finalize_kernel(QueryOut part, int S, int
     extern __shared__ int s_off_all[];   // [4 waves][S + 1]
     const int w = threadIdx.x >> 6;
     const int q = blockIdx.x * 4 + w;
-    finalize_body(part, S, nt, cap, q, q < nt, s_off_all + w * (S + 1), acc, adot, nnb, list);
+    finalize_body(part, S, nt, cap, q, q, q < nt, s_off_all + w * (S + 1), acc, adot, nnb, list);
 }
 
 // Single workgroup: d_stats[0] = max over targets of min(|nnb|, cap),
@@ -560,24 +568,58 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const ClusterDesc *__rest
         if (it >= nitems) break;
         const WorkItem w = items[it];
         const ClusterDesc &d = cd[w.cluster];
-        const int jspan = d.js.n;   // unsplit: one j-range per cluster
+        const int fc = d.splits == 1 ? 1 : 0;
+        const QueryOut out = fc ? QueryOut{d.acc, d.adot, d.nnb, d.list} : d.part;
         switch (d.jl) {
             case 1:
-                regf_body<1, 2, false>(d.js, d.tgt, d.nt, jspan, d.out, cap, 1, w.tblock, 0,
-                                       s_pos, s_m);
+                regf_body<1, 2, false>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock,
+                                       w.split, d.q_base, d.nt_stride, s_pos, s_m);
                 break;
             case 4:
-                regf_body_jl<4>(d.js, d.tgt, d.nt, jspan, d.out, cap, 1, w.tblock, 0, s_pos, s_m);
+                regf_body_jl<4>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock, w.split,
+                                d.q_base, d.nt_stride, s_pos, s_m);
                 break;
             case 16:
-                regf_body_jl<16>(d.js, d.tgt, d.nt, jspan, d.out, cap, 1, w.tblock, 0, s_pos,
-                                 s_m);
+                regf_body_jl<16>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock, w.split,
+                                 d.q_base, d.nt_stride, s_pos, s_m);
                 break;
             default:
-                regf_body_jl<64>(d.js, d.tgt, d.nt, jspan, d.out, cap, 1, w.tblock, 0, s_pos,
-                                 s_m);
+                regf_body_jl<64>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock, w.split,
+                                 d.q_base, d.nt_stride, s_pos, s_m);
                 break;
         }
+    }
+}
+
+// Persistent over the flat index of (split slice, target); the loop bound is
+// the same for every wave of a workgroup, so the body's barrier is uniform.
+__global__ __launch_bounds__(256) void batch_finalize_kernel(const ClusterDesc *__restrict__ cd,
+                                                             const int *__restrict__ fin_cl,
+                                                             const int *__restrict__ fin_off,
+                                                             const BatchHeader *__restrict__ hdr,
+                                                             int cap)
+{
+    __shared__ int s_off_all[4 * (kMaxSplitsBatch + 1)];
+    const int w = threadIdx.x >> 6;
+    const int total = hdr->nfin;
+    const int nfc = hdr->nfc;
+    int *off = s_off_all + w * (kMaxSplitsBatch + 1);
+    for (int base = blockIdx.x * 4; base < total; base += gridDim.x * 4) {
+        const int g = base + w;
+        const bool active = g < total;
+        int k = 0;
+        if (active) {
+            int lo = 0, hi = nfc - 1;   // largest k with fin_off[k] <= g
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (fin_off[mid] <= g) lo = mid; else hi = mid - 1;
+            }
+            k = lo;
+        }
+        const ClusterDesc &d = cd[fin_cl[k]];
+        const int q_rel = active ? g - fin_off[k] : 0;
+        finalize_body(d.part, active ? d.splits : 1, d.nt_stride, cap, q_rel, d.q_base + q_rel,
+                      active, off, d.acc, d.adot, d.nnb, d.list);
     }
 }
 
@@ -658,6 +700,14 @@ hipError_t launch_batch(const ClusterDesc *cd, const WorkItem *items, const Batc
 {
     hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(kBlock), 0, st, cd, items, hdr, counter,
                        cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_finalize(const ClusterDesc *cd, const int *fin_cl, const int *fin_off,
+                                 const BatchHeader *hdr, int cap, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(batch_finalize_kernel, dim3(grid), dim3(256), 0, st, cd, fin_cl, fin_off,
+                       hdr, cap);
     return hipGetLastError();
 }
 

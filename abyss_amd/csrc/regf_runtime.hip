@@ -485,7 +485,12 @@ int abyss_ctx_sync(abyss_ctx_t c)
     return collect_timing(c);
 }
 
-int abyss_ctx_fetch(abyss_ctx_t c, double *acc, double *adot, int *nnb, int *list)
+}  // extern "C"
+
+namespace abyss {
+
+int ctx_fetch_on_stream(abyss_ctx_t c, hipStream_t st, double *acc, double *adot, int *nnb,
+                        int *list)
 {
     if (!c) return ABYSS_ERR_INVALID_ARG;
     if (c->nt <= 0) return set_detail(c, ABYSS_ERR_INVALID_ARG, "no targets set");
@@ -494,18 +499,20 @@ int abyss_ctx_fetch(abyss_ctx_t c, double *acc, double *adot, int *nnb, int *lis
     HIPCHK(c, hipSetDevice(c->device));
     const int nt = c->nt;
     const int cap = c->cap;
-    HIPCHK(c, abyss::launch_count_stats(c->d_nnb, nt, cap, c->d_stats, c->stream));
+    HIPCHK(c, abyss::launch_count_stats(c->d_nnb, nt, cap, c->d_stats, st));
     HIPCHK(c, hipMemcpyAsync(c->h_stats, c->d_stats, 2 * sizeof(int), hipMemcpyDeviceToHost,
-                             c->stream));
+                             st));
     HIPCHK(c, hipMemcpyAsync(c->h_res, c->d_acc, (size_t)nt * 3 * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream));
+                             hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipMemcpyAsync(c->h_res + (size_t)nt * 3, c->d_adot, (size_t)nt * 3 * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream));
+                             hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipMemcpyAsync(c->h_nnb, c->d_nnb, (size_t)nt * sizeof(int), hipMemcpyDeviceToHost,
-                             c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    int rc = collect_timing(c);
-    if (rc != ABYSS_OK) return rc;
+                             st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    if (st == c->stream) {
+        int rc = collect_timing(c);
+        if (rc != ABYSS_OK) return rc;
+    }
     const int width = c->h_stats[0];  // widest populated row, <= cap
     const int overflowed = c->h_stats[1];
     if (width > 0) {
@@ -518,8 +525,8 @@ int abyss_ctx_fetch(abyss_ctx_t c, double *acc, double *adot, int *nnb, int *lis
         }
         HIPCHK(c, hipMemcpy2DAsync(c->h_list, (size_t)width * sizeof(int), c->d_list,
                                    (size_t)cap * sizeof(int), (size_t)width * sizeof(int), nt,
-                                   hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+                                   hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
     }
     memcpy(acc, c->h_res, (size_t)nt * 3 * sizeof(double));
     memcpy(adot, c->h_res + (size_t)nt * 3, (size_t)nt * 3 * sizeof(double));
@@ -535,6 +542,16 @@ int abyss_ctx_fetch(abyss_ctx_t c, double *acc, double *adot, int *nnb, int *lis
         return set_detail(c, ABYSS_ERR_NEIGHBOR_OVERFLOW,
                           "%d target(s) exceeded the neighbour cap %d", overflowed, cap);
     return ABYSS_OK;
+}
+
+}  // namespace abyss
+
+extern "C" {
+
+int abyss_ctx_fetch(abyss_ctx_t c, double *acc, double *adot, int *nnb, int *list)
+{
+    if (!c) return ABYSS_ERR_INVALID_ARG;
+    return abyss::ctx_fetch_on_stream(c, c->stream, acc, adot, nnb, list);
 }
 
 int abyss_ctx_query(abyss_ctx_t c, int nt, const int *targets, double *acc, double *adot,

@@ -50,7 +50,8 @@ EXPORTS = [
     "abyss_ens_stream", "abyss_ens_device_bytes", "abyss_ens_error_detail",
     "abyss_srv_create", "abyss_srv_destroy", "abyss_srv_add_cluster", "abyss_srv_remove_cluster",
     "abyss_srv_submit", "abyss_srv_wait", "abyss_srv_stats", "abyss_srv_reset_stats",
-    "abyss_srv_cluster_footprint",
+    "abyss_srv_cluster_footprint", "abyss_srv_set_batch_budget", "abyss_ens_blocks",
+    "abyss_ens_set_block_range",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -131,6 +132,10 @@ def load():
                                            ctypes.POINTER(ctypes.c_double)]),
         "abyss_srv_reset_stats": (ctypes.c_int, [_vp]),
         "abyss_srv_cluster_footprint": (ctypes.c_ulonglong, [_vp, ctypes.c_int]),
+        "abyss_srv_set_batch_budget": (ctypes.c_int, [_vp, ctypes.c_double]),
+        "abyss_ens_blocks": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
+        "abyss_ens_set_block_range": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
+                                                     ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -408,6 +413,14 @@ class Ensemble:
         else:
             self.nt.pop(cluster, None)
 
+    def blocks(self, cluster):
+        nb, tpb = ctypes.c_int(), ctypes.c_int()
+        self._chk(self._lib.abyss_ens_blocks(self.h, cluster, ctypes.byref(nb), ctypes.byref(tpb)))
+        return nb.value, tpb.value
+
+    def set_block_range(self, cluster, first, count):
+        self._chk(self._lib.abyss_ens_set_block_range(self.h, cluster, first, count))
+
     def launch(self):
         self._chk(self._lib.abyss_ens_launch(self.h))
 
@@ -468,6 +481,11 @@ class Server:
 
     def footprint(self, n):
         return self._lib.abyss_srv_cluster_footprint(self.h, n)
+
+    def set_batch_budget(self, pairs):
+        rc = self._lib.abyss_srv_set_batch_budget(self.h, float(pairs))
+        if rc != OK:
+            raise AbyssError(rc)
 
     def add_cluster(self, m, x, v, r2):
         x = _f64(x, (-1, 3))

@@ -238,14 +238,11 @@ def bench_ensemble(args):
         err = force_errors(res_b[k][0][sel], res_b[k][1][sel], ra, rj, sa, sj)
         worst = max(worst, err["a_norm"]["max"], err["jdot_norm"]["max"])
         hard += compare_lists(res_b[k][2][sel], res_b[k][3][sel], rn, rl, x, r2, s)["hard_mismatches"]
-    line = {
-        "metric": "ensemble regular-force pair-interactions/s",
-        "value": inter / (ms_batched * 1e-3), "unit": "interactions/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_batched,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (f64 i-j differences and per-particle sums)", "data": "synthetic",
-        "config": {"workload": "ensemble_256_plummer_n1024-65536_active0.01-1",
-                   "clusters": len(cl), "interactions_per_step": inter, "cap": cap},
+    return {
+        "workload": "ensemble_256_plummer_n1024-65536_active0.01-1",
+        "clusters": len(cl), "interactions_per_step": inter, "cap": cap,
+        "batched_ms_per_step": ms_batched,
+        "interactions_per_s": inter / (ms_batched * 1e-3),
         "per_cluster_device_launches_ms": ms_plain,
         "per_cluster_legacy_calls_ms": ms_legacy,
         "batched_with_fetch_ms": ms_batched_fetch,
@@ -253,7 +250,6 @@ def bench_ensemble(args):
         "speedup_vs_legacy_calls": ms_legacy / ms_batched_fetch,
         "accuracy": {"worst_normalised_error": worst, "list_hard_mismatches": hard},
     }
-    print(json.dumps(line), flush=True)
 
 
 def served_workload(count=64, lo=4096, hi=1048576, seed=5, k_nb=32):
@@ -358,6 +354,8 @@ def main():
     ap.add_argument("--ipl", type=int, default=0)
     ap.add_argument("--splits", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-ensemble", action="store_true",
+                    help="skip the batched-ensemble and served-ensemble legs")
     ap.add_argument("--no-check", action="store_true", help="skip the sampled accuracy check")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -371,7 +369,8 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     if args.workload == "ensemble":
-        bench_ensemble(args)
+        print(json.dumps({"metric": "batched ensemble regular-force pair-interactions/s",
+                          "batched_ensemble": bench_ensemble(args)}), flush=True)
         return
     if args.workload == "serve":
         print(json.dumps({"metric": "served ensemble regular-step latency",
@@ -430,6 +429,12 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(js, args.cpu_seconds)
         log(f"cpu baseline {cpu}")
+    ens_info = serve_info = None
+    if not args.no_ensemble:
+        ens_args = argparse.Namespace(steps=4, warmup=1)
+        ens_info = bench_ensemble(ens_args)
+        log(f"batched ensemble {ens_info}")
+        serve_info = bench_serve(args)
 
     line = {
         "metric": "regular-force pair-interactions/s",
@@ -456,6 +461,8 @@ def main():
                      "interactions_per_launch": inter},
         "cpu_baseline": cpu,
         "accuracy": acc_info,
+        "batched_ensemble": ens_info,
+        "served_ensemble": serve_info,
     }
     print(json.dumps(line), flush=True)
 

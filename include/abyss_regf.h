@@ -190,6 +190,12 @@ int abyss_ens_set_targets(abyss_ens_t ens, int cluster, int nt, const int *targe
 /* One batched step over all clusters with targets (asynchronous). */
 int abyss_ens_launch(abyss_ens_t ens);
 int abyss_ens_sync(abyss_ens_t ens);
+/* Target blocks (work items) of a cluster's current targets and the
+ * targets per block; the launch covers blocks [first, first + count) when a
+ * range is set (count < 0: all; set_targets resets to all).  Splitting a
+ * cluster's blocks over several launches gives the same bits. */
+int abyss_ens_blocks(abyss_ens_t ens, int cluster, int *nblocks, int *targets_per_block);
+int abyss_ens_set_block_range(abyss_ens_t ens, int cluster, int first, int count);
 /* Results of a cluster from the last launch (list row stride = cap). */
 int abyss_ens_fetch(abyss_ens_t ens, int cluster, double *acc, double *adot, int *nnb,
                     int *list);
@@ -199,15 +205,21 @@ unsigned long long abyss_ens_device_bytes(abyss_ens_t ens);
 const char *abyss_ens_error_detail(abyss_ens_t ens);
 
 /* Served ensemble: clusters stay resident; requests (one regular step of
- * one cluster) are submitted from any host thread and completed by a worker
- * thread that continuously batches every pending request (at most one per
- * cluster per batch) into the next ensemble launch.  New clusters are
- * admitted while their estimated HBM footprint fits the budget. */
+ * one cluster) are submitted from any host thread.  A worker thread runs
+ * time-sliced continuous batching: each batch carries at least one target
+ * block of every running request, smallest remaining work first, and more
+ * blocks until a pair budget is used; requests that arrive meanwhile join
+ * the next batch.  A completion thread copies finished results to the host
+ * on its own stream while the next batch runs.  At most one request per
+ * cluster runs at a time (later ones queue).  New clusters are admitted
+ * while their estimated HBM footprint fits the budget. */
 typedef struct abyss_srv_s *abyss_srv_t;
 
 int abyss_srv_create(int device, int cap, int max_clusters, unsigned long long hbm_budget,
                      abyss_srv_t *out);
 int abyss_srv_destroy(abyss_srv_t srv);
+/* Work budget of one batch in pair-interactions (default 3e10, ~25 ms). */
+int abyss_srv_set_batch_budget(abyss_srv_t srv, double pairs);
 /* Admits a cluster (ABYSS_ERR_ADMISSION if it does not fit the budget,
  * ABYSS_ERR_BUSY if no slot is free); returns its id in *cluster. */
 int abyss_srv_add_cluster(abyss_srv_t srv, int n, const double *m, const double *x,

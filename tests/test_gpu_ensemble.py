@@ -116,10 +116,34 @@ def test_ensemble_graph_replays_with_changing_mix(clusters):
             say(f"  replay {rep}: clusters {chosen} ok")
 
 
-def test_server_concurrent_requests_match_alone(clusters):
-    say("case: server concurrent requests")
+def test_ensemble_block_ranges_give_same_bits(clusters):
+    say("case: ensemble block ranges")
+    x, v, m, r2 = clusters[10]
+    t = np.random.default_rng(11).permutation(len(m))[:20000].astype(np.int32)
+    with regf.Ensemble(device=0, cap=128, max_clusters=2) as ens:
+        ens.upload(0, m, x, v, r2)
+        ens.set_targets(0, t)
+        ens.launch()
+        ens.sync()
+        whole = ens.fetch(0)
+        ens.set_targets(0, t)
+        nb, tpb = ens.blocks(0)
+        say(f"  {nb} blocks of {tpb} targets")
+        for first, count in ((0, 3), (3, nb // 2), (3 + nb // 2, -1)):
+            ens.set_block_range(0, first, count)
+            ens.launch()
+            ens.sync()
+        sliced = ens.fetch(0)
+    for a, b in zip(whole, sliced):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("budget", [3e10, 2e7])
+def test_server_concurrent_requests_match_alone(clusters, budget):
+    say(f"case: server concurrent requests, batch budget {budget:g} pairs")
     sub = clusters[:8]
     with regf.Server(device=0, cap=128, max_clusters=8, hbm_budget=8 << 30) as srv:
+        srv.set_batch_budget(budget)
         ids = [srv.add_cluster(m, x, v, r2) for (x, v, m, r2) in sub]
         reqs = [(ids[c % 8], random_targets(len(sub[c % 8][2]), 500 + c)) for c in range(40)]
         results = [None] * len(reqs)
