@@ -36,6 +36,15 @@ struct Plan {
     int jl = 1;      // lanes per target (1: one target per lane; 4/16/64)
 };
 
+// Plan choice measured by the autotuner for one target-count bucket.
+struct TuneEntry {
+    int jl;         // lanes per target
+    int unroll;     // j-unroll of the one-target-per-lane kernel
+    int wg_per_cu;  // workgroups per CU the j-split count aims for
+};
+// The tuned entry for nt targets on `device`, if a table is installed.
+bool tuned_plan(int device, int nt, TuneEntry *out);
+
 // Whether a (precision, targets-per-lane, unroll, lanes-per-target) kernel
 // variant exists.
 bool variant_supported(int prec, int ipl, int unroll, int jl);

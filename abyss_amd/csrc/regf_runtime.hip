@@ -106,9 +106,12 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
     const int cu = c->cu_count > 0 ? c->cu_count : 256;
     // Lanes per target: one target per lane once there are enough targets
     // to occupy the chip; small blocks share a target among 4/16/64 lanes.
+    abyss::TuneEntry te{0, 0, 16};
+    const bool tuned = c->precision == ABYSS_PREC_SPLIT && abyss::tuned_plan(c->device, nt, &te);
     int jl = c->force_jl;
     if (jl == 0) {
-        if (c->precision == ABYSS_PREC_FP64 || nt >= 4096) jl = 1;
+        if (tuned) jl = te.jl;
+        else if (c->precision == ABYSS_PREC_FP64 || nt >= 4096) jl = 1;
         else if (nt >= 1024) jl = 4;
         else if (nt >= 64) jl = 16;
         else jl = 64;
@@ -118,7 +121,7 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
     // block size on MI355X; ipl 2/4 keep more targets in registers but cut
     // occupancy to 3 and 1 waves per SIMD.
     int ipl = c->force_ipl ? c->force_ipl : 1;
-    int unroll = c->force_unroll ? c->force_unroll : 2;
+    int unroll = c->force_unroll ? c->force_unroll : (tuned ? te.unroll : 2);
     if (c->precision == ABYSS_PREC_FP64) {
         ipl = std::min(ipl, 2);
         unroll = 2;
@@ -131,8 +134,9 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
     p.grid_x = (nt + tpb - 1) / tpb;
     int S = c->force_splits;
     if (S == 0) {
-        // ~16 workgroups per CU; every split keeps at least one tile.
-        const int want_wg = cu * 16;
+        // ~16 workgroups per CU (or the tuned count); every split keeps at
+        // least one tile.
+        const int want_wg = cu * (tuned ? te.wg_per_cu : 16);
         S = (want_wg + p.grid_x - 1) / p.grid_x;
         const int max_by_n = std::max(1, n / abyss::kTile);
         S = std::min(S, std::min(max_by_n, abyss::kMaxSplits));

@@ -51,7 +51,7 @@ EXPORTS = [
     "abyss_srv_create", "abyss_srv_destroy", "abyss_srv_add_cluster", "abyss_srv_remove_cluster",
     "abyss_srv_submit", "abyss_srv_wait", "abyss_srv_stats", "abyss_srv_reset_stats",
     "abyss_srv_cluster_footprint", "abyss_srv_set_batch_budget", "abyss_ens_blocks",
-    "abyss_ens_set_block_range",
+    "abyss_ens_set_block_range", "abyss_autotune", "abyss_autotune_load", "abyss_autotune_clear",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -136,6 +136,9 @@ def load():
         "abyss_ens_blocks": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
         "abyss_ens_set_block_range": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int,
                                                      ctypes.c_int]),
+        "abyss_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
+        "abyss_autotune_load": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p]),
+        "abyss_autotune_clear": (ctypes.c_int, [ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -245,6 +248,25 @@ def CalculateAccelerationOnDevice(targets):
 
 def device_count():
     return load().abyss_device_count()
+
+
+def autotune(device=0, n=65536, cache_path=None):
+    """Measure and install the plan table for `device` (see abyss_autotune)."""
+    rc = load().abyss_autotune(device, n, cache_path.encode() if cache_path else None)
+    if rc != OK:
+        raise AbyssError(rc)
+
+
+def autotune_load(device, cache_path):
+    rc = load().abyss_autotune_load(device, cache_path.encode())
+    if rc != OK:
+        raise AbyssError(rc)
+
+
+def autotune_clear(device=0):
+    rc = load().abyss_autotune_clear(device)
+    if rc != OK:
+        raise AbyssError(rc)
 
 
 # --------------------------------------------------------------------------

@@ -356,6 +356,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the batched-ensemble and served-ensemble legs")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the built-in plan rules instead of tuning first")
     ap.add_argument("--no-check", action="store_true", help="skip the sampled accuracy check")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -387,6 +389,12 @@ def main():
     t0 = time.perf_counter()
     js = workloads.king_binaries(args.n)
     log(f"workload {js.name}: built in {time.perf_counter() - t0:.1f} s")
+    tuned = False
+    if not args.no_autotune and args.ipl == 0 and args.splits == 0:
+        t0 = time.perf_counter()
+        regf.autotune(0, 65536)
+        tuned = True
+        log(f"autotuned plan table in {time.perf_counter() - t0:.1f} s")
     import numpy as np
     targets = np.arange(js.n, dtype=np.int32)
     ctx = regf.Context(device=0, cap=js.cap,
@@ -452,7 +460,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{js.name}_fullN", "n": js.n, "active": int(len(targets)),
                    "neighbours_target": js.meta["k_nb"], "cap": js.cap,
-                   "precision": args.precision, "plan": plan},
+                   "precision": args.precision, "plan": plan, "autotuned": tuned},
         "pct_fp32_peak": 100.0 * achieved / FP32_PEAK_TFLOPS,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,

@@ -165,6 +165,19 @@ int abyss_ctx_plan(abyss_ctx_t ctx, int *i_per_lane, int *j_splits,
 /* Detail string of the context's last error. */
 const char *abyss_ctx_error_detail(abyss_ctx_t ctx);
 
+/* Plan autotuner.  Times candidate plans (lanes per target, j-unroll,
+ * workgroups per CU -> j-split count) for target counts 1, 4, 16, ... n on
+ * a synthetic n-particle cluster (n >= 256) and installs the fastest per
+ * power-of-two target-count bucket for every later automatic plan on
+ * `device` (split precision; forced tunings still win).  Writes the table
+ * to `cache_path` when not NULL.  Takes seconds; results of a query depend
+ * on its plan only through the FP64 summation order. */
+int abyss_autotune(int device, int n, const char *cache_path);
+/* Installs a table written by abyss_autotune. */
+int abyss_autotune_load(int device, const char *cache_path);
+/* Removes the table: plans use the built-in rules again. */
+int abyss_autotune_clear(int device);
+
 /* ===================================================================== */
 /* Part 3: batched ensembles and the served ensemble.                   */
 /* No counterpart in the reference (one cluster per process there).     */

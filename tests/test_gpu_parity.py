@@ -291,3 +291,26 @@ def test_golden_fixtures(name):
     say(f"[golden {name}] {err}")
     assert err["a_rel"]["max"] <= 1e-6
     assert err["jdot_rel"]["p99"] <= 1e-6
+
+
+def test_autotune_table_roundtrip(tmp_path):
+    """The autotuner installs a 32-bucket table, tuned plans still meet the
+    oracle bars, and a saved table loads back."""
+    say("case: autotune")
+    path = str(tmp_path / "plans.txt")
+    try:
+        regf.autotune(0, 16384, path)
+        lines = [ln for ln in open(path) if not ln.startswith("#")]
+        assert len(lines) == 32
+        js = workloads.plummer_kroupa(65536)
+        with regf.Context(device=0, cap=js.cap) as c:
+            c.upload(js.m, js.x, js.v, js.r2)
+            for size in (1, 100, 5000):
+                t = np.random.default_rng(size).permutation(js.n)[:size].astype(np.int32)
+                r = c.query(t)
+                say(f"  tuned plan for {size} targets: {c.plan()}")
+                check(js, t, r, label=f"tuned/{size}")
+        regf.autotune_clear(0)
+        regf.autotune_load(0, path)
+    finally:
+        regf.autotune_clear(0)
