@@ -290,9 +290,69 @@ __global__ __launch_bounds__(kBlock) void regf_kernel(JState js, const int *__re
 // t*JL + sub (sub = lane mod JL).  Neighbours of a step are compacted with a
 // wave ballot: a lane's slot is the group's running count plus the number
 // of neighbour lanes below it in the group, which keeps every list in
-// ascending j.  Each lane keeps FP32 partials folded into FP64 every kFold
-// of its own pairs; at the end the JL FP64 sums of a group are combined by a
-// fixed xor-butterfly (bitwise reproducible).
+// ascending j.  Each lane keeps FP32 partials folded into FP64 at the end of
+// the tile in which it reaches kFold pairs; at the end the JL FP64 sums of a
+// group are combined by a fixed xor-butterfly (bitwise reproducible).
+// Per-lane state of the lanes-per-target kernel.
+struct JLLane {
+    double xi, yi, zi, vxi, vyi, vzi;
+    float rnbf;
+    int idx, sub, cnt, cap;
+    int *lst;
+    unsigned long long gmask, below;
+    float pa[6];
+};
+
+// Steps of one tile.  FULL: all kTile records are valid; SELF: the tile may
+// hold a lane's own target.  The common case (FULL, !SELF) lists exactly the
+// near pairs, so its only extra instruction over the force is the compare.
+template <int JL, bool FULL, bool SELF>
+__device__ __forceinline__ void jl_tile(JLLane &L, const double2 *__restrict__ pos,
+                                        const float *__restrict__ mass, int count, int jt)
+{
+    const int steps = FULL ? kTile / JL : (count + JL - 1) / JL;
+#pragma unroll 2
+    for (int st = 0; st < steps; ++st) {
+        const int r = st * JL + L.sub;
+        const double2 p0 = pos[3 * r + 0];
+        const double2 p1 = pos[3 * r + 1];
+        const double2 p2 = pos[3 * r + 2];
+        const float mj = mass[r];
+        const float dx = (float)(p0.x - L.xi);
+        const float dy = (float)(p0.y - L.yi);
+        const float dz = (float)(p1.x - L.zi);
+        const float dvx = (float)(p1.y - L.vxi);
+        const float dvy = (float)(p2.x - L.vyi);
+        const float dvz = (float)(p2.y - L.vzi);
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        float rinv = __builtin_amdgcn_rsqf(r2);
+        const bool near = !(r2 > L.rnbf);
+        if (near) rinv = 0.0f;
+        bool listed = near;
+        if (!FULL) listed = listed && r < count;              // zero-filled records
+        if (SELF) listed = listed && jt + r != L.idx;         // self-exclusion by index
+        const unsigned long long bal = __ballot(listed);
+        if (bal) {
+            const unsigned long long gb = bal & L.gmask;
+            if (listed) {
+                const int p = L.cnt + __popcll(gb & L.below);
+                if (p < L.cap) L.lst[p] = jt + r;
+            }
+            L.cnt += __popcll(gb);
+        }
+        const float rinv2 = rinv * rinv;
+        const float mr3 = mj * rinv * rinv2;
+        const float rv = fmaf(dz, dvz, fmaf(dy, dvy, dx * dvx));
+        const float al = -3.0f * rv * rinv2;
+        L.pa[0] = fmaf(mr3, dx, L.pa[0]);
+        L.pa[1] = fmaf(mr3, dy, L.pa[1]);
+        L.pa[2] = fmaf(mr3, dz, L.pa[2]);
+        L.pa[3] = fmaf(mr3, fmaf(al, dx, dvx), L.pa[3]);
+        L.pa[4] = fmaf(mr3, fmaf(al, dy, dvy), L.pa[4]);
+        L.pa[5] = fmaf(mr3, fmaf(al, dz, dvz), L.pa[5]);
+    }
+}
+
 template <int JL>
 __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__restrict__ tgt, int nt,
                                              int jspan, const QueryOut &out, int cap,
@@ -301,33 +361,34 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
                                              double2 (*s_pos)[kTile * 3], float (*s_m)[kTile])
 {
     constexpr int TW = 64 / JL;            // targets per wave
-    constexpr int STEPS = kTile / JL;      // steps per full tile
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int sub = lane % JL;
     const int g = lane / JL;
     const int j0 = s * jspan;
     const int j1 = min(js.n, j0 + jspan);
     const int q = bx * (4 * TW) + (tid >> 6) * TW + g;
-    const unsigned long long gmask =
-        (JL == 64) ? ~0ull : (((1ull << JL) - 1ull) << (g * JL));
-    const unsigned long long below = (1ull << lane) - 1ull;   // lanes < this lane
 
-    double xi = 0, yi = 0, zi = 0, vxi = 0, vyi = 0, vzi = 0;
-    float rnbf = -1.0f;   // inactive target: every j regular, nothing listed
-    int idx = -1;
-    int *lst = nullptr;
+    JLLane L;
+    L.sub = lane % JL;
+    L.gmask = (JL == 64) ? ~0ull : (((1ull << JL) - 1ull) << (g * JL));
+    L.below = (1ull << lane) - 1ull;   // lanes < this lane
+    L.cap = cap;
+    L.cnt = 0;
+    L.xi = L.yi = L.zi = L.vxi = L.vyi = L.vzi = 0.0;
+    L.rnbf = -1.0f;   // inactive target: every j regular, nothing listed
+    L.idx = -1;
+    L.lst = nullptr;
     if (q < nt) {
-        idx = tgt[q];
-        xi = js.x[idx]; yi = js.y[idx]; zi = js.z[idx];
-        vxi = js.vx[idx]; vyi = js.vy[idx]; vzi = js.vz[idx];
-        rnbf = (float)js.rnb2[idx];
-        lst = out.list + ((size_t)s * nt_stride + (q - q_base)) * (size_t)cap;
+        L.idx = tgt[q];
+        L.xi = js.x[L.idx]; L.yi = js.y[L.idx]; L.zi = js.z[L.idx];
+        L.vxi = js.vx[L.idx]; L.vyi = js.vy[L.idx]; L.vzi = js.vz[L.idx];
+        L.rnbf = (float)js.rnb2[L.idx];
+        L.lst = out.list + ((size_t)s * nt_stride + (q - q_base)) * (size_t)cap;
     }
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    float pa[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) L.pa[c] = 0.0f;
     int since_fold = 0;
-    int cnt = 0;
 
     const int ntile = (j1 - j0 + kTile - 1) / kTile;
     TileLoader<float> ld;
@@ -341,69 +402,33 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
         if (t + 1 < ntile) ld.load(js, j0 + (t + 1) * kTile + tid, j1);
         const int jt = j0 + t * kTile;
         const int count = min(kTile, j1 - jt);
-        const bool self_tile = __any((unsigned)(idx - jt) < (unsigned)count);
-        const bool full = count == kTile;
-        const int steps = full ? STEPS : (count + JL - 1) / JL;
-        const double2 *pos = s_pos[b];
-        const float *mass = s_m[b];
-        for (int st = 0; st < steps; ++st) {
-            const int r = st * JL + sub;
-            const double2 p0 = pos[3 * r + 0];
-            const double2 p1 = pos[3 * r + 1];
-            const double2 p2 = pos[3 * r + 2];
-            const float mj = mass[r];
-            const float dx = (float)(p0.x - xi);
-            const float dy = (float)(p0.y - yi);
-            const float dz = (float)(p1.x - zi);
-            const float dvx = (float)(p1.y - vxi);
-            const float dvy = (float)(p2.x - vyi);
-            const float dvz = (float)(p2.y - vzi);
-            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            float rinv = __builtin_amdgcn_rsqf(r2);
-            const bool near = !(r2 > rnbf);
-            if (near) rinv = 0.0f;
-            const int jg = jt + r;
-            bool listed = near;
-            if (!full) listed = listed && r < count;          // zero-filled records
-            if (self_tile) listed = listed && jg != idx;      // self-exclusion by index
-            const unsigned long long bal = __ballot(listed);
-            if (bal) {
-                const unsigned long long gb = bal & gmask;
-                if (listed) {
-                    const int p = cnt + __popcll(gb & below);
-                    if (p < cap) lst[p] = jg;
-                }
-                cnt += __popcll(gb);
-            }
-            const float rinv2 = rinv * rinv;
-            const float mr3 = mj * rinv * rinv2;
-            const float rv = fmaf(dz, dvz, fmaf(dy, dvy, dx * dvx));
-            const float al = -3.0f * rv * rinv2;
-            pa[0] = fmaf(mr3, dx, pa[0]);
-            pa[1] = fmaf(mr3, dy, pa[1]);
-            pa[2] = fmaf(mr3, dz, pa[2]);
-            pa[3] = fmaf(mr3, fmaf(al, dx, dvx), pa[3]);
-            pa[4] = fmaf(mr3, fmaf(al, dy, dvy), pa[4]);
-            pa[5] = fmaf(mr3, fmaf(al, dz, dvz), pa[5]);
-            if (++since_fold == kFold) {
+        if (__any((unsigned)(L.idx - jt) < (unsigned)count))
+            jl_tile<JL, false, true>(L, s_pos[b], s_m[b], count, jt);
+        else if (count == kTile)
+            jl_tile<JL, true, false>(L, s_pos[b], s_m[b], count, jt);
+        else
+            jl_tile<JL, false, false>(L, s_pos[b], s_m[b], count, jt);
+        // fold the FP32 partials once a lane has summed >= kFold pairs
+        // (every tile for JL = 4, every 4 / 16 tiles for JL = 16 / 64)
+        since_fold += (count + JL - 1) / JL;
+        if (since_fold >= kFold) {
 #pragma unroll
-                for (int c = 0; c < 6; ++c) {
-                    acc[c] += (double)pa[c];
-                    pa[c] = 0.0f;
-                }
-                since_fold = 0;
+            for (int c = 0; c < 6; ++c) {
+                acc[c] += (double)L.pa[c];
+                L.pa[c] = 0.0f;
             }
+            since_fold = 0;
         }
         if (t + 1 < ntile) ld.store(s_pos[b ^ 1], s_m[b ^ 1], tid);
         __syncthreads();
     }
 #pragma unroll
-    for (int c = 0; c < 6; ++c) acc[c] += (double)pa[c];
+    for (int c = 0; c < 6; ++c) acc[c] += (double)L.pa[c];
 #pragma unroll
     for (int d = JL / 2; d >= 1; d >>= 1)
 #pragma unroll
         for (int c = 0; c < 6; ++c) acc[c] += __shfl_xor(acc[c], d, 64);
-    if (q < nt && sub == 0) {
+    if (q < nt && L.sub == 0) {
         const size_t o = (size_t)s * nt_stride + (q - q_base);
         out.acc[3 * o + 0] = acc[0];
         out.acc[3 * o + 1] = acc[1];
@@ -411,7 +436,7 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
         out.adot[3 * o + 0] = acc[3];
         out.adot[3 * o + 1] = acc[4];
         out.adot[3 * o + 2] = acc[5];
-        out.cnt[o] = (final_counts && cnt > cap) ? -cnt : cnt;
+        out.cnt[o] = (final_counts && L.cnt > cap) ? -L.cnt : L.cnt;
     }
 }
 

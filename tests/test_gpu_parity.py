@@ -314,3 +314,22 @@ def test_autotune_table_roundtrip(tmp_path):
         regf.autotune_load(0, path)
     finally:
         regf.autotune_clear(0)
+
+
+def test_hermite_run_one_nbody_time(tmp_path):
+    """N=1024 equal-mass Plummer (seed 1, ~48 neighbours) integrated for one
+    N-body time with block time steps, the regular force and neighbour lists
+    coming from the six legacy entry points: energy is conserved at the same
+    level as with the FP64 oracle as provider."""
+    from tests import hermite_util
+    say("case: Hermite run, N=1024, 1 N-body time")
+    js = workloads.plummer_equal(1024)
+    ic = str(tmp_path / "ic.bin")
+    hermite_util.write_ic(ic, js)
+    gpu = hermite_util.run("gpu", ic, 1.0)
+    ref = hermite_util.run("oracle", ic, 1.0)
+    say(f"  gpu:    {gpu}")
+    say(f"  oracle: {ref}")
+    assert abs(gpu["dE_over_E"]) < 1e-6
+    assert abs(gpu["dE_over_E"]) < 100 * max(abs(ref["dE_over_E"]), 1e-9)
+    assert abs(gpu["blocks"] - ref["blocks"]) < 0.05 * ref["blocks"]
