@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libabyss_regf.so")
-SOURCES = ["regf_kernels.hip", "regf_runtime.hip", "regf_ensemble.hip", "regf_autotune.hip"]
+SOURCES = ["regf_kernels.hip", "regf_runtime.hip", "regf_ensemble.hip", "regf_autotune.hip",
+           "regf_state.hip"]
 ARCH = os.environ.get("ABYSS_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [

@@ -35,6 +35,14 @@ struct abyss_ctx_s {
     size_t h_list_bytes = 0;
     int *h_stats = nullptr;
 
+    // device-resident particle state (regf_state.hip): SoA x,v,a,adot (3n
+    // each), t, m, r2 (n each); staging for incremental updates
+    double *d_state = nullptr, *h_state = nullptr;
+    size_t state_bytes = 0;
+    int state_n = 0;
+    char *d_upd = nullptr, *h_upd = nullptr;
+    size_t upd_bytes = 0;
+
     // split-j scratch
     size_t scratch_bytes = 0;
     char *d_scratch = nullptr;

@@ -199,7 +199,8 @@ void release_all(abyss_ctx_t c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->d_j); hfree(c->h_j);
     dfree(c->d_tgt); dfree(c->d_acc); dfree(c->d_adot); dfree(c->d_nnb); dfree(c->d_list);
-    dfree(c->d_stats); dfree(c->d_scratch);
+    dfree(c->d_stats); dfree(c->d_scratch); dfree(c->d_state); dfree(c->d_upd);
+    hfree(c->h_state); hfree(c->h_upd);
     hfree(c->h_tgt); hfree(c->h_res); hfree(c->h_nnb); hfree(c->h_list); hfree(c->h_stats);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -371,6 +372,7 @@ int abyss_ctx_upload(abyss_ctx_t c, int n, const double *m, const double *x, con
     HIPCHK(c, hipSetDevice(c->device));
     // The previous upload / query may still read the staging buffer.
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->state_n = 0;   // a device-resident state no longer matches the j-set
     const size_t need = jset_bytes(n);
     if (need > c->j_bytes) {
         dfree(c->d_j);

@@ -52,6 +52,7 @@ EXPORTS = [
     "abyss_srv_submit", "abyss_srv_wait", "abyss_srv_stats", "abyss_srv_reset_stats",
     "abyss_srv_cluster_footprint", "abyss_srv_set_batch_budget", "abyss_ens_blocks",
     "abyss_ens_set_block_range", "abyss_autotune", "abyss_autotune_load", "abyss_autotune_clear",
+    "abyss_ctx_upload_state", "abyss_ctx_update_state", "abyss_ctx_predict",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -139,6 +140,11 @@ def load():
         "abyss_autotune": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_char_p]),
         "abyss_autotune_load": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p]),
         "abyss_autotune_clear": (ctypes.c_int, [ctypes.c_int]),
+        "abyss_ctx_upload_state": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+                                                  _dp]),
+        "abyss_ctx_update_state": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp,
+                                                  _dp, _dp]),
+        "abyss_ctx_predict": (ctypes.c_int, [_vp, ctypes.c_double]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -331,6 +337,26 @@ class Context:
         t = np.ascontiguousarray(targets, dtype=np.int32)
         self._chk(self._lib.abyss_ctx_set_targets(self.h, t.size, _i(t) if t.size else None))
         self.nt = t.size
+
+    def upload_state(self, m, x, v, a, adot, t, r2):
+        """Device-resident state (positions at each particle's own time t)."""
+        x = _f64(x, (-1, 3))
+        n = x.shape[0]
+        self._chk(self._lib.abyss_ctx_upload_state(
+            self.h, n, _d(_f64(m, (n,))), _d(x), _d(_f64(v, (n, 3))), _d(_f64(a, (n, 3))),
+            _d(_f64(adot, (n, 3))), _d(_f64(t, (n,))), _d(_f64(r2, (n,)))))
+        self.n = n
+
+    def update_state(self, idx, m, x, v, a, adot, t, r2):
+        i = np.ascontiguousarray(idx, dtype=np.int32)
+        k = i.size
+        self._chk(self._lib.abyss_ctx_update_state(
+            self.h, k, _i(i), _d(_f64(m, (k,))), _d(_f64(x, (k, 3))), _d(_f64(v, (k, 3))),
+            _d(_f64(a, (k, 3))), _d(_f64(adot, (k, 3))), _d(_f64(t, (k,))),
+            _d(_f64(r2, (k,)))))
+
+    def predict(self, t_new):
+        self._chk(self._lib.abyss_ctx_predict(self.h, float(t_new)))
 
     def launch(self):
         self._chk(self._lib.abyss_ctx_launch(self.h))

@@ -165,6 +165,27 @@ int abyss_ctx_plan(abyss_ctx_t ctx, int *i_per_lane, int *j_splits,
 /* Detail string of the context's last error. */
 const char *abyss_ctx_error_detail(abyss_ctx_t ctx);
 
+/* Device-resident particle state: j prediction on the device instead of
+ * the caller's O(N) predict + pack + copy before every regular step
+ * (src_mpi/cuda/CalculateRegularAcceleration.cpp:330-449).
+ *   upload_state: x, v at each particle's own time t, its total
+ *     acceleration a and jerk adot there ([n][3] each), t, m, r2 ([n]);
+ *   update_state: replaces `count` particles idx[k] (e.g. after their
+ *     corrector), arrays indexed by k;
+ *   predict: every particle to t_new with the second-order predictor
+ *     x + h(v + h/2(a + h/3 adot)), v + h(a + h/2 adot), h = t_new - t
+ *     (src_mpi/Particle/UpdateParticle.cpp:37-59), written to the j-set
+ *     the next query uses.  The arithmetic is plain IEEE double in the
+ *     reference's order, so it equals a host prediction bit for bit.
+ * A plain abyss_ctx_upload discards the resident state. */
+int abyss_ctx_upload_state(abyss_ctx_t ctx, int n, const double *m, const double *x,
+                           const double *v, const double *a, const double *adot,
+                           const double *t, const double *r2);
+int abyss_ctx_update_state(abyss_ctx_t ctx, int count, const int *idx, const double *m,
+                           const double *x, const double *v, const double *a,
+                           const double *adot, const double *t, const double *r2);
+int abyss_ctx_predict(abyss_ctx_t ctx, double t_new);
+
 /* Plan autotuner.  Times candidate plans (lanes per target, j-unroll,
  * workgroups per CU -> j-split count) for target counts 1, 4, 16, ... n on
  * a synthetic n-particle cluster (n >= 256) and installs the fastest per

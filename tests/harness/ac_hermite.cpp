@@ -14,10 +14,14 @@
 // regular time scale), so the run checks the split force and jerk and the
 // neighbour lists in a dynamical setting through energy conservation.
 //
-// Providers: "gpu" = libabyss_regf.so through the six legacy entry points;
-// "oracle" = the FP64 oracle (oracle/liboracle.so).  Output: one JSON line.
+// Providers: "gpu" = libabyss_regf.so through the six legacy entry points
+// (host prediction, full upload every block); "resident" = the same library
+// with the particle state resident on the device (abyss_ctx_upload_state,
+// abyss_ctx_predict on the device, abyss_ctx_update_state of the active
+// block after its corrector); "oracle" = the FP64 oracle
+// (oracle/liboracle.so).  Output: one JSON line.
 //
-// usage: ac_hermite <gpu|oracle> <ic.bin> <t_end> <eta>
+// usage: ac_hermite <gpu|resident|oracle> <ic.bin> <t_end> <eta>
 //   ic.bin: int32 n, then x[n][3], v[n][3], m[n], r2[n] as float64
 #include <math.h>
 #include <stdio.h>
@@ -45,19 +49,32 @@ struct System {
     std::vector<double> t, dt;           // last update time, step
 };
 
-bool use_gpu = true;
+enum Provider { GPU, RESIDENT, ORACLE };
+Provider prov = GPU;
+abyss_ctx_t rctx = nullptr;   // RESIDENT: the device-resident context
 int cap = ABYSS_NUM_NEIGHBOR_MAX;
 long long reg_calls = 0, reg_targets = 0;
 
 // Regular force + jerk and neighbour lists of targets[k] from the provider,
 // with j predicted to time tnow; then irregular part over the neighbours.
-void total_force(const System &s, const std::vector<double> &xp, const std::vector<double> &vp,
-                 const std::vector<int> &tg, std::vector<double> &acc, std::vector<double> &jerk)
+void total_force(const System &s, double tn, const std::vector<double> &xp,
+                 const std::vector<double> &vp, const std::vector<int> &tg,
+                 std::vector<double> &acc, std::vector<double> &jerk)
 {
     const int nt = (int)tg.size();
     std::vector<double> ar(3 * nt), jr(3 * nt);
     std::vector<int> nnb(nt), list((size_t)nt * cap);
-    if (use_gpu) {
+    if (prov == RESIDENT) {
+        int rc = abyss_ctx_predict(rctx, tn);
+        if (rc == ABYSS_OK)
+            rc = abyss_ctx_query(rctx, nt, tg.data(), ar.data(), jr.data(), nnb.data(),
+                                 list.data());
+        if (rc != ABYSS_OK) {
+            fprintf(stderr, "resident regular force failed: %s\n",
+                    abyss_ctx_error_detail(rctx));
+            exit(2);
+        }
+    } else if (prov == GPU) {
         int n = s.n, ntt = nt;
         SendToDevice(&n, const_cast<double *>(s.m.data()),
                      reinterpret_cast<double(*)[3]>(const_cast<double *>(xp.data())),
@@ -166,7 +183,7 @@ int main(int argc, char **argv)
         fprintf(stderr, "usage: %s <gpu|oracle> <ic.bin> <t_end> <eta>\n", argv[0]);
         return 1;
     }
-    use_gpu = strcmp(argv[1], "gpu") == 0;
+    prov = strcmp(argv[1], "gpu") == 0 ? GPU : strcmp(argv[1], "resident") == 0 ? RESIDENT : ORACLE;
     const double t_end = atof(argv[3]);
     const double eta = atof(argv[4]);
     System s;
@@ -186,11 +203,19 @@ int main(int argc, char **argv)
     fclose(f);
     s.a.assign(3 * n, 0); s.ad.assign(3 * n, 0); s.a2.assign(3 * n, 0); s.a3.assign(3 * n, 0);
     s.t.assign(n, 0.0); s.dt.assign(n, 0.0);
-    if (use_gpu) {
+    if (prov == GPU) {
         int rank = 0;
         OpenDevice(&rank);
         if (abyss_last_error() != ABYSS_OK) {
             fprintf(stderr, "OpenDevice: %s\n", abyss_last_error_detail());
+            return 2;
+        }
+    } else if (prov == RESIDENT) {
+        std::vector<double> zero(3 * n, 0.0);
+        if (abyss_ctx_create(0, cap, &rctx) != ABYSS_OK ||
+            abyss_ctx_upload_state(rctx, n, s.m.data(), s.x.data(), s.v.data(), zero.data(),
+                                   zero.data(), s.t.data(), s.r2.data()) != ABYSS_OK) {
+            fprintf(stderr, "resident context setup failed\n");
             return 2;
         }
     }
@@ -201,7 +226,7 @@ int main(int argc, char **argv)
     for (int i = 0; i < n; ++i) all[i] = i;
     std::vector<double> xp, vp, acc, jerk;
     predict_all(s, 0.0, xp, vp);
-    total_force(s, xp, vp, all, acc, jerk);
+    total_force(s, 0.0, xp, vp, all, acc, jerk);
     const double dtmax = 1.0 / 16.0;
     for (int i = 0; i < n; ++i) {
         for (int d = 0; d < 3; ++d) {
@@ -212,6 +237,29 @@ int main(int argc, char **argv)
         double q = pow(2.0, floor(log2(std::max(dt0, 1e-12))));
         s.dt[i] = std::min(q, dtmax);
     }
+    auto push_state = [&](const std::vector<int> &idx) {
+        if (prov != RESIDENT) return;
+        const int k = (int)idx.size();
+        std::vector<double> m(k), x(3 * k), v(3 * k), a(3 * k), ad(3 * k), t(k), r2(k);
+        for (int q = 0; q < k; ++q) {
+            const int i = idx[q];
+            m[q] = s.m[i];
+            t[q] = s.t[i];
+            r2[q] = s.r2[i];
+            for (int d = 0; d < 3; ++d) {
+                x[3 * q + d] = s.x[3 * i + d];
+                v[3 * q + d] = s.v[3 * i + d];
+                a[3 * q + d] = s.a[3 * i + d];
+                ad[3 * q + d] = s.ad[3 * i + d];
+            }
+        }
+        if (abyss_ctx_update_state(rctx, k, idx.data(), m.data(), x.data(), v.data(), a.data(),
+                                   ad.data(), t.data(), r2.data()) != ABYSS_OK) {
+            fprintf(stderr, "update_state failed: %s\n", abyss_ctx_error_detail(rctx));
+            exit(2);
+        }
+    };
+    push_state(all);
     long long steps = 0, blocks = 0;
     for (;;) {
         // next block time among particles that have not reached t_end
@@ -223,7 +271,7 @@ int main(int argc, char **argv)
         for (int i = 0; i < n; ++i)
             if (s.t[i] < t_end && s.t[i] + s.dt[i] == tn) act.push_back(i);
         predict_all(s, tn, xp, vp);
-        total_force(s, xp, vp, act, acc, jerk);
+        total_force(s, tn, xp, vp, act, acc, jerk);
         for (size_t k = 0; k < act.size(); ++k) {
             const int i = act[k];
             const double h = tn - s.t[i];
@@ -248,16 +296,19 @@ int main(int argc, char **argv)
             s.dt[i] = block_step(norm3(&s.a[3 * i]), norm3(&s.ad[3 * i]), norm3(s2v), norm3(s3v),
                                  eta, tn, s.dt[i], dtmax);
         }
+        push_state(act);
         steps += (long long)act.size();
         blocks += 1;
     }
     const double E1 = energy(s);
     const double wall =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
-    if (use_gpu) {
+    if (prov == GPU) {
         int rank = 0;
         ProfileDevice(&rank);
         CloseDevice();
+    } else if (prov == RESIDENT) {
+        abyss_ctx_destroy(rctx);
     }
     double cx = 0, cy = 0, cz = 0;
     for (int i = 0; i < n; ++i) {
@@ -269,7 +320,7 @@ int main(int argc, char **argv)
            "\"E1\": %.15e, \"dE_over_E\": %.6e, \"blocks\": %lld, \"particle_steps\": %lld, "
            "\"regular_calls\": %lld, \"mean_block\": %.2f, \"com\": [%.6e, %.6e, %.6e], "
            "\"x0\": [%.15e, %.15e, %.15e], \"wall_s\": %.3f}\n",
-           use_gpu ? "gpu" : "oracle", n, t_end, eta, E0, E1, (E1 - E0) / fabs(E0), blocks, steps,
+           argv[1], n, t_end, eta, E0, E1, (E1 - E0) / fabs(E0), blocks, steps,
            reg_calls, (double)reg_targets / std::max(1LL, reg_calls), cx, cy, cz, s.x[0], s.x[1],
            s.x[2], wall);
     return 0;

@@ -333,3 +333,21 @@ def test_hermite_run_one_nbody_time(tmp_path):
     assert abs(gpu["dE_over_E"]) < 1e-6
     assert abs(gpu["dE_over_E"]) < 100 * max(abs(ref["dE_over_E"]), 1e-9)
     assert abs(gpu["blocks"] - ref["blocks"]) < 0.05 * ref["blocks"]
+
+
+def test_hermite_resident_state_matches_host_prediction(tmp_path):
+    """The device-resident path (state uploaded once, prediction on the
+    device, only the corrected block scattered back) reproduces the legacy
+    path, which predicts on the host and uploads all j every block, bit for
+    bit over a quarter N-body time."""
+    from tests import hermite_util
+    say("case: Hermite resident vs legacy")
+    js = workloads.plummer_equal(1024)
+    ic = str(tmp_path / "ic.bin")
+    hermite_util.write_ic(ic, js)
+    legacy = hermite_util.run("gpu", ic, 0.25)
+    resident = hermite_util.run("resident", ic, 0.25)
+    say(f"  legacy:   {legacy}")
+    say(f"  resident: {resident}")
+    for key in ("E1", "dE_over_E", "blocks", "particle_steps", "com", "x0"):
+        assert legacy[key] == resident[key], key
