@@ -43,6 +43,16 @@ struct abyss_ctx_s {
     char *d_upd = nullptr, *h_upd = nullptr;
     size_t upd_bytes = 0;
 
+    // resident neighbour lists (regf_irregular.hip): [nnb: n ints, padded
+    // to 64][list: n rows of cap ints], row = particle index; filled after
+    // each launch while keep_lists is on
+    bool keep_lists = false;
+    char *d_rlists = nullptr;
+    size_t rlist_bytes = 0;
+    int rlist_n = 0;
+    char *d_irr = nullptr, *h_irr = nullptr;  // targets + [nt][6] results
+    size_t irr_bytes = 0;
+
     // split-j scratch
     size_t scratch_bytes = 0;
     char *d_scratch = nullptr;
@@ -71,4 +81,7 @@ int ctx_fetch_on_stream(abyss_ctx_t c, hipStream_t st, double *acc, double *adot
                         int *list);
 // Device bytes a context holds (j-set, targets, outputs, scratch).
 unsigned long long ctx_device_bytes(abyss_ctx_t c);
+// After a launch: copies its neighbour rows into the resident per-particle
+// lists when they are enabled (no-op otherwise).
+int keep_launch_lists(abyss_ctx_t c);
 }  // namespace abyss

@@ -200,7 +200,8 @@ void release_all(abyss_ctx_t c)
     dfree(c->d_j); hfree(c->h_j);
     dfree(c->d_tgt); dfree(c->d_acc); dfree(c->d_adot); dfree(c->d_nnb); dfree(c->d_list);
     dfree(c->d_stats); dfree(c->d_scratch); dfree(c->d_state); dfree(c->d_upd);
-    hfree(c->h_state); hfree(c->h_upd);
+    dfree(c->d_rlists); dfree(c->d_irr);
+    hfree(c->h_state); hfree(c->h_upd); hfree(c->h_irr);
     hfree(c->h_tgt); hfree(c->h_res); hfree(c->h_nnb); hfree(c->h_list); hfree(c->h_stats);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -288,7 +289,8 @@ int ctx_create_on_stream(int device, int cap, hipStream_t stream, abyss_ctx_t *o
 unsigned long long ctx_device_bytes(abyss_ctx_t c)
 {
     if (!c) return 0;
-    return (unsigned long long)c->j_bytes + c->scratch_bytes +
+    return (unsigned long long)c->j_bytes + c->scratch_bytes + c->state_bytes + c->upd_bytes +
+           c->rlist_bytes + c->irr_bytes +
            (unsigned long long)c->t_capacity * (sizeof(int) * 2 + 6 * sizeof(double) +
                                                 (size_t)c->cap * sizeof(int));
 }
@@ -477,6 +479,10 @@ int abyss_ctx_launch(abyss_ctx_t c)
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->timing_pending = true;
+    {
+        int rc = abyss::keep_launch_lists(c);
+        if (rc != ABYSS_OK) return set_detail(c, rc, "keeping resident neighbour lists failed");
+    }
     c->n_grav += 1;
     c->sum_ni += c->nt;
     c->interactions += (double)c->nt * (double)(c->n - 1);

@@ -53,6 +53,7 @@ EXPORTS = [
     "abyss_srv_cluster_footprint", "abyss_srv_set_batch_budget", "abyss_ens_blocks",
     "abyss_ens_set_block_range", "abyss_autotune", "abyss_autotune_load", "abyss_autotune_clear",
     "abyss_ctx_upload_state", "abyss_ctx_update_state", "abyss_ctx_predict",
+    "abyss_ctx_keep_lists", "abyss_ctx_irregular",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -145,6 +146,8 @@ def load():
         "abyss_ctx_update_state": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, _dp,
                                                   _dp, _dp]),
         "abyss_ctx_predict": (ctypes.c_int, [_vp, ctypes.c_double]),
+        "abyss_ctx_keep_lists": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "abyss_ctx_irregular": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -357,6 +360,19 @@ class Context:
 
     def predict(self, t_new):
         self._chk(self._lib.abyss_ctx_predict(self.h, float(t_new)))
+
+    def keep_lists(self, enable=True):
+        """Keep every launch's neighbour lists resident, by particle index."""
+        self._chk(self._lib.abyss_ctx_keep_lists(self.h, 1 if enable else 0))
+
+    def irregular(self, targets):
+        """Irregular force and jerk over the resident lists: (acc, adot) [nt,3]."""
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        acc = np.empty((t.size, 3))
+        adot = np.empty((t.size, 3))
+        self._chk(self._lib.abyss_ctx_irregular(self.h, t.size, _i(t) if t.size else None,
+                                                _d(acc), _d(adot)))
+        return acc, adot
 
     def launch(self):
         self._chk(self._lib.abyss_ctx_launch(self.h))

@@ -186,6 +186,25 @@ int abyss_ctx_update_state(abyss_ctx_t ctx, int count, const int *idx, const dou
                            const double *adot, const double *t, const double *r2);
 int abyss_ctx_predict(abyss_ctx_t ctx, double t_new);
 
+/* Irregular force on the device.  The reference sums the irregular force
+ * and jerk over each particle's neighbour list on the host, in FP64, with
+ * m_r3 = m_j/(r2*sqrt(r2)), a += m_r3 r, adot += m_r3 (v - 3 r (r.v)/r2)
+ * (Particle::computeAccelerationIrr,
+ * src_mpi/Particle/ComputeAcceleration.cpp:14-190).
+ *   keep_lists(ctx, 1): from then on every abyss_ctx_launch / _query also
+ *     stores its neighbour lists in HBM by particle index (n x cap ints);
+ *     a particle's list is the one from its last regular step, and a
+ *     particle never targeted has an empty list.  A j-set of another size
+ *     empties all lists.  keep_lists(ctx, 0) frees them.
+ *   irregular(ctx, nt, targets, acc, adot): for each target i, the FP64
+ *     sum over its resident list evaluated on the current j-set (e.g. after
+ *     abyss_ctx_predict), acc/adot [nt][3].  A neighbour at r = 0 adds
+ *     nothing.  Lists truncated by overflow sum their first cap entries.
+ *     Synchronous. */
+int abyss_ctx_keep_lists(abyss_ctx_t ctx, int enable);
+int abyss_ctx_irregular(abyss_ctx_t ctx, int nt, const int *targets, double *acc,
+                        double *adot);
+
 /* Plan autotuner.  Times candidate plans (lanes per target, j-unroll,
  * workgroups per CU -> j-split count) for target counts 1, 4, 16, ... n on
  * a synthetic n-particle cluster (n >= 256) and installs the fastest per

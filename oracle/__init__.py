@@ -32,6 +32,9 @@ def _load():
                                       ctypes.c_double, _ip, _ip, ctypes.c_int, ctypes.c_int]
     lib.oracle_predict2.restype = None
     lib.oracle_predict2.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.oracle_irregular.restype = None
+    lib.oracle_irregular.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _ip, _ip, _ip,
+                                     ctypes.c_int, _dp, _dp]
     lib.oracle_max_threads.restype = ctypes.c_int
     _lib = lib
     return lib
@@ -103,6 +106,23 @@ def predict2(x, v, a, adot, dt):
     vp = np.empty_like(v)
     lib.oracle_predict2(x.shape[0], _d(x), _d(v), _d(a), _d(adot), _d(dt), _d(xp), _d(vp))
     return xp, vp
+
+
+def irregular(x, v, m, targets, nnb, lists, cap):
+    """FP64 irregular force + jerk of each target over its list row (stride cap)."""
+    lib = _load()
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    nnb = np.ascontiguousarray(nnb, dtype=np.int32)
+    lists = np.ascontiguousarray(lists, dtype=np.int32).reshape(-1)
+    assert nnb.size == t.size and lists.size >= t.size * cap
+    acc = np.empty((t.size, 3))
+    adot = np.empty((t.size, 3))
+    lib.oracle_irregular(x.shape[0], _d(x), _d(v), _d(m), t.size, _i(t), _i(nnb), _i(lists),
+                         cap, _d(acc), _d(adot))
+    return acc, adot
 
 
 def max_threads():

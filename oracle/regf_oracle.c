@@ -26,7 +26,8 @@
  * src_mpi/cuda/cuda_kernels.cu:363,432).  Here the first `cap` entries are
  * written and nnb[k] = -(true count), the convention the HIP library uses.
  *
- * Parity pinning (tests/test_oracle.py): two-body closed forms, a = -grad(phi)
+ * Parity pinning (tests/test_oracle.py): two-body closed forms,
+ * regular + irregular = the unsplit direct sum (oracle_irregular), a = -grad(phi)
  * and jdot = d a/dt by finite differences, momentum conservation, homogeneity
  * scalings, r_nb -> 0 / inf limits, the exact "<=" boundary case, coincident
  * particles, and a vectorised numpy brute force on small N.
@@ -173,6 +174,47 @@ void oracle_predict2(int n, const double *x, const double *v, const double *a,
                 xp[q] = ((adot[q] * h / 3 + a[q]) * h / 2 + v[q]) * h + x[q];
                 vp[q] = (adot[q] * h / 2 + a[q]) * h + v[q];
             }
+        }
+    }
+}
+
+/*
+ * Irregular force and jerk of each target over its neighbour list, FP64:
+ *   m_r3 = m_j/(r2*sqrt(r2)); a += m_r3*x; adot += m_r3*(v - 3*x*(x.v)/r2)
+ * with x = x_j - x_i, v = v_j - v_i, summed in list order
+ * (Particle::computeAccelerationIrr, src_mpi/Particle/ComputeAcceleration.cpp:14-190).
+ * Row k of `list` (stride cap) holds min(|nnb[k]|, cap) indices.  An entry at
+ * r2 == 0 adds nothing, as in the regular split.
+ */
+void oracle_irregular(int n, const double *x, const double *v, const double *m, int nt,
+                      const int *targets, const int *nnb, const int *list, int cap,
+                      double *acc, double *adot)
+{
+    (void)n;
+    for (int k = 0; k < nt; ++k) {
+        const int i = targets[k];
+        int c = nnb[k] < 0 ? -nnb[k] : nnb[k];
+        if (c > cap) c = cap;
+        double a[3] = {0, 0, 0}, ad[3] = {0, 0, 0};
+        for (int e = 0; e < c; ++e) {
+            const int j = list[(size_t)k * cap + e];
+            double dx[3], dv[3];
+            for (int d = 0; d < 3; ++d) {
+                dx[d] = x[3 * j + d] - x[3 * i + d];
+                dv[d] = v[3 * j + d] - v[3 * i + d];
+            }
+            const double r2 = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+            if (r2 == 0.0) continue;
+            const double rv = dx[0] * dv[0] + dx[1] * dv[1] + dx[2] * dv[2];
+            const double m_r3 = m[j] / (r2 * sqrt(r2));
+            for (int d = 0; d < 3; ++d) {
+                a[d] += m_r3 * dx[d];
+                ad[d] += m_r3 * (dv[d] - 3 * dx[d] * rv / r2);
+            }
+        }
+        for (int d = 0; d < 3; ++d) {
+            acc[3 * k + d] = a[d];
+            adot[3 * k + d] = ad[d];
         }
     }
 }

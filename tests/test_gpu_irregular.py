@@ -1,0 +1,172 @@
+"""GPU parity of the irregular (neighbour) force over resident lists
+(abyss_ctx_keep_lists / abyss_ctx_irregular) against the FP64 oracle.
+
+Both sides are FP64; they differ only in summation order and FMA
+contraction, so the bar is a normalised error |d a| / sum_j |term_j|
+<= 1e-13 per target.  The split is also checked as a whole: the regular
+part of a step plus the irregular part over that step's lists equals the
+unsplit direct sum to the regular kernel's own accuracy (1e-6).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from abyss_amd import regf, workloads
+
+pytestmark = pytest.mark.gpu
+
+TOL_IRR_NORM = 1e-13
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _scales(x, v, m, t, nnb, lst, cap):
+    """Per target: sum_j m/r^2 and sum_j 4 m |v|/r^3 over the list."""
+    sa = np.zeros(len(t))
+    sj = np.zeros(len(t))
+    for q, i in enumerate(t):
+        c = min(abs(int(nnb[q])), cap)
+        j = lst[q, :c]
+        d = x[j] - x[i]
+        r2 = np.sum(d * d, axis=1)
+        ok = r2 > 0
+        r2 = np.where(ok, r2, 1.0)
+        w = np.where(ok, m[j] / r2, 0.0)
+        sa[q] = np.sum(w)
+        sj[q] = np.sum(4 * w * np.linalg.norm(v[j] - v[i], axis=1) / np.sqrt(r2))
+    return sa, sj
+
+
+def _norm_err(got, want, scale):
+    d = np.linalg.norm(got - want, axis=1)
+    return d / np.where(scale > 0, scale, 1.0)
+
+
+def check_irregular(x, v, m, t, nnb, lst, cap, acc, adot, label):
+    ra, rj = oracle.irregular(x, v, m, t, nnb, lst, cap)
+    sa, sj = _scales(x, v, m, t, nnb, lst, cap)
+    ea = _norm_err(acc, ra, sa)
+    ej = _norm_err(adot, rj, sj)
+    say(f"[{label}] nt={len(t)} a_norm_max={ea.max():.3e} jdot_norm_max={ej.max():.3e} "
+        f"mean_nnb={np.mean(np.abs(nnb)):.1f}")
+    assert ea.max() <= TOL_IRR_NORM
+    assert ej.max() <= TOL_IRR_NORM
+    # a target without neighbours gets exactly zero
+    empty = np.asarray(nnb) == 0
+    assert np.all(acc[empty] == 0) and np.all(adot[empty] == 0)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = regf.Context(device=0, cap=256)
+    c.keep_lists(True)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("size", [1, 100, 8192, 65536])
+def test_irregular_matches_oracle_kroupa_65536(ctx, size):
+    say(f"case: irregular plummer_kroupa N=65536 block={size}")
+    js = workloads.plummer_kroupa(65536)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    t = np.random.default_rng(size + 7).permutation(js.n)[:size].astype(np.int32)
+    acc_r, adot_r, nnb, lst = ctx.query(t)
+    acc_i, adot_i = ctx.irregular(t)
+    check_irregular(js.x, js.v, js.m, t, nnb, lst, js.cap, acc_i, adot_i, f"irr/{size}")
+    # regular + irregular = unsplit direct sum (oracle with r_nb = 0)
+    fa, fj, _, _, sa, sj = oracle.regf(js.x, js.v, js.m, np.zeros(js.n), t, js.cap, scales=True)
+    ea = _norm_err(acc_r + acc_i, fa, sa)
+    ej = _norm_err(adot_r + adot_i, fj, sj)
+    say(f"  reg+irr vs unsplit: a_norm_max={ea.max():.3e} jdot_norm_max={ej.max():.3e}")
+    assert ea.max() <= 1e-6 and ej.max() <= 1e-6
+    # bitwise reproducible
+    a2, j2 = ctx.irregular(t)
+    assert np.array_equal(a2, acc_i) and np.array_equal(j2, adot_i)
+
+
+def test_irregular_uses_each_particles_last_list(ctx):
+    say("case: irregular last list per particle, plummer_equal N=1024")
+    js = workloads.plummer_equal(1024)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    rng = np.random.default_rng(3)
+    perm = rng.permutation(js.n).astype(np.int32)
+    t1, t2 = perm[:600], perm[400:900]   # 200 particles in both launches
+    _, _, n1, l1 = ctx.query(t1)
+    # second launch with wider radii: the overlap's lists change
+    r2b = js.r2 * 1.7
+    ctx.upload(js.m, js.x, js.v, r2b)
+    _, _, n2, l2 = ctx.query(t2)
+    # the last list of each particle; never-targeted particles have none
+    cap = js.cap
+    last_n = np.zeros(js.n, dtype=np.int32)
+    last_l = np.zeros((js.n, cap), dtype=np.int32)
+    last_n[t1], last_l[t1] = n1, l1
+    last_n[t2], last_l[t2] = n2, l2
+    q = np.arange(js.n, dtype=np.int32)
+    acc, adot = ctx.irregular(q)
+    check_irregular(js.x, js.v, js.m, q, last_n[q], last_l[q], cap, acc, adot, "irr/last")
+    assert np.sum(last_n == 0) >= 100   # untargeted: exactly zero (checked above)
+
+
+def test_irregular_on_predicted_positions(ctx):
+    say("case: irregular after device prediction, plummer_equal N=1024")
+    js = workloads.plummer_equal(1024)
+    rng = np.random.default_rng(11)
+    a = rng.normal(size=(js.n, 3)) * 0.1
+    ad = rng.normal(size=(js.n, 3)) * 0.1
+    t0 = rng.uniform(0.0, 0.01, js.n)
+    ctx.upload_state(js.m, js.x, js.v, a, ad, t0, js.r2)
+    ctx.predict(0.01)
+    t = rng.permutation(js.n)[:512].astype(np.int32)
+    _, _, nnb, lst = ctx.query(t)
+    ctx.predict(0.0125)   # irregular step later than the regular one
+    acc, adot = ctx.irregular(t)
+    xp, vp = oracle.predict2(js.x, js.v, a, ad, 0.0125 - t0)
+    check_irregular(xp, vp, js.m, t, nnb, lst, js.cap, acc, adot, "irr/predicted")
+
+
+def test_irregular_overflowed_lists_are_truncated():
+    say("case: irregular with overflowed lists, cap=16")
+    js = workloads.plummer_equal(1024)
+    with regf.Context(device=0, cap=16) as small:
+        small.keep_lists(True)
+        small.upload(js.m, js.x, js.v, js.r2)
+        t = np.arange(0, js.n, 5, dtype=np.int32)
+        _, _, nnb, lst = small.query(t)
+        assert small.last_status == regf.ERR_NEIGHBOR_OVERFLOW and np.all(nnb < 0)
+        acc, adot = small.irregular(t)
+        check_irregular(js.x, js.v, js.m, t, nnb, lst, 16, acc, adot, "irr/overflow")
+
+
+def test_irregular_errors_and_resets():
+    say("case: irregular errors and resets")
+    js = workloads.plummer_equal(1024)
+    with regf.Context(device=0, cap=256) as c:
+        c.upload(js.m, js.x, js.v, js.r2)
+        with pytest.raises(regf.AbyssError) as e:
+            c.irregular([0])
+        assert e.value.code == regf.ERR_NO_JSET
+        c.keep_lists(True)
+        acc, adot = c.irregular([0, 5])          # never targeted: empty lists
+        assert np.all(acc == 0) and np.all(adot == 0)
+        c.query(np.arange(64, dtype=np.int32))
+        with pytest.raises(regf.AbyssError) as e:
+            c.irregular([js.n])
+        assert e.value.code == regf.ERR_TARGET_RANGE
+        with pytest.raises(regf.AbyssError) as e:
+            c.irregular([])
+        assert e.value.code == regf.ERR_INVALID_ARG
+        acc, _ = c.irregular([3])
+        assert np.linalg.norm(acc) > 0
+        # a j-set of another size empties every list
+        c.upload(js.m[:512], js.x[:512], js.v[:512], js.r2[:512])
+        acc, adot = c.irregular([3])
+        assert np.all(acc == 0) and np.all(adot == 0)
+        c.keep_lists(False)
+        with pytest.raises(regf.AbyssError):
+            c.irregular([3])

@@ -213,3 +213,62 @@ def test_predictor_matches_taylor_series():
     assert vp[0, 0] == pytest.approx(2 + 1.5 + 0.5, rel=1e-15)
     xp, vp = oracle.predict2([[1.0, 2, 3]], [[2.0, 0, 0]], [[3.0, 0, 0]], [[4.0, 0, 0]], 0.0)
     assert list(xp[0]) == [1.0, 2, 3] and list(vp[0]) == [2.0, 0, 0]
+
+
+def test_irregular_two_body_closed_form():
+    # same pair as the regular closed form, but j is a neighbour: the force
+    # moves from the regular to the irregular part unchanged
+    x = np.array([[0.0, 0.0, 0.0], [3.0, 4.0, 0.0]])
+    v = np.array([[0.1, -0.2, 0.3], [0.4, 0.1, -0.5]])
+    m = np.array([1.5, 2.5])
+    _, _, nnb, lst = oracle.regf(x, v, m, np.full(2, 26.0), [0, 1], cap=4)
+    assert list(nnb) == [1, 1]
+    acc, adot = oracle.irregular(x, v, m, [0, 1], nnb, lst, 4)
+    np.testing.assert_allclose(acc[0], [0.06, 0.08, 0.0], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(adot[0], [-0.00912, -0.01416, -0.016], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(acc[1], [-0.036, -0.048, 0.0], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(adot[1], [0.005472, 0.008496, 0.0096], rtol=0, atol=1e-15)
+
+
+def test_regular_plus_irregular_is_unsplit_brute_force():
+    # a_reg + a_irr over the list of the same step = the direct sum over all
+    # j != i, computed independently in numpy
+    x, v, m = _rand_cluster(200, 41)
+    n = len(m)
+    rng = np.random.default_rng(43)
+    r2 = rng.uniform(0.05, 3.0, n)
+    t = rng.permutation(n)[:120]
+    acc, adot, c, lst = oracle.regf(x, v, m, r2, t, cap=n)
+    assert c.max() > 0 and c.max() < n - 1 and np.mean(c > 0) > 0.5
+    ai, ji = oracle.irregular(x, v, m, t, c, lst, n)
+    D = x[None, :, :] - x[t][:, None, :]
+    V = v[None, :, :] - v[t][:, None, :]
+    R2 = np.sum(D * D, axis=2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = np.where(R2 > 0, 1.0 / np.sqrt(R2), 0.0)
+    w = m[None, :] * inv ** 3
+    RV = np.sum(D * V, axis=2)
+    a_np = np.sum(w[:, :, None] * D, axis=1)
+    j_np = np.sum(w[:, :, None] * (V - 3.0 * (RV * inv * inv)[:, :, None] * D), axis=1)
+    np.testing.assert_allclose(acc + ai, a_np, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(adot + ji, j_np, rtol=1e-11, atol=1e-13)
+    # and the irregular part alone is the brute force restricted to the list
+    near = np.zeros_like(R2, dtype=bool)
+    for q in range(len(t)):
+        near[q, lst[q, :c[q]]] = True
+    a_n = np.sum((w * near)[:, :, None] * D, axis=1)
+    np.testing.assert_allclose(ai, a_n, rtol=1e-12, atol=1e-14)
+
+
+def test_irregular_empty_truncated_and_coincident():
+    x = np.array([[0.0, 0, 0], [1.0, 0, 0], [0.0, 0, 0], [0.0, 2.0, 0]])
+    v = np.zeros((4, 3))
+    m = np.array([1.0, 2.0, 3.0, 4.0])
+    # target 0: coincident j=2 adds nothing; -3 with cap 2 sums the first 2
+    lists = np.array([[1, 2], [0, 0]], dtype=np.int32)
+    acc, _ = oracle.irregular(x, v, m, [0, 1], [-3, 0], lists, 2)
+    np.testing.assert_allclose(acc[0], [2.0, 0, 0], rtol=0, atol=0)
+    np.testing.assert_array_equal(acc[1], [0.0, 0, 0])
+    lists = np.array([[1, 2, 3]], dtype=np.int32)
+    acc, _ = oracle.irregular(x, v, m, [0], [3], lists, 3)
+    np.testing.assert_allclose(acc[0], [2.0, 1.0, 0], rtol=0, atol=1e-15)
