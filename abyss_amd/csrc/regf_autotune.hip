@@ -4,7 +4,7 @@
 // cluster (uniform cube, r_nb for ~48 neighbours), every candidate plan is
 // timed with HIP events (median of 3 launches):
 //   lanes per target  JL in {1, 4, 16, 64} (JL = 1 also with j-unroll 2, 4)
-//   workgroups per CU W in {1, 2, 4, 8, 16, 32, 64}, which sets the j-split
+//   workgroups per CU W in {1, 2, 4, 8, 16, 32, 48, 64, 96}, which sets the j-split
 //                     count S = ceil(W * CUs / target blocks)
 // The fastest (JL, unroll, W) per power-of-two target-count bucket is kept in
 // a per-device table that plan selection consults for every later query on
@@ -146,7 +146,7 @@ int abyss_autotune(int device, int n, const char *path)
             for (int un : {2, 4}) {
                 if (jl > 1 && un == 4) continue;
                 int prev_S = -1;
-                for (int w : {1, 2, 4, 8, 16, 32, 64}) {
+                for (int w : {1, 2, 4, 8, 16, 32, 48, 64, 96}) {
                     const int tpb = jl > 1 ? 4 * (64 / jl) : abyss::kBlock;
                     const int gx = (mt + tpb - 1) / tpb;
                     const int S = std::max(1, std::min((w * cu + gx - 1) / gx,

@@ -81,6 +81,8 @@ int ctx_fetch_on_stream(abyss_ctx_t c, hipStream_t st, double *acc, double *adot
                         int *list);
 // Device bytes a context holds (j-set, targets, outputs, scratch).
 unsigned long long ctx_device_bytes(abyss_ctx_t c);
+// abyss_ctx_launch without keeping the resident lists (force kernels only).
+int ctx_launch_force(abyss_ctx_t c);
 // After a launch: copies its neighbour rows into the resident per-particle
 // lists when they are enabled (no-op otherwise).
 int keep_launch_lists(abyss_ctx_t c);

@@ -123,7 +123,9 @@ __device__ __forceinline__ void run_tile(Targets<IPL, F64> &T, const double2 *__
                     // Neighbour (or self, or coincident j): rare branch that
                     // records j and zeroes rinv, so the pair adds exactly 0.
                     if (!(r2 > T.rnbf[k])) {
-                        rinv = 0.0f;
+                        // volatile keeps the zeroing inside the rare branch
+                        // (a select would cost every pair an instruction)
+                        asm volatile("v_mov_b32 %0, 0" : "=v"(rinv));
                         append<IPL, F64, SELF>(T, k, jg0 + r, cap);
                     }
                     const float rinv2 = rinv * rinv;
@@ -327,12 +329,15 @@ __device__ __forceinline__ void jl_tile(JLLane &L, const double2 *__restrict__ p
         const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
         float rinv = __builtin_amdgcn_rsqf(r2);
         const bool near = !(r2 > L.rnbf);
-        if (near) rinv = 0.0f;
-        bool listed = near;
-        if (!FULL) listed = listed && r < count;              // zero-filled records
-        if (SELF) listed = listed && jt + r != L.idx;         // self-exclusion by index
-        const unsigned long long bal = __ballot(listed);
-        if (bal) {
+        const unsigned long long nb = __ballot(near);
+        if (nb) {
+            // rare: zero rinv of the near lanes here rather than with a
+            // select on every pair (volatile keeps it inside the branch)
+            if (near) asm volatile("v_mov_b32 %0, 0" : "=v"(rinv));
+            bool listed = near;
+            if (!FULL) listed = listed && r < count;          // zero-filled records
+            if (SELF) listed = listed && jt + r != L.idx;     // self-exclusion by index
+            const unsigned long long bal = (FULL && !SELF) ? nb : __ballot(listed);
             const unsigned long long gb = bal & L.gmask;
             if (listed) {
                 const int p = L.cnt + __popcll(gb & L.below);

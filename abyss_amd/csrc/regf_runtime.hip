@@ -104,14 +104,16 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
     Plan p;
     p.tile = abyss::kTile;
     const int cu = c->cu_count > 0 ? c->cu_count : 256;
-    // Lanes per target: one target per lane once there are enough targets
-    // to occupy the chip; small blocks share a target among 4/16/64 lanes.
+    // Lanes per target: 4 lanes share a target from 1024 targets up (its
+    // inner loop is one VALU op per pair shorter than one target per lane
+    // and it holds 5 waves per SIMD instead of 4: 47.8% vs 45.4% of FP32
+    // peak at N = m = 262144); smaller blocks spread a target over 16/64.
     abyss::TuneEntry te{0, 0, 16};
     const bool tuned = c->precision == ABYSS_PREC_SPLIT && abyss::tuned_plan(c->device, nt, &te);
     int jl = c->force_jl;
     if (jl == 0) {
         if (tuned) jl = te.jl;
-        else if (c->precision == ABYSS_PREC_FP64 || nt >= 4096) jl = 1;
+        else if (c->precision == ABYSS_PREC_FP64) jl = 1;
         else if (nt >= 1024) jl = 4;
         else if (nt >= 64) jl = 16;
         else jl = 64;
@@ -134,9 +136,11 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
     p.grid_x = (nt + tpb - 1) / tpb;
     int S = c->force_splits;
     if (S == 0) {
-        // ~16 workgroups per CU (or the tuned count); every split keeps at
-        // least one tile.
-        const int want_wg = cu * (tuned ? te.wg_per_cu : 16);
+        // Workgroups per CU over the whole launch (or the tuned count):
+        // ~48 for the 1- and 4-lane kernels (tail of the last wave of
+        // workgroups; measured best at m = 16K..256K, N = 256K), 16 for the
+        // 16/64-lane ones; every split keeps at least one tile.
+        const int want_wg = cu * (tuned ? te.wg_per_cu : (jl <= 4 ? 48 : 16));
         S = (want_wg + p.grid_x - 1) / p.grid_x;
         const int max_by_n = std::max(1, n / abyss::kTile);
         S = std::min(S, std::min(max_by_n, abyss::kMaxSplits));
@@ -453,7 +457,11 @@ int abyss_ctx_plan(abyss_ctx_t c, int *ipl, int *splits, int *grid_x, int *tile,
     return ABYSS_OK;
 }
 
-int abyss_ctx_launch(abyss_ctx_t c)
+}  // extern "C"
+
+namespace abyss {
+
+int ctx_launch_force(abyss_ctx_t c)
 {
     if (!c) return ABYSS_ERR_INVALID_ARG;
     if (c->n <= 0) return set_detail(c, ABYSS_ERR_NO_JSET, "no j-set uploaded");
@@ -479,13 +487,22 @@ int abyss_ctx_launch(abyss_ctx_t c)
     }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->timing_pending = true;
-    {
-        int rc = abyss::keep_launch_lists(c);
-        if (rc != ABYSS_OK) return set_detail(c, rc, "keeping resident neighbour lists failed");
-    }
     c->n_grav += 1;
     c->sum_ni += c->nt;
     c->interactions += (double)c->nt * (double)(c->n - 1);
+    return ABYSS_OK;
+}
+
+}  // namespace abyss
+
+extern "C" {
+
+int abyss_ctx_launch(abyss_ctx_t c)
+{
+    int rc = abyss::ctx_launch_force(c);
+    if (rc != ABYSS_OK) return rc;
+    rc = abyss::keep_launch_lists(c);
+    if (rc != ABYSS_OK) return set_detail(c, rc, "keeping resident neighbour lists failed");
     return ABYSS_OK;
 }
 
