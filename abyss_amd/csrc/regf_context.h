@@ -52,6 +52,12 @@ struct abyss_ctx_s {
     int rlist_n = 0;
     char *d_irr = nullptr, *h_irr = nullptr;  // targets + [nt][6] results
     size_t irr_bytes = 0;
+    // 64-byte AoS copy of the j-set {x,y,z,vx,vy,vz,m,0} for the list
+    // gathers (one cache line per neighbour instead of seven); rebuilt when
+    // jset_version (bumped by every upload / predict) has moved on
+    char *d_aos = nullptr;
+    size_t aos_bytes = 0;
+    long long jset_version = 0, aos_version = -1;
 
     // split-j scratch
     size_t scratch_bytes = 0;

@@ -229,6 +229,7 @@ int abyss_ctx_predict(abyss_ctx_t c, double t_new)
     float *m32 = reinterpret_cast<float *>(jx + 8 * (size_t)n);
     hipLaunchKernelGGL(predict_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream,
                        state_view(c->d_state, n), n, t_new, jx, m32);
+    c->jset_version += 1;
     return hipGetLastError() == hipSuccess ? ABYSS_OK : ABYSS_ERR_HIP;
 }
 

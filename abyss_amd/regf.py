@@ -53,7 +53,7 @@ EXPORTS = [
     "abyss_srv_cluster_footprint", "abyss_srv_set_batch_budget", "abyss_ens_blocks",
     "abyss_ens_set_block_range", "abyss_autotune", "abyss_autotune_load", "abyss_autotune_clear",
     "abyss_ctx_upload_state", "abyss_ctx_update_state", "abyss_ctx_predict",
-    "abyss_ctx_keep_lists", "abyss_ctx_irregular",
+    "abyss_ctx_keep_lists", "abyss_ctx_irregular", "abyss_ctx_regular_step",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -148,6 +148,8 @@ def load():
         "abyss_ctx_predict": (ctypes.c_int, [_vp, ctypes.c_double]),
         "abyss_ctx_keep_lists": (ctypes.c_int, [_vp, ctypes.c_int]),
         "abyss_ctx_irregular": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp]),
+        "abyss_ctx_regular_step": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp,
+                                                  _ip, _ip, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -373,6 +375,25 @@ class Context:
         self._chk(self._lib.abyss_ctx_irregular(self.h, t.size, _i(t) if t.size else None,
                                                 _d(acc), _d(adot)))
         return acc, adot
+
+    def regular_step(self, targets, dt, a_old):
+        """Regular step with neighbour-set correction and regular corrector:
+        (acc, adot, nnb, lists[nt, cap], out[nt, 18])."""
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        nt = t.size
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (nt,)))
+        a_old = _f64(a_old, (nt, 6))
+        acc = np.zeros((max(nt, 1), 3))
+        adot = np.zeros((max(nt, 1), 3))
+        nnb = np.zeros(max(nt, 1), dtype=np.int32)
+        lst = np.full((max(nt, 1), self.cap), -1, dtype=np.int32)
+        out = np.zeros((max(nt, 1), 18))
+        self.last_status = self._chk(
+            self._lib.abyss_ctx_regular_step(self.h, nt, _i(t) if nt else None, _d(dt),
+                                             _d(a_old), _d(acc), _d(adot), _i(nnb), _i(lst),
+                                             _d(out)), allow_overflow=True)
+        self.nt = nt
+        return acc[:nt], adot[:nt], nnb[:nt], lst[:nt], out[:nt]
 
     def launch(self):
         self._chk(self._lib.abyss_ctx_launch(self.h))

@@ -315,6 +315,43 @@ def serve_stream(srv, ids, sizes, rate, window_s, seed=9):
     return np.array(lat), inter, wall
 
 
+def bench_neighbour(args, reps=5):
+    """Irregular force over resident lists and the fused regular step
+    (neighbour-set correction + corrector) on the King N=262144 workload,
+    all N particles per call; wall-clock per synchronous call."""
+    import numpy as np
+    from abyss_amd import regf, workloads
+    js = workloads.king_binaries(args.n)
+    t = np.arange(js.n, dtype=np.int32)
+    rng = np.random.default_rng(3)
+    dt = np.full(js.n, 1.0 / 1024)
+    a_old = rng.normal(size=(js.n, 6))
+    out = {"workload": f"{js.name}_fullN", "n": js.n}
+    with regf.Context(device=0, cap=js.cap) as ctx:
+        ctx.keep_lists(True)
+        ctx.upload(js.m, js.x, js.v, js.r2)
+        _, _, nnb, _ = ctx.query(t)
+        entries = int(np.minimum(np.abs(nnb), js.cap).sum())
+
+        def wall(f):
+            f()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                f()
+            return (time.perf_counter() - t0) / reps * 1e3
+
+        ms_irr = wall(lambda: ctx.irregular(t))
+        ms_query = wall(lambda: ctx.query(t))
+        ms_step = wall(lambda: ctx.regular_step(t, dt, a_old))
+    out["irregular"] = {"targets": js.n, "list_entries": entries, "ms_per_call": ms_irr,
+                        "entries_per_s": entries / (ms_irr * 1e-3),
+                        "includes": "target H2D, kernel, result D2H"}
+    out["regular_step"] = {"targets": js.n, "ms_per_call": ms_step, "query_ms_per_call": ms_query,
+                           "correction_overhead_ms": ms_step - ms_query}
+    log(f"neighbour legs {out}")
+    return out
+
+
 def bench_serve(args, rate=2000.0, window_s=1.0):
     import numpy as np
     from abyss_amd import regf
@@ -362,7 +399,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration)")
-    ap.add_argument("--workload", choices=["king262144", "ensemble", "serve"],
+    ap.add_argument("--workload", choices=["king262144", "ensemble", "serve", "neighbour"],
                     default="king262144")
     ap.add_argument("--serve-rate", type=float, default=100.0,
                     help="sustained request rate of the served-ensemble leg (req/s)")
@@ -373,6 +410,10 @@ def main():
     if args.workload == "ensemble":
         print(json.dumps({"metric": "batched ensemble regular-force pair-interactions/s",
                           "batched_ensemble": bench_ensemble(args)}), flush=True)
+        return
+    if args.workload == "neighbour":
+        print(json.dumps({"metric": "irregular force and fused regular step (wall-clock)",
+                          "neighbour": bench_neighbour(args)}), flush=True)
         return
     if args.workload == "serve":
         print(json.dumps({"metric": "served ensemble regular-step latency",

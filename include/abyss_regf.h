@@ -205,6 +205,31 @@ int abyss_ctx_keep_lists(abyss_ctx_t ctx, int enable);
 int abyss_ctx_irregular(abyss_ctx_t ctx, int nt, const int *targets, double *acc,
                         double *adot);
 
+/* One regular step with the neighbour-set correction and the fourth-order
+ * regular corrector fused on the device, in place of the caller's per-
+ * particle loop after CalculateAccelerationOnDevice
+ * (Particle::updateRegularParticleCuda,
+ * src_mpi/Particle/ComputeAcceleration.cpp:400-661).  Needs keep_lists on
+ * and the j-set predicted to the step's end time t_new (e.g. by
+ * abyss_ctx_predict).  For each target k (particle i):
+ *   - acc/adot/nnb/list: the new regular force and list, as abyss_ctx_query;
+ *   - with L_old = i's resident list (its last regular step; empty if none)
+ *     and L_new = the list just found, terms f_j on the current j-set:
+ *       a_irr = sum_{L_new} f_j,
+ *       a_tmp = sum_{L_new \ L_old} f_j - sum_{L_old \ L_new} f_j;
+ *   - with dt = dt[k] (> 0), a0 = a_old[k][0..2], j0 = a_old[k][3..5] (the
+ *     previous regular force and jerk), a1 = acc[k], j1 = adot[k]:
+ *       da_dt2 = (a0 - a1 - a_tmp)/dt^2, adot_dt = (j0 + j1 + adot_tmp)/dt,
+ *       a2 = -6 da_dt2 - 2 adot_dt - 2 j0/dt, a3 = (12 da_dt2 + 6 adot_dt)/dt,
+ *       x_new = x + a2 dt^4/24 + a3 dt^5/120, v_new = v + a2 dt^3/6 + a3 dt^4/24
+ *     with x, v the target's own j-set entry (at t_new);
+ *   - out[k][18] = x_new[3], v_new[3], a2[3], a3[3], a_irr[3], adot_irr[3];
+ *   - i's resident list becomes L_new.
+ * A neighbour at r = 0 adds nothing (the reference would divide by zero). */
+int abyss_ctx_regular_step(abyss_ctx_t ctx, int nt, const int *targets, const double *dt,
+                           const double *a_old, double *acc, double *adot, int *nnb,
+                           int *list, double *out);
+
 /* Plan autotuner.  Times candidate plans (lanes per target, j-unroll,
  * workgroups per CU -> j-split count) for target counts 1, 4, 16, ... n on
  * a synthetic n-particle cluster (n >= 256) and installs the fastest per

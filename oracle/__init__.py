@@ -35,6 +35,9 @@ def _load():
     lib.oracle_irregular.restype = None
     lib.oracle_irregular.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _ip, _ip, _ip,
                                      ctypes.c_int, _dp, _dp]
+    lib.oracle_regular_correct.restype = None
+    lib.oracle_regular_correct.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _ip, _dp,
+                                           _dp, _dp, _dp, _ip, _ip, _ip, _ip, ctypes.c_int, _dp]
     lib.oracle_max_threads.restype = ctypes.c_int
     _lib = lib
     return lib
@@ -123,6 +126,33 @@ def irregular(x, v, m, targets, nnb, lists, cap):
     lib.oracle_irregular(x.shape[0], _d(x), _d(v), _d(m), t.size, _i(t), _i(nnb), _i(lists),
                          cap, _d(acc), _d(adot))
     return acc, adot
+
+
+def regular_correct(x, v, m, targets, dt, a_old, new_acc, new_adot, old_nnb, old_lists,
+                    new_nnb, new_lists, cap):
+    """Neighbour-set correction + regular corrector: out [nt, 18] =
+    x_new, v_new, a2, a3, a_irr, adot_irr."""
+    lib = _load()
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    nt = t.size
+    dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (nt,)))
+    a_old = np.ascontiguousarray(a_old, dtype=np.float64).reshape(nt, 6)
+    new_acc = np.ascontiguousarray(new_acc, dtype=np.float64).reshape(nt, 3)
+    new_adot = np.ascontiguousarray(new_adot, dtype=np.float64).reshape(nt, 3)
+    old_nnb = np.ascontiguousarray(old_nnb, dtype=np.int32)
+    new_nnb = np.ascontiguousarray(new_nnb, dtype=np.int32)
+    old_lists = np.ascontiguousarray(old_lists, dtype=np.int32).reshape(-1)
+    new_lists = np.ascontiguousarray(new_lists, dtype=np.int32).reshape(-1)
+    assert old_nnb.size == nt and new_nnb.size == nt
+    assert old_lists.size >= nt * cap and new_lists.size >= nt * cap
+    out = np.empty((nt, 18))
+    lib.oracle_regular_correct(x.shape[0], _d(x), _d(v), _d(m), nt, _i(t), _d(dt), _d(a_old),
+                               _d(new_acc), _d(new_adot), _i(old_nnb), _i(old_lists),
+                               _i(new_nnb), _i(new_lists), cap, _d(out))
+    return out
 
 
 def max_threads():

@@ -27,7 +27,9 @@
  * written and nnb[k] = -(true count), the convention the HIP library uses.
  *
  * Parity pinning (tests/test_oracle.py): two-body closed forms,
- * regular + irregular = the unsplit direct sum (oracle_irregular), a = -grad(phi)
+ * regular + irregular = the unsplit direct sum (oracle_irregular), the
+ * corrector's cubic closed form and split invariance (oracle_regular_correct),
+ * a = -grad(phi)
  * and jdot = d a/dt by finite differences, momentum conservation, homogeneity
  * scalings, r_nb -> 0 / inf limits, the exact "<=" boundary case, coincident
  * particles, and a vectorised numpy brute force on small N.
@@ -215,6 +217,94 @@ void oracle_irregular(int n, const double *x, const double *v, const double *m, 
         for (int d = 0; d < 3; ++d) {
             acc[3 * k + d] = a[d];
             adot[3 * k + d] = ad[d];
+        }
+    }
+}
+
+/*
+ * Neighbour-set correction + fourth-order regular corrector of one regular
+ * step, per target, in the reference's loop and expression order
+ * (Particle::updateRegularParticleCuda,
+ * src_mpi/Particle/ComputeAcceleration.cpp:400-661):
+ *   for e < max(|L_old|, |L_new|):
+ *     e < |L_old| and L_old[e] not in L_new: a_tmp -= f(L_old[e])
+ *     e < |L_new|: a_irr += f(L_new[e]); if L_new[e] not in L_old: a_tmp += f
+ *   f: m_r3 = m/dr2/sqrt(dr2); a += m_r3 dx; adot += m_r3 (dv - 3 dx dxdv/dr2)
+ *   (dr2 == 0 adds nothing here; the reference would divide by zero)
+ *   da_dt2 = (a0 - a1 - a_tmp)/dt^2; adot_dt = (j0 + j1 + adot_tmp)/dt
+ *   a2 = -6 da_dt2 - 2 adot_dt - 2 j0/dt; a3 = (12 da_dt2 + 6 adot_dt)/dt
+ *   x_new = x + a2 dt^4/24 + a3 dt^5/120; v_new = v + a2 dt^3/6 + a3 dt^4/24
+ * x, v: the current (predicted) j-set; a_old [nt][6] = a0, j0; new_acc /
+ * new_adot [nt][3] = a1, j1; lists as rows of stride cap with counts
+ * (negative = overflowed, first cap entries used).
+ * out [nt][18] = x_new, v_new, a2, a3, a_irr, adot_irr.
+ */
+static int in_row(const int *row, int n, int j)
+{
+    for (int e = 0; e < n; ++e)
+        if (row[e] == j) return 1;
+    return 0;
+}
+
+static void corr_term(const double *x, const double *v, const double *m, int i, int j,
+                      double sgn, double *a, double *ad)
+{
+    double dx[3], dv[3], dr2 = 0.0, dxdv = 0.0;
+    for (int d = 0; d < 3; ++d) {
+        dx[d] = x[3 * j + d] - x[3 * i + d];
+        dv[d] = v[3 * j + d] - v[3 * i + d];
+        dr2 += dx[d] * dx[d];
+        dxdv += dx[d] * dv[d];
+    }
+    if (dr2 == 0.0) return;
+    const double m_r3 = m[j] / dr2 / sqrt(dr2);
+    for (int d = 0; d < 3; ++d) {
+        a[d] += sgn * (m_r3 * dx[d]);
+        ad[d] += sgn * (m_r3 * (dv[d] - 3 * dx[d] * dxdv / dr2));
+    }
+}
+
+void oracle_regular_correct(int n, const double *x, const double *v, const double *m, int nt,
+                            const int *targets, const double *dts, const double *a_old,
+                            const double *new_acc, const double *new_adot, const int *old_nnb,
+                            const int *old_list, const int *new_nnb, const int *new_list,
+                            int cap, double *out)
+{
+    (void)n;
+    for (int k = 0; k < nt; ++k) {
+        const int i = targets[k];
+        int co = old_nnb[k] < 0 ? -old_nnb[k] : old_nnb[k];
+        int cn = new_nnb[k] < 0 ? -new_nnb[k] : new_nnb[k];
+        if (co > cap) co = cap;
+        if (cn > cap) cn = cap;
+        const int *lo = old_list + (size_t)k * cap;
+        const int *ln = new_list + (size_t)k * cap;
+        double a_tmp[3] = {0, 0, 0}, adot_tmp[3] = {0, 0, 0};
+        double a_irr[3] = {0, 0, 0}, adot_irr[3] = {0, 0, 0};
+        const int size = co > cn ? co : cn;
+        for (int e = 0; e < size; ++e) {
+            if (e < co && !in_row(ln, cn, lo[e]))
+                corr_term(x, v, m, i, lo[e], -1.0, a_tmp, adot_tmp);
+            if (e < cn) {
+                corr_term(x, v, m, i, ln[e], 1.0, a_irr, adot_irr);
+                if (!in_row(lo, co, ln[e])) corr_term(x, v, m, i, ln[e], 1.0, a_tmp, adot_tmp);
+            }
+        }
+        const double dt = dts[k];
+        const double dt2 = dt * dt, dt3 = dt2 * dt, dt4 = dt3 * dt, dt5 = dt4 * dt;
+        double *o = out + (size_t)k * 18;
+        for (int d = 0; d < 3; ++d) {
+            const double a0 = a_old[6 * (size_t)k + d], j0 = a_old[6 * (size_t)k + 3 + d];
+            const double da_dt2 = (a0 - new_acc[3 * (size_t)k + d] - a_tmp[d]) / dt2;
+            const double adot_dt = (j0 + new_adot[3 * (size_t)k + d] + adot_tmp[d]) / dt;
+            const double a2 = -6 * da_dt2 - 2 * adot_dt - 2 * j0 / dt;
+            const double a3 = (12 * da_dt2 + 6 * adot_dt) / dt;
+            o[d] = x[3 * i + d] + a2 * dt4 / 24 + a3 * dt5 / 120;
+            o[3 + d] = v[3 * i + d] + a2 * dt3 / 6 + a3 * dt4 / 24;
+            o[6 + d] = a2;
+            o[9 + d] = a3;
+            o[12 + d] = a_irr[d];
+            o[15 + d] = adot_irr[d];
         }
     }
 }

@@ -170,3 +170,112 @@ def test_irregular_errors_and_resets():
         c.keep_lists(False)
         with pytest.raises(regf.AbyssError):
             c.irregular([3])
+
+
+def _step_case(ctx, js, t, r2_new, dt, a_old, label):
+    """Regular step after a first launch with js.r2 (the old lists): the GPU
+    correction against the oracle's, fed with the GPU's own lists and new
+    regular force (so only the correction itself is compared)."""
+    _, _, no, lo = ctx.query(t)                 # old lists, kept resident
+    ctx.upload(js.m, js.x, js.v, r2_new)        # same n: resident lists stay
+    acc, adot, nn, ln, out = ctx.regular_step(t, dt, a_old)
+    ref = oracle.regular_correct(js.x, js.v, js.m, t, dt, a_old, acc, adot, no, lo, nn, ln,
+                                 js.cap)
+    # summation order differs (lanes + butterfly vs list order): bound by the
+    # summed magnitudes of every term either list contributes
+    so, _ = _scales(js.x, js.v, js.m, t, no, lo, js.cap)
+    sn, sjn = _scales(js.x, js.v, js.m, t, nn, ln, js.cap)
+    scale = (np.linalg.norm(a_old[:, :3], axis=1) + np.linalg.norm(acc, axis=1) + so + sn)
+    e2 = np.linalg.norm(out[:, 6:9] - ref[:, 6:9], axis=1) * dt ** 2 / scale
+    e3 = np.linalg.norm(out[:, 9:12] - ref[:, 9:12], axis=1) * dt ** 3 / scale
+    ex = np.linalg.norm(out[:, 0:3] - ref[:, 0:3], axis=1)
+    ev = np.linalg.norm(out[:, 3:6] - ref[:, 3:6], axis=1)
+    ei = _norm_err(out[:, 12:15], ref[:, 12:15], sn)
+    ej = _norm_err(out[:, 15:18], ref[:, 15:18], sjn)
+    gained = sum(len(set(ln[q, :abs(nn[q])]) - set(lo[q, :abs(no[q])])) for q in range(len(t)))
+    lost = sum(len(set(lo[q, :abs(no[q])]) - set(ln[q, :abs(nn[q])])) for q in range(len(t)))
+    say(f"[{label}] nt={len(t)} gained={gained} lost={lost} a2_err={e2.max():.2e} "
+        f"a3_err={e3.max():.2e} x_err={ex.max():.2e} v_err={ev.max():.2e} "
+        f"irr={ei.max():.2e}/{ej.max():.2e}")
+    assert e2.max() <= 1e-13 and e3.max() <= 1e-12
+    xs = np.abs(js.x[t]).max() + 1.0
+    vs = np.abs(js.v[t]).max() + 1.0
+    assert ex.max() <= 1e-14 * xs and ev.max() <= 1e-14 * vs
+    assert ei.max() <= TOL_IRR_NORM and ej.max() <= TOL_IRR_NORM
+    # the resident lists are now the new ones: the irregular force over them
+    # is the step's a_irr, bit for bit (same terms, same lane order)
+    ai, ji = ctx.irregular(t)
+    assert np.array_equal(ai, out[:, 12:15]) and np.array_equal(ji, out[:, 15:18])
+    return gained, lost
+
+
+@pytest.mark.parametrize("size", [1, 100, 8192])
+def test_regular_step_matches_oracle_kroupa_65536(ctx, size):
+    say(f"case: regular_step plummer_kroupa N=65536 block={size}")
+    js = workloads.plummer_kroupa(65536)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    rng = np.random.default_rng(size + 31)
+    t = rng.permutation(js.n)[:size].astype(np.int32)
+    r2_new = js.r2 * np.where(rng.uniform(size=js.n) < 0.5, 1.4, 0.7)
+    dt = rng.uniform(1e-3, 1e-2, size)
+    a_old = rng.normal(size=(size, 6))
+    gained, lost = _step_case(ctx, js, t, r2_new, dt, a_old, f"step/{size}")
+    if size >= 100:
+        assert gained > 0 and lost > 0
+
+
+def test_regular_step_first_step_and_overflow():
+    say("case: regular_step without old lists, and with overflowed lists (cap=16)")
+    js = workloads.plummer_equal(1024)
+    rng = np.random.default_rng(5)
+    t = np.arange(0, js.n, 3, dtype=np.int32)
+    dt = np.full(t.size, 0.01)
+    a_old = rng.normal(size=(t.size, 6))
+    with regf.Context(device=0, cap=256) as c:
+        c.keep_lists(True)
+        c.upload(js.m, js.x, js.v, js.r2)
+        # no old lists: every new neighbour is gained
+        acc, adot, nn, ln, out = c.regular_step(t, dt, a_old)
+        empty = np.zeros((t.size, 256), dtype=np.int32)
+        ref = oracle.regular_correct(js.x, js.v, js.m, t, dt, a_old, acc, adot,
+                                     np.zeros(t.size, np.int32), empty, nn, ln, 256)
+        np.testing.assert_allclose(out[:, 12:], ref[:, 12:], rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(out[:, 6:12], ref[:, 6:12], rtol=1e-9, atol=1e-9)
+    with regf.Context(device=0, cap=16) as c:
+        c.keep_lists(True)
+        c.upload(js.m, js.x, js.v, js.r2)
+        _, _, no, lo = c.query(t)
+        c.upload(js.m, js.x, js.v, js.r2 * 1.3)
+        acc, adot, nn, ln, out = c.regular_step(t, dt, a_old)
+        assert c.last_status == regf.ERR_NEIGHBOR_OVERFLOW and np.all(nn < 0)
+        ref = oracle.regular_correct(js.x, js.v, js.m, t, dt, a_old, acc, adot, no, lo, nn, ln,
+                                     16)
+        np.testing.assert_allclose(out[:, 12:], ref[:, 12:], rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(out[:, 6:12], ref[:, 6:12], rtol=1e-9, atol=1e-9)
+
+
+def test_regular_step_errors():
+    say("case: regular_step invalid inputs")
+    js = workloads.plummer_equal(1024)
+    with regf.Context(device=0, cap=256) as c:
+        c.upload(js.m, js.x, js.v, js.r2)
+        t = np.arange(8, dtype=np.int32)
+        a_old = np.zeros((8, 6))
+        with pytest.raises(regf.AbyssError) as e:
+            c.regular_step(t, 0.01, a_old)
+        assert e.value.code == regf.ERR_NO_JSET
+        c.keep_lists(True)
+        for bad_dt in (0.0, -1.0, np.nan):
+            with pytest.raises(regf.AbyssError) as e:
+                c.regular_step(t, bad_dt, a_old)
+            assert e.value.code == regf.ERR_INVALID_INPUT
+        bad = a_old.copy()
+        bad[3, 2] = np.inf
+        with pytest.raises(regf.AbyssError) as e:
+            c.regular_step(t, 0.01, bad)
+        assert e.value.code == regf.ERR_INVALID_INPUT
+        with pytest.raises(regf.AbyssError) as e:
+            c.regular_step(np.array([0, js.n], dtype=np.int32), 0.01, np.zeros((2, 6)))
+        assert e.value.code == regf.ERR_TARGET_RANGE
+        # still usable afterwards
+        c.regular_step(t, 0.01, a_old)

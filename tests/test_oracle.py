@@ -272,3 +272,64 @@ def test_irregular_empty_truncated_and_coincident():
     lists = np.array([[1, 2, 3]], dtype=np.int32)
     acc, _ = oracle.irregular(x, v, m, [0], [3], lists, 3)
     np.testing.assert_allclose(acc[0], [2.0, 1.0, 0], rtol=0, atol=1e-15)
+
+
+def test_regular_corrector_cubic_closed_form():
+    # no neighbours (a_tmp = 0): for a regular force that is a cubic in time,
+    # a(t) = A0 + A1 t + A2 t^2/2 + A3 t^3/6, the corrector recovers a2 = A2,
+    # a3 = A3 and moves the predicted x, v by the remaining Taylor terms
+    x = np.array([[0.0, 0.0, 0.0], [10.0, 0.0, 0.0]])
+    v = np.array([[0.1, 0.2, 0.3], [0.0, 0.0, 0.0]])
+    m = np.array([1.0, 1.0])
+    A0, A1 = np.array([0.3, -0.2, 0.1]), np.array([1.5, 0.25, -0.75])
+    A2, A3 = np.array([-2.0, 0.5, 4.0]), np.array([8.0, -3.0, 1.0])
+    dt = 0.125
+    a1 = A0 + A1 * dt + A2 * dt ** 2 / 2 + A3 * dt ** 3 / 6
+    j1 = A1 + A2 * dt + A3 * dt ** 2 / 2
+    empty = np.zeros((1, 4), dtype=np.int32)
+    out = oracle.regular_correct(x, v, m, [0], dt, np.concatenate([A0, A1])[None], a1[None],
+                                 j1[None], [0], empty, [0], empty, 4)[0]
+    np.testing.assert_allclose(out[6:9], A2, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(out[9:12], A3, rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(out[0:3] - x[0], A2 * dt ** 4 / 24 + A3 * dt ** 5 / 120,
+                               rtol=1e-10, atol=1e-16)
+    np.testing.assert_allclose(out[3:6] - v[0], A2 * dt ** 3 / 6 + A3 * dt ** 4 / 24,
+                               rtol=1e-10, atol=1e-16)
+    assert np.all(out[12:] == 0)
+
+
+def test_regular_corrector_is_invariant_under_the_split():
+    # Moving pairs between the regular and irregular parts must not change
+    # the corrector: the step (L_old -> L_new, new force of the L_new split)
+    # equals the step (L_old -> L_old, new force of the L_old split), because
+    # a_tmp converts the new force back to the old neighbour set.  Radii grow
+    # for half the targets and shrink for the rest, so pairs are both gained
+    # and lost.
+    x, v, m = _rand_cluster(300, 61)
+    n = len(m)
+    rng = np.random.default_rng(67)
+    r2_old = rng.uniform(0.1, 1.5, n)
+    r2_new = r2_old * np.where(rng.uniform(size=n) < 0.5, 1.6, 0.55)
+    t = rng.permutation(n)[:150]
+    ao, jo, no, lo = oracle.regf(x, v, m, r2_old, t, cap=n)
+    an, jn, nn, ln = oracle.regf(x, v, m, r2_new, t, cap=n)
+    gained = sum(len(set(ln[q, :nn[q]]) - set(lo[q, :no[q]])) for q in range(len(t)))
+    lost = sum(len(set(lo[q, :no[q]]) - set(ln[q, :nn[q]])) for q in range(len(t)))
+    assert gained > 100 and lost > 100
+    dt = rng.uniform(0.01, 0.05, len(t))
+    a_old = np.concatenate([ao, jo], axis=1) + rng.normal(size=(len(t), 6)) * 0.01
+    o1 = oracle.regular_correct(x, v, m, t, dt, a_old, an, jn, no, lo, nn, ln, n)
+    o2 = oracle.regular_correct(x, v, m, t, dt, a_old, ao, jo, no, lo, no, lo, n)
+    sa = np.abs(ao).max()
+    np.testing.assert_allclose(o1[:, 6:9], o2[:, 6:9], rtol=0, atol=1e-11 * sa / dt.min() ** 2)
+    np.testing.assert_allclose(o1[:, 9:12], o2[:, 9:12], rtol=0, atol=1e-11 * sa / dt.min() ** 3)
+    np.testing.assert_allclose(o1[:, 0:6], o2[:, 0:6], rtol=1e-13, atol=1e-15)
+    # the irregular part is the sum over the new list
+    ai, ji = oracle.irregular(x, v, m, t, nn, ln, n)
+    np.testing.assert_allclose(o1[:, 12:15], ai, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(o1[:, 15:18], ji, rtol=1e-11, atol=1e-13)
+    # and with the old split the correction vanishes: a2 from a0 - a1 alone
+    da = (a_old[:, :3] - ao) / dt[:, None] ** 2
+    ad = (a_old[:, 3:] + jo) / dt[:, None]
+    np.testing.assert_allclose(o2[:, 6:9], -6 * da - 2 * ad - 2 * a_old[:, 3:] / dt[:, None],
+                               rtol=1e-12, atol=1e-12 * np.abs(da).max())
