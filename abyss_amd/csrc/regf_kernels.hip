@@ -28,6 +28,7 @@
 // Neighbours are appended to the lane's own list in ascending j order; the
 // append branch is taken only for neighbours, so the self test (j != i)
 // costs nothing on the common path.
+#include "../../include/abyss_regf.h"
 #include "regf_internal.h"
 
 namespace abyss {
@@ -286,31 +287,70 @@ __global__ __launch_bounds__(kBlock) void regf_kernel(JState js, const int *__re
                            0, nt, s_pos, s_m);
 }
 
-// Small-block variant: JL lanes share one target (JL = 4, 16 or 64), so a
-// wave serves 64/JL targets and a few active particles still fill the chip.
-// In a tile, step t hands the group's lanes the JL consecutive records
-// t*JL + sub (sub = lane mod JL).  Neighbours of a step are compacted with a
-// wave ballot: a lane's slot is the group's running count plus the number
-// of neighbour lanes below it in the group, which keeps every list in
-// ascending j.  Each lane keeps FP32 partials folded into FP64 at the end of
-// the tile in which it reaches kFold pairs; at the end the JL FP64 sums of a
-// group are combined by a fixed xor-butterfly (bitwise reproducible).
+// Lanes-per-target variant: JL lanes share one target (JL = 4, 16 or 64),
+// so a wave serves 64/JL targets and a few active particles still fill the
+// chip.  In a tile, step t hands the group's lanes the JL consecutive
+// records t*JL + sub (sub = lane mod JL).  Neighbours of a step are
+// compacted with a wave ballot: a lane's slot is the group's running count
+// plus the number of neighbour lanes below it in the group, which keeps
+// every list in ascending j.  At the end the JL FP64 sums of a group are
+// combined by a fixed xor-butterfly (bitwise reproducible).
+//
+// Term modes TM:
+//   0  FP64 differences rounded to FP32; r^2, rsqrt and terms in FP32; FP32
+//      partials folded into FP64 sums at the end of the tile in which a lane
+//      reaches kFold pairs
+//   1  FP64 differences, r^2 and terms, accumulated in FP64; rsqrt is the
+//      FP32 v_rsq_f32 of the rounded r^2 (about 1e-7 relative); the
+//      neighbour test compares FP64 r^2 with FP64 r_nb^2
+//   2  as 1, with one Newton step y += y (1 - r^2 y^2) / 2 in FP64 on the
+//      rsqrt (relative error ~1e-14)
+// FP64 vector instructions issue at the FP32 rate on MI355X, so modes 1 and
+// 2 trade the six FP64->FP32 conversions for FP64 arithmetic.
+template <int TM>
+struct TermMass { using type = double; };
+template <>
+struct TermMass<0> { using type = float; };
+
 // Per-lane state of the lanes-per-target kernel.
 struct JLLane {
     double xi, yi, zi, vxi, vyi, vzi;
-    float rnbf;
+    double rnbd;     // r_nb^2 (modes 1, 2)
+    float rnbf;      // r_nb^2 (mode 0)
     int idx, sub, cnt, cap;
     int *lst;
     unsigned long long gmask, below;
-    float pa[6];
+    float pa[6];     // mode 0: FP32 partials
+    double acc[6];   // FP64 sums
 };
+
+// Rare branch of a step: zero the near lanes' rsqrt here rather than with a
+// select on every pair (volatile keeps it inside the branch), and append the
+// listed neighbours in lane order.
+template <bool FULL, bool SELF>
+__device__ __forceinline__ void jl_near(JLLane &L, bool near, unsigned long long nb, int r,
+                                        int count, int jt, float &rinv)
+{
+    if (near) asm volatile("v_mov_b32 %0, 0" : "=v"(rinv));
+    bool listed = near;
+    if (!FULL) listed = listed && r < count;          // zero-filled records
+    if (SELF) listed = listed && jt + r != L.idx;     // self-exclusion by index
+    const unsigned long long bal = (FULL && !SELF) ? nb : __ballot(listed);
+    const unsigned long long gb = bal & L.gmask;
+    if (listed) {
+        const int p = L.cnt + __popcll(gb & L.below);
+        if (p < L.cap) L.lst[p] = jt + r;
+    }
+    L.cnt += __popcll(gb);
+}
 
 // Steps of one tile.  FULL: all kTile records are valid; SELF: the tile may
 // hold a lane's own target.  The common case (FULL, !SELF) lists exactly the
 // near pairs, so its only extra instruction over the force is the compare.
-template <int JL, bool FULL, bool SELF>
+template <int JL, int TM, bool FULL, bool SELF>
 __device__ __forceinline__ void jl_tile(JLLane &L, const double2 *__restrict__ pos,
-                                        const float *__restrict__ mass, int count, int jt)
+                                        const typename TermMass<TM>::type *__restrict__ mass,
+                                        int count, int jt)
 {
     const int steps = FULL ? kTile / JL : (count + JL - 1) / JL;
 #pragma unroll 2
@@ -319,52 +359,69 @@ __device__ __forceinline__ void jl_tile(JLLane &L, const double2 *__restrict__ p
         const double2 p0 = pos[3 * r + 0];
         const double2 p1 = pos[3 * r + 1];
         const double2 p2 = pos[3 * r + 2];
-        const float mj = mass[r];
-        const float dx = (float)(p0.x - L.xi);
-        const float dy = (float)(p0.y - L.yi);
-        const float dz = (float)(p1.x - L.zi);
-        const float dvx = (float)(p1.y - L.vxi);
-        const float dvy = (float)(p2.x - L.vyi);
-        const float dvz = (float)(p2.y - L.vzi);
-        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        float rinv = __builtin_amdgcn_rsqf(r2);
-        const bool near = !(r2 > L.rnbf);
-        const unsigned long long nb = __ballot(near);
-        if (nb) {
-            // rare: zero rinv of the near lanes here rather than with a
-            // select on every pair (volatile keeps it inside the branch)
-            if (near) asm volatile("v_mov_b32 %0, 0" : "=v"(rinv));
-            bool listed = near;
-            if (!FULL) listed = listed && r < count;          // zero-filled records
-            if (SELF) listed = listed && jt + r != L.idx;     // self-exclusion by index
-            const unsigned long long bal = (FULL && !SELF) ? nb : __ballot(listed);
-            const unsigned long long gb = bal & L.gmask;
-            if (listed) {
-                const int p = L.cnt + __popcll(gb & L.below);
-                if (p < L.cap) L.lst[p] = jt + r;
+        if constexpr (TM == 0) {
+            const float mj = mass[r];
+            const float dx = (float)(p0.x - L.xi);
+            const float dy = (float)(p0.y - L.yi);
+            const float dz = (float)(p1.x - L.zi);
+            const float dvx = (float)(p1.y - L.vxi);
+            const float dvy = (float)(p2.x - L.vyi);
+            const float dvz = (float)(p2.y - L.vzi);
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            float rinv = __builtin_amdgcn_rsqf(r2);
+            const bool near = !(r2 > L.rnbf);
+            const unsigned long long nb = __ballot(near);
+            if (nb) jl_near<FULL, SELF>(L, near, nb, r, count, jt, rinv);
+            const float rinv2 = rinv * rinv;
+            const float mr3 = mj * rinv * rinv2;
+            const float rv = fmaf(dz, dvz, fmaf(dy, dvy, dx * dvx));
+            const float al = -3.0f * rv * rinv2;
+            L.pa[0] = fmaf(mr3, dx, L.pa[0]);
+            L.pa[1] = fmaf(mr3, dy, L.pa[1]);
+            L.pa[2] = fmaf(mr3, dz, L.pa[2]);
+            L.pa[3] = fmaf(mr3, fmaf(al, dx, dvx), L.pa[3]);
+            L.pa[4] = fmaf(mr3, fmaf(al, dy, dvy), L.pa[4]);
+            L.pa[5] = fmaf(mr3, fmaf(al, dz, dvz), L.pa[5]);
+        } else {
+            const double mj = mass[r];
+            const double dx = p0.x - L.xi;
+            const double dy = p0.y - L.yi;
+            const double dz = p1.x - L.zi;
+            const double dvx = p1.y - L.vxi;
+            const double dvy = p2.x - L.vyi;
+            const double dvz = p2.y - L.vzi;
+            const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
+            float rf = __builtin_amdgcn_rsqf((float)r2);
+            const bool near = !(r2 > L.rnbd);
+            const unsigned long long nb = __ballot(near);
+            if (nb) jl_near<FULL, SELF>(L, near, nb, r, count, jt, rf);
+            double rinv = (double)rf;
+            if constexpr (TM == 2) {
+                const double h = fma(-(r2 * rinv), rinv, 1.0);
+                rinv = fma(0.5 * rinv, h, rinv);
             }
-            L.cnt += __popcll(gb);
+            const double rinv2 = rinv * rinv;
+            const double mr3 = mj * rinv * rinv2;
+            const double rv = fma(dz, dvz, fma(dy, dvy, dx * dvx));
+            const double al = -3.0 * rv * rinv2;
+            L.acc[0] = fma(mr3, dx, L.acc[0]);
+            L.acc[1] = fma(mr3, dy, L.acc[1]);
+            L.acc[2] = fma(mr3, dz, L.acc[2]);
+            L.acc[3] = fma(mr3, fma(al, dx, dvx), L.acc[3]);
+            L.acc[4] = fma(mr3, fma(al, dy, dvy), L.acc[4]);
+            L.acc[5] = fma(mr3, fma(al, dz, dvz), L.acc[5]);
         }
-        const float rinv2 = rinv * rinv;
-        const float mr3 = mj * rinv * rinv2;
-        const float rv = fmaf(dz, dvz, fmaf(dy, dvy, dx * dvx));
-        const float al = -3.0f * rv * rinv2;
-        L.pa[0] = fmaf(mr3, dx, L.pa[0]);
-        L.pa[1] = fmaf(mr3, dy, L.pa[1]);
-        L.pa[2] = fmaf(mr3, dz, L.pa[2]);
-        L.pa[3] = fmaf(mr3, fmaf(al, dx, dvx), L.pa[3]);
-        L.pa[4] = fmaf(mr3, fmaf(al, dy, dvy), L.pa[4]);
-        L.pa[5] = fmaf(mr3, fmaf(al, dz, dvz), L.pa[5]);
     }
 }
 
-template <int JL>
+template <int JL, int TM>
 __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__restrict__ tgt, int nt,
                                              int jspan, const QueryOut &out, int cap,
                                              int final_counts, int bx, int s, int q_base,
-                                             int nt_stride,
-                                             double2 (*s_pos)[kTile * 3], float (*s_m)[kTile])
+                                             int nt_stride, double2 (*s_pos)[kTile * 3],
+                                             typename TermMass<TM>::type (*s_m)[kTile])
 {
+    using MT = typename TermMass<TM>::type;
     constexpr int TW = 64 / JL;            // targets per wave
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -381,22 +438,26 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
     L.cnt = 0;
     L.xi = L.yi = L.zi = L.vxi = L.vyi = L.vzi = 0.0;
     L.rnbf = -1.0f;   // inactive target: every j regular, nothing listed
+    L.rnbd = -1.0;
     L.idx = -1;
     L.lst = nullptr;
     if (q < nt) {
         L.idx = tgt[q];
         L.xi = js.x[L.idx]; L.yi = js.y[L.idx]; L.zi = js.z[L.idx];
         L.vxi = js.vx[L.idx]; L.vyi = js.vy[L.idx]; L.vzi = js.vz[L.idx];
-        L.rnbf = (float)js.rnb2[L.idx];
+        L.rnbd = js.rnb2[L.idx];
+        L.rnbf = (float)L.rnbd;
         L.lst = out.list + ((size_t)s * nt_stride + (q - q_base)) * (size_t)cap;
     }
-    double acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-    for (int c = 0; c < 6; ++c) L.pa[c] = 0.0f;
+    for (int c = 0; c < 6; ++c) {
+        L.acc[c] = 0.0;
+        L.pa[c] = 0.0f;
+    }
     int since_fold = 0;
 
     const int ntile = (j1 - j0 + kTile - 1) / kTile;
-    TileLoader<float> ld;
+    TileLoader<MT> ld;
     if (ntile > 0) {
         ld.load(js, j0 + tid, j1);
         ld.store(s_pos[0], s_m[0], tid);
@@ -408,27 +469,31 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
         const int jt = j0 + t * kTile;
         const int count = min(kTile, j1 - jt);
         if (__any((unsigned)(L.idx - jt) < (unsigned)count))
-            jl_tile<JL, false, true>(L, s_pos[b], s_m[b], count, jt);
+            jl_tile<JL, TM, false, true>(L, s_pos[b], s_m[b], count, jt);
         else if (count == kTile)
-            jl_tile<JL, true, false>(L, s_pos[b], s_m[b], count, jt);
+            jl_tile<JL, TM, true, false>(L, s_pos[b], s_m[b], count, jt);
         else
-            jl_tile<JL, false, false>(L, s_pos[b], s_m[b], count, jt);
-        // fold the FP32 partials once a lane has summed >= kFold pairs
-        // (every tile for JL = 4, every 4 / 16 tiles for JL = 16 / 64)
-        since_fold += (count + JL - 1) / JL;
-        if (since_fold >= kFold) {
+            jl_tile<JL, TM, false, false>(L, s_pos[b], s_m[b], count, jt);
+        if constexpr (TM == 0) {
+            // fold the FP32 partials once a lane has summed >= kFold pairs
+            // (every tile for JL = 4, every 4 / 16 tiles for JL = 16 / 64)
+            since_fold += (count + JL - 1) / JL;
+            if (since_fold >= kFold) {
 #pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                acc[c] += (double)L.pa[c];
-                L.pa[c] = 0.0f;
+                for (int c = 0; c < 6; ++c) {
+                    L.acc[c] += (double)L.pa[c];
+                    L.pa[c] = 0.0f;
+                }
+                since_fold = 0;
             }
-            since_fold = 0;
         }
         if (t + 1 < ntile) ld.store(s_pos[b ^ 1], s_m[b ^ 1], tid);
         __syncthreads();
     }
+    double acc[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) acc[c] += (double)L.pa[c];
+    for (int c = 0; c < 6; ++c)
+        acc[c] = TM == 0 ? L.acc[c] + (double)L.pa[c] : L.acc[c];
 #pragma unroll
     for (int d = JL / 2; d >= 1; d >>= 1)
 #pragma unroll
@@ -445,15 +510,17 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
     }
 }
 
-template <int JL>
-__global__ __launch_bounds__(kBlock) void regf_kernel_jl(JState js, const int *__restrict__ tgt,
-                                                         int nt, int jspan, QueryOut out, int cap,
-                                                         int final_counts)
+// 5 waves per SIMD (<= 96 VGPRs); the FP64-term modes would otherwise take
+// 98 registers and drop to 4.
+template <int JL, int TM>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+void regf_kernel_jl(JState js, const int *__restrict__ tgt, int nt, int jspan, QueryOut out,
+                    int cap, int final_counts)
 {
     __shared__ double2 s_pos[2][kTile * 3];
-    __shared__ float s_m[2][kTile];
-    regf_body_jl<JL>(js, tgt, nt, jspan, out, cap, final_counts, blockIdx.x, blockIdx.y, 0, nt,
-                     s_pos, s_m);
+    __shared__ typename TermMass<TM>::type s_m[2][kTile];
+    regf_body_jl<JL, TM>(js, tgt, nt, jspan, out, cap, final_counts, blockIdx.x, blockIdx.y, 0,
+                         nt, s_pos, s_m);
 }
 
 // One wave per target.  Partial sums: lane l adds splits l, l+64, ... in
@@ -606,15 +673,15 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const ClusterDesc *__rest
                                        w.split, d.q_base, d.nt_stride, s_pos, s_m);
                 break;
             case 4:
-                regf_body_jl<4>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock, w.split,
+                regf_body_jl<4, 0>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock, w.split,
                                 d.q_base, d.nt_stride, s_pos, s_m);
                 break;
             case 16:
-                regf_body_jl<16>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock, w.split,
+                regf_body_jl<16, 0>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock, w.split,
                                  d.q_base, d.nt_stride, s_pos, s_m);
                 break;
             default:
-                regf_body_jl<64>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock, w.split,
+                regf_body_jl<64, 0>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock, w.split,
                                  d.q_base, d.nt_stride, s_pos, s_m);
                 break;
         }
@@ -663,22 +730,42 @@ hipError_t launch_t(const JState &js, const int *tgt, int nt, const Plan &p, con
     return hipGetLastError();
 }
 
-}  // namespace
-
 template <int JL>
 hipError_t launch_jl(const JState &js, const int *tgt, int nt, const Plan &p, const QueryOut &out,
-                     int cap, hipStream_t st)
+                     int cap, int tm, hipStream_t st)
 {
     dim3 grid(p.grid_x, p.splits);
-    hipLaunchKernelGGL((regf_kernel_jl<JL>), grid, dim3(kBlock), 0, st, js, tgt, nt, p.jspan,
-                       out, cap, p.splits == 1 ? 1 : 0);
+    const int fc = p.splits == 1 ? 1 : 0;
+    if (tm == 0)
+        hipLaunchKernelGGL((regf_kernel_jl<JL, 0>), grid, dim3(kBlock), 0, st, js, tgt, nt,
+                           p.jspan, out, cap, fc);
+    else if (tm == 1)
+        hipLaunchKernelGGL((regf_kernel_jl<JL, 1>), grid, dim3(kBlock), 0, st, js, tgt, nt,
+                           p.jspan, out, cap, fc);
+    else
+        hipLaunchKernelGGL((regf_kernel_jl<JL, 2>), grid, dim3(kBlock), 0, st, js, tgt, nt,
+                           p.jspan, out, cap, fc);
     return hipGetLastError();
 }
 
+// Term mode of the lanes-per-target kernel for a precision mode.
+int term_mode(int prec)
+{
+    switch (prec) {
+        case ABYSS_PREC_MIXED: return 1;
+        case ABYSS_PREC_REFINED: return 2;
+        default: return 0;
+    }
+}
+
+}  // namespace
+
 bool variant_supported(int prec, int ipl, int unroll, int jl)
 {
-    if (jl != 1) return prec == 0 && (jl == 4 || jl == 16 || jl == 64);
-    if (prec == 1) return (ipl == 1 || ipl == 2) && unroll == 2;
+    if (prec == ABYSS_PREC_MIXED || prec == ABYSS_PREC_REFINED)
+        return jl == 4 || jl == 16 || jl == 64;
+    if (jl != 1) return prec == ABYSS_PREC_SPLIT && (jl == 4 || jl == 16 || jl == 64);
+    if (prec == ABYSS_PREC_FP64) return (ipl == 1 || ipl == 2) && unroll == 2;
     return (ipl == 1 || ipl == 2 || ipl == 4) && (unroll == 2 || unroll == 4);
 }
 
@@ -691,10 +778,11 @@ hipError_t launch_regf(const JState &js, const int *tgt, int nt, const Plan &p,
                        const QueryOut &out, int cap, int prec, hipStream_t st)
 {
     if (!variant_supported(prec, p.ipl, p.unroll, p.jl)) return hipErrorInvalidValue;
-    if (p.jl == 4) return launch_jl<4>(js, tgt, nt, p, out, cap, st);
-    if (p.jl == 16) return launch_jl<16>(js, tgt, nt, p, out, cap, st);
-    if (p.jl == 64) return launch_jl<64>(js, tgt, nt, p, out, cap, st);
-    if (prec == 1) {
+    const int tm = term_mode(prec);
+    if (p.jl == 4) return launch_jl<4>(js, tgt, nt, p, out, cap, tm, st);
+    if (p.jl == 16) return launch_jl<16>(js, tgt, nt, p, out, cap, tm, st);
+    if (p.jl == 64) return launch_jl<64>(js, tgt, nt, p, out, cap, tm, st);
+    if (prec == ABYSS_PREC_FP64) {
         if (p.ipl == 1) return launch_t<1, 2, true>(js, tgt, nt, p, out, cap, st);
         return launch_t<2, 2, true>(js, tgt, nt, p, out, cap, st);
     }

@@ -119,6 +119,9 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
         else jl = 64;
     }
     if (c->precision == ABYSS_PREC_FP64) jl = 1;
+    // the FP64-term modes exist only in the lanes-per-target kernel
+    if ((c->precision == ABYSS_PREC_MIXED || c->precision == ABYSS_PREC_REFINED) && jl == 1)
+        jl = 4;
     // One target per lane (ipl = 1, unroll 2) measured fastest at every
     // block size on MI355X; ipl 2/4 keep more targets in registers but cut
     // occupancy to 3 and 1 waves per SIMD.
@@ -318,7 +321,7 @@ int abyss_ctx_destroy(abyss_ctx_t c)
 
 int abyss_ctx_set_precision(abyss_ctx_t c, int mode)
 {
-    if (!c || (mode != ABYSS_PREC_SPLIT && mode != ABYSS_PREC_FP64)) return ABYSS_ERR_INVALID_ARG;
+    if (!c || mode < ABYSS_PREC_SPLIT || mode > ABYSS_PREC_REFINED) return ABYSS_ERR_INVALID_ARG;
     c->precision = mode;
     c->plan_valid = false;
     return ABYSS_OK;
@@ -661,7 +664,7 @@ int abyss_set_num_neighbor_max(int cap)
 
 int abyss_set_precision(int mode)
 {
-    if (mode != ABYSS_PREC_SPLIT && mode != ABYSS_PREC_FP64) return ABYSS_ERR_INVALID_ARG;
+    if (mode < ABYSS_PREC_SPLIT || mode > ABYSS_PREC_REFINED) return ABYSS_ERR_INVALID_ARG;
     g_prec = mode;
     if (g_ctx) abyss_ctx_set_precision(g_ctx, mode);
     return ABYSS_OK;

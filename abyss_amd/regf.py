@@ -33,6 +33,8 @@ ERR_BUSY = 11
 
 PREC_SPLIT = 0
 PREC_FP64 = 1
+PREC_MIXED = 2
+PREC_REFINED = 3
 
 # Every exported symbol of include/abyss_regf.h (checked by the CPU tests).
 EXPORTS = [

@@ -29,7 +29,11 @@
  * Precision (default ABYSS_PREC_SPLIT): positions and velocities are FP64 on
  * the device and the i-j differences are taken in FP64; r^2, rsqrt and the
  * force/jerk terms are FP32, summed in FP32 over short runs of j and folded
- * into FP64 per-particle sums.  ABYSS_PREC_FP64 evaluates everything in FP64.
+ * into FP64 per-particle sums.  ABYSS_PREC_MIXED keeps r^2, the neighbour
+ * test and the terms in FP64 as well (sums in FP64); only rsqrt is the FP32
+ * hardware estimate (~1e-7 relative).  ABYSS_PREC_REFINED adds one FP64
+ * Newton step to that estimate (~1e-14).  ABYSS_PREC_FP64 evaluates
+ * everything in FP64 with a correctly rounded rsqrt.
  *
  * Threading: the legacy entry points share one process-wide default context
  * and are not re-entrant (as in the reference, whose state is file-static,
@@ -67,8 +71,10 @@ extern "C" {
 #define ABYSS_ERR_ADMISSION        10  /* served ensemble: HBM budget exhausted */
 #define ABYSS_ERR_BUSY             11  /* served ensemble: no free slot */
 
-#define ABYSS_PREC_SPLIT 0
-#define ABYSS_PREC_FP64  1
+#define ABYSS_PREC_SPLIT   0
+#define ABYSS_PREC_FP64    1
+#define ABYSS_PREC_MIXED   2
+#define ABYSS_PREC_REFINED 3
 
 /* ===================================================================== */
 /* Part 1: legacy entry points (src_mpi/cuda/cuda_functions.h:3-10,      */

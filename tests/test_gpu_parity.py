@@ -50,6 +50,10 @@ def check(js, targets, res, band=1e-6, prec=regf.PREC_SPLIT, label=""):
     if prec == regf.PREC_FP64:
         assert err["a_rel"]["max"] <= 1e-12
         assert err["jdot_rel"]["max"] <= 1e-12
+    elif prec == regf.PREC_REFINED:
+        # FP64 terms with a Newton-refined rsqrt: FP64-level per term
+        assert err["a_rel"]["p99"] <= 1e-12 and err["jdot_rel"]["p99"] <= 1e-12
+        assert err["a_norm"]["max"] <= 1e-12 and err["jdot_norm"]["max"] <= 1e-12
     else:
         assert err["a_rel"]["p99"] <= TOL_REL_P99
         assert err["jdot_rel"]["p99"] <= TOL_REL_P99
@@ -97,6 +101,23 @@ def test_fp64_kernel(ctx):
     ctx.set_precision(regf.PREC_FP64)
     try:
         check(js, t, ctx.query(t), band=1e-12, prec=regf.PREC_FP64, label="fp64")
+    finally:
+        ctx.set_precision(regf.PREC_SPLIT)
+
+
+@pytest.mark.parametrize("prec", [regf.PREC_MIXED, regf.PREC_REFINED])
+@pytest.mark.parametrize("size", [1, 100, 4096, 65536])
+def test_other_precision_modes(ctx, prec, size):
+    name = {regf.PREC_MIXED: "mixed", regf.PREC_REFINED: "refined"}[prec]
+    say(f"case: {name} precision plummer_kroupa N=65536 block={size}")
+    js = workloads.plummer_kroupa(65536)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    t = np.random.default_rng(size + 17).permutation(js.n)[:size].astype(np.int32)
+    ctx.set_precision(prec)
+    try:
+        # these modes test neighbours in FP64: lists exact to 1e-12
+        check(js, t, ctx.query(t), band=1e-12, prec=prec, label=f"{name}/{size}")
+        assert ctx.plan()["lanes_per_target"] > 1
     finally:
         ctx.set_precision(regf.PREC_SPLIT)
 
