@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libabyss_regf.so")
 SOURCES = ["regf_kernels.hip", "regf_runtime.hip", "regf_ensemble.hip", "regf_autotune.hip",
-           "regf_state.hip", "regf_irregular.hip"]
+           "regf_state.hip", "regf_irregular.hip", "regf_derivs.hip"]
 ARCH = os.environ.get("ABYSS_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [

@@ -58,6 +58,12 @@ struct abyss_ctx_s {
     char *d_aos = nullptr;
     size_t aos_bytes = 0;
     long long jset_version = 0, aos_version = -1;
+    // start-up derivatives (regf_derivs.hip): records, split partials,
+    // targets, results; pinned targets + results; radius results
+    char *d_drv = nullptr, *h_drv = nullptr;
+    size_t drv_bytes = 0, h_drv_bytes = 0;
+    char *d_rad = nullptr;
+    size_t rad_bytes = 0;
 
     // split-j scratch
     size_t scratch_bytes = 0;

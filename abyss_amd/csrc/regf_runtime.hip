@@ -207,7 +207,8 @@ void release_all(abyss_ctx_t c)
     dfree(c->d_j); hfree(c->h_j);
     dfree(c->d_tgt); dfree(c->d_acc); dfree(c->d_adot); dfree(c->d_nnb); dfree(c->d_list);
     dfree(c->d_stats); dfree(c->d_scratch); dfree(c->d_state); dfree(c->d_upd);
-    dfree(c->d_rlists); dfree(c->d_irr); dfree(c->d_aos);
+    dfree(c->d_rlists); dfree(c->d_irr); dfree(c->d_aos); dfree(c->d_drv); dfree(c->d_rad);
+    hfree(c->h_drv);
     hfree(c->h_state); hfree(c->h_upd); hfree(c->h_irr);
     hfree(c->h_tgt); hfree(c->h_res); hfree(c->h_nnb); hfree(c->h_list); hfree(c->h_stats);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -297,7 +298,7 @@ unsigned long long ctx_device_bytes(abyss_ctx_t c)
 {
     if (!c) return 0;
     return (unsigned long long)c->j_bytes + c->scratch_bytes + c->state_bytes + c->upd_bytes +
-           c->rlist_bytes + c->irr_bytes + c->aos_bytes +
+           c->rlist_bytes + c->irr_bytes + c->aos_bytes + c->drv_bytes + c->rad_bytes +
            (unsigned long long)c->t_capacity * (sizeof(int) * 2 + 6 * sizeof(double) +
                                                 (size_t)c->cap * sizeof(int));
 }

@@ -56,6 +56,7 @@ EXPORTS = [
     "abyss_ens_set_block_range", "abyss_autotune", "abyss_autotune_load", "abyss_autotune_clear",
     "abyss_ctx_upload_state", "abyss_ctx_update_state", "abyss_ctx_predict",
     "abyss_ctx_keep_lists", "abyss_ctx_irregular", "abyss_ctx_regular_step",
+    "abyss_ctx_derivs23", "abyss_ctx_update_radius",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -152,6 +153,8 @@ def load():
         "abyss_ctx_irregular": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp]),
         "abyss_ctx_regular_step": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _dp, _dp, _dp, _dp,
                                                   _ip, _ip, _dp]),
+        "abyss_ctx_derivs23": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.c_int, _dp]),
+        "abyss_ctx_update_radius": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -377,6 +380,20 @@ class Context:
         self._chk(self._lib.abyss_ctx_irregular(self.h, t.size, _i(t) if t.size else None,
                                                 _d(acc), _d(adot)))
         return acc, adot
+
+    def derivs23(self, targets, reference_signs=True):
+        """Start-up a2/a3 split: [nt, 12] = a_reg2, a_reg3, a_irr2, a_irr3."""
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        out = np.empty((max(t.size, 1), 12))
+        self._chk(self._lib.abyss_ctx_derivs23(self.h, t.size, _i(t) if t.size else None,
+                                               1 if reference_signs else 0, _d(out)))
+        return out[:t.size]
+
+    def update_radius(self, fix=100, nmax=1000):
+        """Radius rule for the last launch's targets; returns their new r2."""
+        out = np.empty(max(self.nt, 1))
+        self._chk(self._lib.abyss_ctx_update_radius(self.h, fix, nmax, _d(out)))
+        return out[:self.nt]
 
     def regular_step(self, targets, dt, a_old):
         """Regular step with neighbour-set correction and regular corrector:

@@ -236,6 +236,34 @@ int abyss_ctx_regular_step(abyss_ctx_t ctx, int nt, const int *targets, const do
                            const double *a_old, double *acc, double *adot, int *nnb,
                            int *list, double *out);
 
+/* Start-up second and third time derivatives of the acceleration, split
+ * into regular (r^2 > r_nb,i^2) and irregular parts, summed over all j != i
+ * in FP64 on the resident state of abyss_ctx_upload_state (x, v, a = total
+ * acceleration, adot = total jerk, m, r2), in place of the caller's O(N^2)
+ * start-up loop (CalculateAcceleration23,
+ * src_mpi/Particle/Initialize.cpp:102-185).
+ *   out[k][12] = a_reg2[3], a_reg3[3], a_irr2[3], a_irr3[3] of targets[k]
+ *   (the reference adds them to zero-initialised a_reg/a_irr[.][2..3]).
+ * reference_signs = 1 reproduces the reference's expressions, whose
+ * projections x.(a_i - a_j)/r^2, v.(a_i - a_j)/r^2 and x.(adot_i - adot_j)/r^2
+ * enter B and C with the opposite sign to the relative acceleration
+ * a_j - a_i used by its leading terms; reference_signs = 0 uses
+ * x.(a_j - a_i) etc., which gives the exact derivatives of the pair forces.
+ * A j at r = 0 adds nothing. */
+int abyss_ctx_derivs23(abyss_ctx_t ctx, int nt, const int *targets, int reference_signs,
+                       double *out);
+
+/* Neighbour-radius adaptation after a regular step (Particle::updateRadius,
+ * src_mpi/Particle/UpdateParticle.cpp:108-139) for the targets of the last
+ * launch, from their neighbour counts (|count| when overflowed): r2 (the
+ * squared radius, the upload's r2) is scaled by (1 - a), x = count - fix,
+ * c = nmax - fix, a = 0.9 |x| x / c^2 above fix, a = 0.5 x^3 / c^3 below
+ * (the radius grows), unchanged at fix.  The new r2 replaces the j-set's
+ * (the next launch uses it) and the resident state's, and is written to
+ * r2_out[nt].  The reference uses fix = FixNumNeighbor (100) and
+ * nmax = NumNeighborMax (1000), src_mpi/def.h:18,20. */
+int abyss_ctx_update_radius(abyss_ctx_t ctx, int fix, int nmax, double *r2_out);
+
 /* Plan autotuner.  Times candidate plans (lanes per target, j-unroll,
  * workgroups per CU -> j-split count) for target counts 1, 4, 16, ... n on
  * a synthetic n-particle cluster (n >= 256) and installs the fastest per

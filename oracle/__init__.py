@@ -38,6 +38,11 @@ def _load():
     lib.oracle_regular_correct.restype = None
     lib.oracle_regular_correct.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, _ip, _dp,
                                            _dp, _dp, _dp, _ip, _ip, _ip, _ip, ctypes.c_int, _dp]
+    lib.oracle_derivs23.restype = None
+    lib.oracle_derivs23.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, _ip,
+                                    ctypes.c_int, ctypes.c_int, _dp]
+    lib.oracle_update_radius.restype = None
+    lib.oracle_update_radius.argtypes = [ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, _dp]
     lib.oracle_max_threads.restype = ctypes.c_int
     _lib = lib
     return lib
@@ -153,6 +158,31 @@ def regular_correct(x, v, m, targets, dt, a_old, new_acc, new_adot, old_nnb, old
                                _d(new_acc), _d(new_adot), _i(old_nnb), _i(old_lists),
                                _i(new_nnb), _i(new_lists), cap, _d(out))
     return out
+
+
+def derivs23(x, v, a, adot, m, rnb2, targets, ref_signs=True, nthreads=0):
+    """Start-up 2nd/3rd acceleration derivatives: out [nt, 12] =
+    a_reg2, a_reg3, a_irr2, a_irr3 (ref_signs: the reference's projection
+    signs; False: the exact derivatives)."""
+    lib = _load()
+    f = lambda z: np.ascontiguousarray(z, dtype=np.float64).reshape(-1, 3)
+    x, v, a, adot = f(x), f(v), f(a), f(adot)
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    rnb2 = np.ascontiguousarray(rnb2, dtype=np.float64)
+    t = np.ascontiguousarray(targets, dtype=np.int32)
+    out = np.empty((t.size, 12))
+    lib.oracle_derivs23(x.shape[0], _d(x), _d(v), _d(a), _d(adot), _d(m), _d(rnb2), t.size,
+                        _i(t), 1 if ref_signs else 0, nthreads, _d(out))
+    return out
+
+
+def update_radius(nnb, r2, fix=100, nmax=1000):
+    """Neighbour-radius adaptation of r2 (squared radius) from the counts."""
+    lib = _load()
+    nnb = np.ascontiguousarray(nnb, dtype=np.int32)
+    r2 = np.array(r2, dtype=np.float64)
+    lib.oracle_update_radius(nnb.size, _i(nnb), fix, nmax, _d(r2))
+    return r2
 
 
 def max_threads():

@@ -29,6 +29,8 @@
  * Parity pinning (tests/test_oracle.py): two-body closed forms,
  * regular + irregular = the unsplit direct sum (oracle_irregular), the
  * corrector's cubic closed form and split invariance (oracle_regular_correct),
+ * finite-difference derivatives of the pair force (oracle_derivs23), the
+ * radius rule at its end points (oracle_update_radius),
  * a = -grad(phi)
  * and jdot = d a/dt by finite differences, momentum conservation, homogeneity
  * scalings, r_nb -> 0 / inf limits, the exact "<=" boundary case, coincident
@@ -305,6 +307,108 @@ void oracle_regular_correct(int n, const double *x, const double *v, const doubl
             o[9 + d] = a3;
             o[12 + d] = a_irr[d];
             o[15 + d] = adot_irr[d];
+        }
+    }
+}
+
+/*
+ * Start-up second and third derivatives of the acceleration, split into
+ * regular (r2 > r_nb,i^2) and irregular parts, FP64, in the reference's
+ * expression order (CalculateAcceleration23,
+ * src_mpi/Particle/Initialize.cpp:102-185).  x, v, a, adot: [n][3] at the
+ * common start time (a, adot: total acceleration and jerk); every j != i is
+ * summed; r2 == 0 adds nothing (the reference would divide by zero).
+ * out [nt][12] = a_reg2[3], a_reg3[3], a_irr2[3], a_irr3[3].
+ *
+ * ref_signs = 1 keeps the reference's sign in the three projections
+ * rdf_r2 = x.(a1 - a2)/r2, vdf_r2 = v.(a1 - a2)/r2, rdfdot_r2 =
+ * x.(a1dot - a2dot)/r2 (a1: the target's, a2: j's, x = x_j - x_i), which
+ * enter B and C with the opposite sign to the relative acceleration
+ * x'' = a2 - a1 that the leading terms -m (a1 - a2)/r^3 use; ref_signs = 0
+ * uses x.(a2 - a1) etc., which makes adot2 and adot3 the exact second and
+ * third time derivatives of m x/|x|^3 along the relative orbit.
+ */
+void oracle_derivs23(int n, const double *X, const double *V, const double *Acc,
+                     const double *Adot, const double *m, const double *rnb2, int nt,
+                     const int *targets, int ref_signs, int nthreads, double *out)
+{
+    const double ps = ref_signs ? 1.0 : -1.0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+    for (int k = 0; k < nt; ++k) {
+        const int i = targets[k];
+        double reg2[3] = {0, 0, 0}, reg3[3] = {0, 0, 0}, irr2[3] = {0, 0, 0}, irr3[3] = {0, 0, 0};
+        const double *a1 = Acc + 3 * (size_t)i, *a1dot = Adot + 3 * (size_t)i;
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            const double *a2 = Acc + 3 * (size_t)j, *a2dot = Adot + 3 * (size_t)j;
+            double x[3], v[3], r2 = 0, vr = 0, v2 = 0;
+            for (int d = 0; d < 3; ++d) {
+                x[d] = X[3 * (size_t)j + d] - X[3 * (size_t)i + d];
+                v[d] = V[3 * (size_t)j + d] - V[3 * (size_t)i + d];
+                r2 += x[d] * x[d];
+                vr += v[d] * x[d];
+                v2 += v[d] * v[d];
+            }
+            if (r2 == 0.0) continue;
+            const double r3 = r2 * sqrt(r2);
+            const double m_r3 = m[j] / r3;
+            double a21[3], a21dot[3], rdf_r2 = 0, vdf_r2 = 0, rdfdot_r2 = 0;
+            for (int d = 0; d < 3; ++d) {
+                a21[d] = m_r3 * x[d];
+                a21dot[d] = m_r3 * (v[d] - 3 * x[d] * vr / r2);
+                rdf_r2 += ps * (x[d] * (a1[d] - a2[d]) / r2);
+                vdf_r2 += ps * (v[d] * (a1[d] - a2[d]) / r2);
+                rdfdot_r2 += ps * (x[d] * (a1dot[d] - a2dot[d]) / r2);
+            }
+            const double A = vr / r2;
+            const double B = v2 / r2 + rdf_r2 + A * A;
+            const double C = 3 * vdf_r2 + rdfdot_r2 + A * (3 * B - 4 * A * A);
+            const int reg = r2 > rnb2[i];
+            for (int d = 0; d < 3; ++d) {
+                const double adot2 = -m[j] * (a1[d] - a2[d]) / r3 - 6 * A * a21dot[d] - 3 * B * a21[d];
+                const double adot3 = -m[j] * (a1dot[d] - a2dot[d]) / r3 - 9 * A * adot2 -
+                                     9 * B * a21dot[d] - 3 * C * a21[d];
+                if (reg) {
+                    reg2[d] += adot2;
+                    reg3[d] += adot3;
+                } else {
+                    irr2[d] += adot2;
+                    irr3[d] += adot3;
+                }
+            }
+        }
+        double *o = out + (size_t)k * 12;
+        for (int d = 0; d < 3; ++d) {
+            o[d] = reg2[d];
+            o[3 + d] = reg3[d];
+            o[6 + d] = irr2[d];
+            o[9 + d] = irr3[d];
+        }
+    }
+}
+
+/*
+ * Neighbour-radius adaptation (Particle::updateRadius,
+ * src_mpi/Particle/UpdateParticle.cpp:108-139).  r2 (the squared radius,
+ * RadiusOfNeighbor) is scaled by (1 - a) with x = nnb - fix, c = nmax - fix:
+ *   nnb > fix: a = 0.9/c^2 * |x| * x      (n = 2)
+ *   nnb < fix: a = 0.5/c^3 * x^3          (n = 3; a < 0, the radius grows)
+ *   nnb = fix: unchanged.
+ */
+void oracle_update_radius(int count, const int *nnb, int fix, int nmax, double *r2)
+{
+    for (int k = 0; k < count; ++k) {
+        const double c = (double)(nmax - fix);
+        const double x = (double)(nnb[k] - fix);
+        if (nnb[k] > fix) {
+            const double b = 0.9 / pow(c, 2);
+            r2[k] *= (1. - b * fabs(x) * pow(x, 1));
+        } else if (nnb[k] < fix) {
+            const double b = 0.5 / pow(c, 3);
+            r2[k] *= (1. - b * pow(x, 3));
         }
     }
 }

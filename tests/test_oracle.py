@@ -333,3 +333,89 @@ def test_regular_corrector_is_invariant_under_the_split():
     ad = (a_old[:, 3:] + jo) / dt[:, None]
     np.testing.assert_allclose(o2[:, 6:9], -6 * da - 2 * ad - 2 * a_old[:, 3:] / dt[:, None],
                                rtol=1e-12, atol=1e-12 * np.abs(da).max())
+
+
+def _pair_force(m, x):
+    r2 = np.sum(x * x, axis=-1, keepdims=True)
+    return m * x / r2 ** 1.5
+
+
+def test_derivs23_are_time_derivatives_of_the_pair_forces():
+    # For every pair, the relative orbit x(t) = x0 + v t + A t^2/2 + J t^3/6
+    # with A = a_j - a_i and J = adot_j - adot_i has pair force
+    # F(t) = m_j x / |x|^3; the start-up terms must be F''(0) and F'''(0).
+    # Checked by 4th-order central differences of F, summed over j, with the
+    # regular / irregular split by r_nb.
+    rng = np.random.default_rng(71)
+    n = 7
+    x = rng.normal(size=(n, 3))
+    v = rng.normal(size=(n, 3)) * 0.5
+    a = rng.normal(size=(n, 3)) * 0.3
+    ad = rng.normal(size=(n, 3)) * 0.3
+    m = rng.uniform(0.5, 2.0, n)
+    rnb2 = np.full(n, 1.0)
+    t = np.arange(n, dtype=np.int32)
+    out = oracle.derivs23(x, v, a, ad, m, rnb2, t, ref_signs=False)
+    h = 2e-3
+
+    def F(i, j, s):
+        d0 = x[j] - x[i]
+        dv = v[j] - v[i]
+        A = a[j] - a[i]
+        J = ad[j] - ad[i]
+        return _pair_force(m[j], d0 + dv * s + A * s * s / 2 + J * s ** 3 / 6)
+
+    for i in range(n):
+        reg2 = np.zeros(3); reg3 = np.zeros(3); irr2 = np.zeros(3); irr3 = np.zeros(3)
+        for j in range(n):
+            if j == i:
+                continue
+            f = {k: F(i, j, k * h) for k in (-3, -2, -1, 0, 1, 2, 3)}
+            d2 = (-f[2] + 16 * f[1] - 30 * f[0] + 16 * f[-1] - f[-2]) / (12 * h * h)
+            d3 = (-f[3] + 8 * f[2] - 13 * f[1] + 13 * f[-1] - 8 * f[-2] + f[-3]) / (8 * h ** 3)
+            if np.sum((x[j] - x[i]) ** 2) > rnb2[i]:
+                reg2 += d2; reg3 += d3
+            else:
+                irr2 += d2; irr3 += d3
+        scale2 = np.abs(out[i, [0, 1, 2, 6, 7, 8]]).max() + 1
+        scale3 = np.abs(out[i, [3, 4, 5, 9, 10, 11]]).max() + 1
+        np.testing.assert_allclose(out[i, 0:3], reg2, rtol=0, atol=1e-6 * scale2)
+        np.testing.assert_allclose(out[i, 6:9], irr2, rtol=0, atol=1e-6 * scale2)
+        np.testing.assert_allclose(out[i, 3:6], reg3, rtol=0, atol=1e-4 * scale3)
+        np.testing.assert_allclose(out[i, 9:12], irr3, rtol=0, atol=1e-4 * scale3)
+    # both parts occur
+    assert np.abs(out[:, 0:6]).max() > 0 and np.abs(out[:, 6:12]).max() > 0
+
+
+def test_update_radius_end_points():
+    fix, nmax = 100, 1000
+    r2 = np.full(5, 2.0)
+    got = oracle.update_radius([fix, nmax, fix + 1, 0, fix - 1], r2, fix, nmax)
+    # nnb = fix: unchanged; nnb = nmax: a = 0.9 -> x0.1; fix+1: a = 0.9/900^2
+    # nnb = 0: a = 0.5 (-100)^3 / 900^3 (< 0: grows); fix-1: a = -0.5/900^3
+    np.testing.assert_allclose(got, [2.0, 0.2, 2.0 * (1 - 0.9 / 810000.0),
+                                     2.0 * (1 + 0.5e6 / 729e6), 2.0 * (1 + 0.5 / 729e6)],
+                               rtol=1e-15)
+
+
+def test_derivs23_reference_signs():
+    # The reference's projections x.(a1 - a2) etc. carry the opposite sign to
+    # the relative acceleration its leading terms use.  With equal
+    # accelerations and jerks everywhere the projections vanish and both
+    # variants equal the exact derivatives; with only the accelerations
+    # differing, adot2 differs by exactly -3 (B_ref - B) a21 = 6 (x.A/r2) a21
+    # per pair (A = a_j - a_i), checked here on a single pair.
+    rng = np.random.default_rng(73)
+    x = rng.normal(size=(5, 3)); v = rng.normal(size=(5, 3)); m = rng.uniform(0.5, 2, 5)
+    a_same = np.tile(rng.normal(size=3), (5, 1)); ad_same = np.tile(rng.normal(size=3), (5, 1))
+    t = np.arange(5, dtype=np.int32)
+    r = oracle.derivs23(x, v, a_same, ad_same, m, np.zeros(5), t, ref_signs=True)
+    e = oracle.derivs23(x, v, a_same, ad_same, m, np.zeros(5), t, ref_signs=False)
+    np.testing.assert_allclose(r, e, rtol=1e-13, atol=1e-13)
+    xp = np.array([[0.0, 0, 0], [1.0, 2.0, -0.5]]); vp = np.zeros((2, 3))
+    ap = np.array([[0.0, 0, 0], [0.3, -0.2, 0.7]]); adp = np.zeros((2, 3)); mp = np.array([1.0, 2.0])
+    r = oracle.derivs23(xp, vp, ap, adp, mp, np.zeros(2), [0], ref_signs=True)[0]
+    e = oracle.derivs23(xp, vp, ap, adp, mp, np.zeros(2), [0], ref_signs=False)[0]
+    d = xp[1]; r2 = d @ d; A = ap[1] - ap[0]
+    a21 = mp[1] * d / r2 ** 1.5
+    np.testing.assert_allclose(r[0:3] - e[0:3], 6 * (d @ A) / r2 * a21, rtol=1e-12)
