@@ -343,11 +343,42 @@ def bench_neighbour(args, reps=5):
         ms_irr = wall(lambda: ctx.irregular(t))
         ms_query = wall(lambda: ctx.query(t))
         ms_step = wall(lambda: ctx.regular_step(t, dt, a_old))
+        ms_radius = wall(lambda: ctx.update_radius(100, 1000))
+        # j prediction: host predictor + pack + H2D (upload) vs the device
+        # predictor over the resident state
+        a = rng.normal(size=(js.n, 3)) * 0.1
+        ad = rng.normal(size=(js.n, 3)) * 0.1
+        t0 = np.zeros(js.n)
+        ctx.upload_state(js.m, js.x, js.v, a, ad, t0, js.r2)
+
+        def host_predict_upload():
+            h = 1.0 / 1024
+            xp = ((ad * h / 3 + a) * h / 2 + js.v) * h + js.x
+            vp = (ad * h / 2 + a) * h + js.v
+            ctx.upload(js.m, xp, vp, js.r2)
+            ctx.sync()
+
+        def device_predict():
+            ctx.predict(1.0 / 1024)
+            ctx.sync()
+
+        ms_host_pred = wall(host_predict_upload)
+        ctx.upload_state(js.m, js.x, js.v, a, ad, t0, js.r2)
+        ms_dev_pred = wall(device_predict)
+        # start-up derivatives over all N (one call)
+        t1 = time.perf_counter()
+        ctx.derivs23(t)
+        ms_d23 = (time.perf_counter() - t1) * 1e3
     out["irregular"] = {"targets": js.n, "list_entries": entries, "ms_per_call": ms_irr,
                         "entries_per_s": entries / (ms_irr * 1e-3),
                         "includes": "target H2D, kernel, result D2H"}
     out["regular_step"] = {"targets": js.n, "ms_per_call": ms_step, "query_ms_per_call": ms_query,
                            "correction_overhead_ms": ms_step - ms_query}
+    out["update_radius"] = {"targets": js.n, "ms_per_call": ms_radius}
+    out["prediction"] = {"host_predict_pack_upload_ms": ms_host_pred,
+                         "device_predict_ms": ms_dev_pred}
+    out["derivs23"] = {"targets": js.n, "pairs": float(js.n) * (js.n - 1), "ms": ms_d23,
+                       "pairs_per_s": float(js.n) * (js.n - 1) / (ms_d23 * 1e-3)}
     log(f"neighbour legs {out}")
     return out
 

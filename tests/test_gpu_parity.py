@@ -372,3 +372,22 @@ def test_hermite_resident_state_matches_host_prediction(tmp_path):
     say(f"  resident: {resident}")
     for key in ("E1", "dE_over_E", "blocks", "particle_steps", "com", "x0"):
         assert legacy[key] == resident[key], key
+
+
+def test_ahmad_cohen_two_time_scales_gpu_vs_oracle(tmp_path):
+    # the device-resident path (predict, irregular, regular_step with the
+    # neighbour-set correction, derivs23, update_radius, update_state) runs
+    # the same Ahmad-Cohen integration as the oracle
+    from tests import hermite_util
+    js = workloads.plummer_equal(1024)
+    ic = str(tmp_path / "ic.bin")
+    hermite_util.write_ic(ic, js)
+    say("case: Ahmad-Cohen N=1024 t=1 eta 0.02/0.001 gpu")
+    g = hermite_util.run_ac("gpu", ic, 1.0, 0.02, 0.001, 48)
+    o = hermite_util.run_ac("oracle", ic, 1.0, 0.02, 0.001, 48)
+    say(f"  gpu:    {g}")
+    say(f"  oracle: {o}")
+    assert abs(g["dE_over_E"]) < 1e-6 and g["overflows"] == 0
+    assert abs(g["dE_over_E"] - o["dE_over_E"]) < 5e-7
+    for k in ("irregular_steps", "regular_steps"):
+        assert abs(g[k] - o[k]) <= 0.05 * o[k], (k, g[k], o[k])
