@@ -24,6 +24,7 @@ struct abyss_ctx_s {
 
     // targets and outputs
     int nt = 0;
+    bool counts_valid = false;   // d_nnb holds the counts of the current targets
     size_t t_capacity = 0;  // targets the output buffers can hold
     int *d_tgt = nullptr, *h_tgt = nullptr;
     double *d_acc = nullptr, *d_adot = nullptr;

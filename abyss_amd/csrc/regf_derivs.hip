@@ -246,7 +246,7 @@ int abyss_ctx_derivs23(abyss_ctx_t c, int nt, const int *targets, int reference_
 int abyss_ctx_update_radius(abyss_ctx_t c, int fix, int nmax, double *r2_out)
 {
     if (!c) return ABYSS_ERR_INVALID_ARG;
-    if (c->n <= 0 || c->nt <= 0) {
+    if (c->n <= 0 || c->nt <= 0 || !c->counts_valid) {
         snprintf(c->detail, sizeof(c->detail), "update_radius: no launch to take counts from");
         return ABYSS_ERR_NO_JSET;
     }

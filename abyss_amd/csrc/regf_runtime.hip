@@ -441,6 +441,7 @@ int abyss_ctx_set_targets(abyss_ctx_t c, int nt, const int *targets)
     HIPCHK(c, hipMemcpyAsync(c->d_tgt, c->h_tgt, (size_t)nt * sizeof(int), hipMemcpyHostToDevice,
                              c->stream));
     c->nt = nt;
+    c->counts_valid = false;   // counts belong to the next launch
     c->plan = make_plan(c, nt, c->n);
     rc = ensure_scratch(c, c->plan, nt);
     if (rc != ABYSS_OK) return rc;
@@ -493,6 +494,7 @@ int ctx_launch_force(abyss_ctx_t c)
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->timing_pending = true;
     c->n_grav += 1;
+    c->counts_valid = true;
     c->sum_ni += c->nt;
     c->interactions += (double)c->nt * (double)(c->n - 1);
     return ABYSS_OK;

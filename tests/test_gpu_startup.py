@@ -87,3 +87,19 @@ def test_update_radius_matches_host_rule_and_feeds_next_launch():
         res = compare_lists(nnb2, lst2, rn, rl, js.x, r2_new, t, band=1e-6)
         say(f"  lists after update: {res}")
         assert res["hard_mismatches"] == 0
+
+
+def test_update_radius_needs_a_launch_of_the_current_targets():
+    js = workloads.plummer_equal(1024)
+    with regf.Context(device=0, cap=256) as c:
+        c.upload(js.m, js.x, js.v, js.r2)
+        c.set_targets(np.arange(10, dtype=np.int32))
+        with pytest.raises(regf.AbyssError) as e:
+            c.update_radius(48, 256)
+        assert e.value.code == regf.ERR_NO_JSET
+        c.launch()
+        c.sync()
+        assert c.update_radius(48, 256).shape == (10,)
+        with pytest.raises(regf.AbyssError) as e:
+            c.update_radius(48, 48)          # nmax must exceed fix
+        assert e.value.code == regf.ERR_INVALID_ARG
