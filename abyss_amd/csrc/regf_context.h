@@ -27,14 +27,17 @@ struct abyss_ctx_s {
     bool counts_valid = false;   // d_nnb holds the counts of the current targets
     size_t t_capacity = 0;  // targets the output buffers can hold
     int *d_tgt = nullptr, *h_tgt = nullptr;
+    // packed per-target results (device and pinned host, same layout):
+    // acc [nt][3] | adot [nt][3] | nnb [nt] | stats [2]; the pointers below
+    // are views of d_out / h_out for the current nt
+    char *d_out = nullptr, *h_out = nullptr;
     double *d_acc = nullptr, *d_adot = nullptr;
-    int *d_nnb = nullptr, *d_list = nullptr;
-    int *d_stats = nullptr;
-    double *h_res = nullptr;  // pinned [nt][6] (acc then adot)
-    int *h_nnb = nullptr;
+    int *d_nnb = nullptr, *d_stats = nullptr;
+    double *h_res = nullptr;  // acc then adot
+    int *h_nnb = nullptr, *h_stats = nullptr;
+    int *d_list = nullptr;
     int *h_list = nullptr;    // pinned, rows of width <= cap
     size_t h_list_bytes = 0;
-    int *h_stats = nullptr;
 
     // device-resident particle state (regf_state.hip): SoA x,v,a,adot (3n
     // each), t, m, r2 (n each); staging for incremental updates

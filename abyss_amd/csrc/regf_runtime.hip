@@ -24,6 +24,10 @@ using abyss::QueryOut;
 
 namespace {
 
+// Below this many bytes of full-width list rows, fetch copies them with the
+// other results in one round trip instead of first reading the widest row.
+constexpr size_t kOneTripListBytes = 1u << 20;
+
 int set_detail(abyss_ctx_t c, int code, const char *fmt, ...)
 {
     if (c) {
@@ -80,22 +84,44 @@ void bind_jstate(abyss_ctx_t c, char *base, int n)
     c->js.n = n;
 }
 
+// Bytes of the packed per-target results of nt targets:
+// acc [nt][3] | adot [nt][3] (FP64) | nnb [nt] | stats [2] (int32).
+size_t packed_bytes(size_t nt)
+{
+    return nt * (6 * sizeof(double) + sizeof(int)) + 2 * sizeof(int);
+}
+
+// Views of the packed result blocks for the current target count, so that
+// one copy moves every per-target result (and the list statistics).
+void set_output_views(abyss_ctx_t c, int nt)
+{
+    const size_t n = (size_t)nt;
+    c->d_acc = reinterpret_cast<double *>(c->d_out);
+    c->d_adot = c->d_acc + 3 * n;
+    c->d_nnb = reinterpret_cast<int *>(c->d_adot + 3 * n);
+    c->d_stats = c->d_nnb + n;
+    c->h_res = reinterpret_cast<double *>(c->h_out);
+    c->h_nnb = reinterpret_cast<int *>(c->h_res + 6 * n);
+    c->h_stats = c->h_nnb + n;
+}
+
 int ensure_outputs(abyss_ctx_t c, int nt)
 {
-    if ((size_t)nt <= c->t_capacity) return ABYSS_OK;
+    if ((size_t)nt <= c->t_capacity) {
+        set_output_views(c, nt);
+        return ABYSS_OK;
+    }
     size_t cap_t = grow((size_t)nt);
-    dfree(c->d_tgt); dfree(c->d_acc); dfree(c->d_adot); dfree(c->d_nnb); dfree(c->d_list);
-    hfree(c->h_tgt); hfree(c->h_res); hfree(c->h_nnb);
+    dfree(c->d_tgt); dfree(c->d_out); dfree(c->d_list);
+    hfree(c->h_tgt); hfree(c->h_out);
     c->t_capacity = 0;
     HIPCHK(c, hipMalloc(&c->d_tgt, cap_t * sizeof(int)));
-    HIPCHK(c, hipMalloc(&c->d_acc, cap_t * 3 * sizeof(double)));
-    HIPCHK(c, hipMalloc(&c->d_adot, cap_t * 3 * sizeof(double)));
-    HIPCHK(c, hipMalloc(&c->d_nnb, cap_t * sizeof(int)));
+    HIPCHK(c, hipMalloc(&c->d_out, packed_bytes(cap_t)));
     HIPCHK(c, hipMalloc(&c->d_list, cap_t * (size_t)c->cap * sizeof(int)));
     HIPCHK(c, hipHostMalloc(&c->h_tgt, cap_t * sizeof(int), hipHostMallocDefault));
-    HIPCHK(c, hipHostMalloc(&c->h_res, cap_t * 6 * sizeof(double), hipHostMallocDefault));
-    HIPCHK(c, hipHostMalloc(&c->h_nnb, cap_t * sizeof(int), hipHostMallocDefault));
+    HIPCHK(c, hipHostMalloc(&c->h_out, packed_bytes(cap_t), hipHostMallocDefault));
     c->t_capacity = cap_t;
+    set_output_views(c, nt);
     return ABYSS_OK;
 }
 
@@ -104,18 +130,19 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
     Plan p;
     p.tile = abyss::kTile;
     const int cu = c->cu_count > 0 ? c->cu_count : 256;
-    // Lanes per target: 4 lanes share a target from 1024 targets up (its
+    // Lanes per target: 4 lanes share a target from 256 targets up (its
     // inner loop is one VALU op per pair shorter than one target per lane
     // and it holds 5 waves per SIMD instead of 4: 47.8% vs 45.4% of FP32
-    // peak at N = m = 262144); smaller blocks spread a target over 16/64.
+    // peak at N = m = 262144); smaller blocks spread a target over 16 lanes
+    // (16..255 targets) or 64 (below 16), measured best at N = 262144.
     abyss::TuneEntry te{0, 0, 16};
     const bool tuned = c->precision == ABYSS_PREC_SPLIT && abyss::tuned_plan(c->device, nt, &te);
     int jl = c->force_jl;
     if (jl == 0) {
         if (tuned) jl = te.jl;
         else if (c->precision == ABYSS_PREC_FP64) jl = 1;
-        else if (nt >= 1024) jl = 4;
-        else if (nt >= 64) jl = 16;
+        else if (nt >= 256) jl = 4;
+        else if (nt >= 16) jl = 16;
         else jl = 64;
     }
     if (c->precision == ABYSS_PREC_FP64) jl = 1;
@@ -142,10 +169,13 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
         // Workgroups per CU over the whole launch (or the tuned count):
         // ~48 for the 1- and 4-lane kernels (tail of the last wave of
         // workgroups; measured best at m = 16K..256K, N = 256K), 16 for the
-        // 16/64-lane ones; every split keeps at least one tile.
+        // 16/64-lane ones.
         const int want_wg = cu * (tuned ? te.wg_per_cu : (jl <= 4 ? 48 : 16));
         S = (want_wg + p.grid_x - 1) / p.grid_x;
-        const int max_by_n = std::max(1, n / abyss::kTile);
+        // every split keeps at least 4 tiles (1024 j): finer splits cost
+        // more in finalize than they gain (m = 1 at N = 262144: S = 256
+        // takes 21.7 us, S = 1024 31.1 us)
+        const int max_by_n = std::max(1, n / (4 * abyss::kTile));
         S = std::min(S, std::min(max_by_n, abyss::kMaxSplits));
     }
     if (S > 1) {
@@ -205,12 +235,12 @@ void release_all(abyss_ctx_t c)
     if (c->device >= 0) (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->d_j); hfree(c->h_j);
-    dfree(c->d_tgt); dfree(c->d_acc); dfree(c->d_adot); dfree(c->d_nnb); dfree(c->d_list);
-    dfree(c->d_stats); dfree(c->d_scratch); dfree(c->d_state); dfree(c->d_upd);
+    dfree(c->d_tgt); dfree(c->d_out); dfree(c->d_list);
+    dfree(c->d_scratch); dfree(c->d_state); dfree(c->d_upd);
     dfree(c->d_rlists); dfree(c->d_irr); dfree(c->d_aos); dfree(c->d_drv); dfree(c->d_rad);
     hfree(c->h_drv);
     hfree(c->h_state); hfree(c->h_upd); hfree(c->h_irr);
-    hfree(c->h_tgt); hfree(c->h_res); hfree(c->h_nnb); hfree(c->h_list); hfree(c->h_stats);
+    hfree(c->h_tgt); hfree(c->h_out); hfree(c->h_list);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -279,10 +309,6 @@ int ctx_create_on_stream(int device, int cap, hipStream_t stream, abyss_ctx_t *o
             hipSuccess) { rc = ABYSS_ERR_HIP; break; }
         if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
             rc = ABYSS_ERR_HIP; break;
-        }
-        if (hipMalloc(&c->d_stats, 4 * sizeof(int)) != hipSuccess ||
-            hipHostMalloc(&c->h_stats, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
-            rc = ABYSS_ERR_OUT_OF_MEMORY; break;
         }
     } while (0);
     if (rc != ABYSS_OK) {
@@ -536,22 +562,30 @@ int ctx_fetch_on_stream(abyss_ctx_t c, hipStream_t st, double *acc, double *adot
     const int nt = c->nt;
     const int cap = c->cap;
     HIPCHK(c, abyss::launch_count_stats(c->d_nnb, nt, cap, c->d_stats, st));
-    HIPCHK(c, hipMemcpyAsync(c->h_stats, c->d_stats, 2 * sizeof(int), hipMemcpyDeviceToHost,
+    // acc, adot, counts and list statistics are one packed block: one copy
+    HIPCHK(c, hipMemcpyAsync(c->h_out, c->d_out, packed_bytes((size_t)nt), hipMemcpyDeviceToHost,
                              st));
-    HIPCHK(c, hipMemcpyAsync(c->h_res, c->d_acc, (size_t)nt * 3 * sizeof(double),
-                             hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipMemcpyAsync(c->h_res + (size_t)nt * 3, c->d_adot, (size_t)nt * 3 * sizeof(double),
-                             hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipMemcpyAsync(c->h_nnb, c->d_nnb, (size_t)nt * sizeof(int), hipMemcpyDeviceToHost,
-                             st));
+    // small results: copy the full list rows in the same round trip rather
+    // than waiting for the populated width
+    const size_t full_bytes = (size_t)nt * cap * sizeof(int);
+    const bool one_trip = full_bytes <= kOneTripListBytes;
+    if (one_trip) {
+        if (full_bytes > c->h_list_bytes) {
+            hfree(c->h_list);
+            c->h_list_bytes = 0;
+            HIPCHK(c, hipHostMalloc(&c->h_list, grow(full_bytes), hipHostMallocDefault));
+            c->h_list_bytes = grow(full_bytes);
+        }
+        HIPCHK(c, hipMemcpyAsync(c->h_list, c->d_list, full_bytes, hipMemcpyDeviceToHost, st));
+    }
     HIPCHK(c, hipStreamSynchronize(st));
     if (st == c->stream) {
         int rc = collect_timing(c);
         if (rc != ABYSS_OK) return rc;
     }
-    const int width = c->h_stats[0];  // widest populated row, <= cap
     const int overflowed = c->h_stats[1];
-    if (width > 0) {
+    const int width = one_trip ? cap : c->h_stats[0];  // row stride of h_list
+    if (!one_trip && width > 0) {
         const size_t need = (size_t)nt * width * sizeof(int);
         if (need > c->h_list_bytes) {
             hfree(c->h_list);

@@ -253,10 +253,14 @@ def test_legacy_entry_points():
         regf.SendToDevice(js.m, js.x, js.v, js.r2)
         t = np.arange(0, js.n, 3, dtype=np.int32)
         acc, adot, nnb, lst = regf.CalculateAccelerationOnDevice(t)
-        ra, rj, rn, rl = oracle.regf(js.x, js.v, js.m, js.r2, t, 1000)
+        ra, rj, rn, rl, sa, sj = oracle.regf(js.x, js.v, js.m, js.r2, t, 1000, scales=True)
         res = compare_lists(nnb, lst, rn, rl, js.x, js.r2, t)
         assert res["hard_mismatches"] == 0
-        np.testing.assert_allclose(acc, ra, rtol=1e-5, atol=1e-9)
+        # the bars of check(): component-wise tolerances fail on components
+        # that cancel, so compare per particle
+        err = force_errors(acc, adot, ra, rj, sa, sj)
+        assert err["a_rel"]["p99"] <= TOL_REL_P99 and err["a_norm"]["max"] <= TOL_NORM_MAX
+        assert err["jdot_rel"]["p99"] <= TOL_REL_P99 and err["jdot_norm"]["max"] <= TOL_NORM_MAX
         regf.ProfileDevice(0)
         # a second open only reports it
         regf.OpenDevice(0)
