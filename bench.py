@@ -107,13 +107,17 @@ def cpu_baseline(js, target_seconds=10.0):
     threads = oracle.max_threads()
     rng = np.random.default_rng(7)
     order = rng.permutation(js.n).astype(np.int32)
-    # calibrate on a small sample, then size the timed sample to ~target_seconds
-    probe = order[:max(threads * 4, 64)]
-    t0 = time.perf_counter()
-    oracle.regf(js.x, js.v, js.m, js.r2, probe, js.cap, nthreads=threads)
-    dt = time.perf_counter() - t0
-    rate = len(probe) / max(dt, 1e-6)
-    count = int(min(js.n, max(len(probe), rate * target_seconds)))
+    # calibrate twice (a tiny probe underestimates the rate: thread start-up),
+    # then size the timed sample to ~target_seconds
+    size = max(threads * 4, 64)
+    rate = 0.0
+    for probe_seconds in (None, 1.0):
+        if probe_seconds is not None:
+            size = int(min(js.n, max(size, rate * probe_seconds)))
+        t0 = time.perf_counter()
+        oracle.regf(js.x, js.v, js.m, js.r2, order[:size], js.cap, nthreads=threads)
+        rate = size / max(time.perf_counter() - t0, 1e-6)
+    count = int(min(js.n, max(size, rate * target_seconds)))
     sample = order[:count]
     t0 = time.perf_counter()
     oracle.regf(js.x, js.v, js.m, js.r2, sample, js.cap, nthreads=threads)
