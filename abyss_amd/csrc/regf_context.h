@@ -13,6 +13,7 @@ struct abyss_ctx_s {
     int cap = ABYSS_NUM_NEIGHBOR_MAX;
     int precision = ABYSS_PREC_SPLIT;
     int force_ipl = 0, force_splits = 0, force_unroll = 0, force_jl = 0;
+    bool batch_member = false;   // an ensemble's cluster: lane-group plans only
     unsigned long long scratch_limit = 1ull << 30;
 
     // j-set: device SoA [x|y|z|vx|vy|vz|m|rnb2] (stride n) + FP32 mass

@@ -253,9 +253,10 @@ int abyss_ens_upload(abyss_ens_t e, int cluster, int n, const double *m, const d
             return ens_detail(e, rc, "context for cluster %d: %s", cluster,
                               abyss_error_string(rc));
         }
-        // batched clusters run unsplit with one target per lane or a lane
-        // group per target (the batch kernel's bodies)
+        // batched clusters run unsplit with a lane group per target (the
+        // batch kernel's bodies: 4, 16 or 64 lanes)
         abyss_ctx_set_tuning(c, 1, 1, 2, 0);
+        c->batch_member = true;
     }
     e->active[cluster] = 0;
     return from_ctx(e, c, abyss_ctx_upload(c, n, m, x, v, r2));

@@ -648,7 +648,10 @@ __global__ void batch_reset_kernel(int *counters)
 // busy.  A cluster's items use the same plan and body as a single-cluster
 // query, so its results do not depend on what else is in the batch.
 // Every workgroup leaves the loop once the queue index passes nitems.
-__global__ __launch_bounds__(kBlock) void batch_kernel(const ClusterDesc *__restrict__ cd,
+// Lanes-per-target bodies only (ensemble members never plan one target per
+// lane), so the kernel fits 5 waves per SIMD like regf_kernel_jl.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+void batch_kernel(const ClusterDesc *__restrict__ cd,
                                                        const WorkItem *__restrict__ items,
                                                        const BatchHeader *__restrict__ hdr,
                                                        int *__restrict__ counter, int cap)
@@ -668,10 +671,6 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(const ClusterDesc *__rest
         const int fc = d.splits == 1 ? 1 : 0;
         const QueryOut out = fc ? QueryOut{d.acc, d.adot, d.nnb, d.list} : d.part;
         switch (d.jl) {
-            case 1:
-                regf_body<1, 2, false>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock,
-                                       w.split, d.q_base, d.nt_stride, s_pos, s_m);
-                break;
             case 4:
                 regf_body_jl<4, 0>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock, w.split,
                                 d.q_base, d.nt_stride, s_pos, s_m);

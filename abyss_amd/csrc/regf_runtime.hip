@@ -147,7 +147,8 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
     }
     if (c->precision == ABYSS_PREC_FP64) jl = 1;
     // the FP64-term modes exist only in the lanes-per-target kernel
-    if ((c->precision == ABYSS_PREC_MIXED || c->precision == ABYSS_PREC_REFINED) && jl == 1)
+    if ((c->precision == ABYSS_PREC_MIXED || c->precision == ABYSS_PREC_REFINED ||
+         c->batch_member) && jl == 1)
         jl = 4;
     // One target per lane (ipl = 1, unroll 2) measured fastest at every
     // block size on MI355X; ipl 2/4 keep more targets in registers but cut

@@ -247,15 +247,22 @@ def set_precision(mode):
         raise AbyssError(rc)
 
 
-def CalculateAccelerationOnDevice(targets):
-    """Returns (acc[nt,3], adot[nt,3], NumNeighbor[nt], NeighborList[nt, NumNeighborMax])."""
+def CalculateAccelerationOnDevice(targets, out=None):
+    """Returns (acc[nt,3], adot[nt,3], NumNeighbor[nt], NeighborList[nt, NumNeighborMax]).
+    `out`: optional preallocated (acc, adot, nnb, lst) of at least those
+    shapes (C-contiguous), written in place like the reference caller's
+    buffers."""
     t = np.ascontiguousarray(targets, dtype=np.int32)
     nt = t.size
     cap = num_neighbor_max()
-    acc = np.zeros((max(nt, 1), 3))
-    adot = np.zeros((max(nt, 1), 3))
-    nnb = np.zeros(max(nt, 1), dtype=np.int32)
-    lst = np.full((max(nt, 1), cap), -1, dtype=np.int32)
+    if out is not None:
+        acc, adot, nnb, lst = out
+        assert acc.shape[0] >= nt and lst.shape[1] == cap and lst.flags.c_contiguous
+    else:
+        acc = np.zeros((max(nt, 1), 3))
+        adot = np.zeros((max(nt, 1), 3))
+        nnb = np.zeros(max(nt, 1), dtype=np.int32)
+        lst = np.full((max(nt, 1), cap), -1, dtype=np.int32)
     load().CalculateAccelerationOnDevice(ctypes.byref(ctypes.c_int(nt)), _i(t) if nt else None,
                                          _d(acc), _d(adot), _i(nnb), _i(lst))
     _check_legacy()
@@ -531,15 +538,20 @@ class Ensemble:
     def sync(self):
         self._chk(self._lib.abyss_ens_sync(self.h))
 
-    def fetch(self, cluster, fill=True):
+    def fetch(self, cluster, fill=True, out=None):
         """Results of `cluster`; list entries past each count are -1 with
-        fill=True and left uninitialised (faster) with fill=False."""
+        fill=True and left uninitialised (faster) with fill=False; `out`:
+        preallocated (acc, adot, nnb, lst) written in place."""
         nt = self.nt[cluster]
-        acc = np.empty((nt, 3))
-        adot = np.empty((nt, 3))
-        nnb = np.empty(nt, dtype=np.int32)
-        lst = (np.full((nt, self.cap), -1, dtype=np.int32) if fill
-               else np.empty((nt, self.cap), dtype=np.int32))
+        if out is not None:
+            acc, adot, nnb, lst = out
+            assert acc.shape[0] >= nt and lst.shape[1] == self.cap and lst.flags.c_contiguous
+        else:
+            acc = np.empty((nt, 3))
+            adot = np.empty((nt, 3))
+            nnb = np.empty(nt, dtype=np.int32)
+            lst = (np.full((nt, self.cap), -1, dtype=np.int32) if fill
+                   else np.empty((nt, self.cap), dtype=np.int32))
         self._chk(self._lib.abyss_ens_fetch(self.h, cluster, _d(acc), _d(adot), _i(nnb),
                                             _i(lst)))
         return acc, adot, nnb, lst
