@@ -1,6 +1,6 @@
 """GPU parity: the HIP library (through its C ABI) against the FP64 oracle.
 
-Bars (north-star accuracy contract, split precision unless stated):
+Bars (the accuracy contract of the path; split precision unless stated):
   * neighbour lists: ascending, identical to the oracle's except pairs with
     |r^2 - r_nb^2| <= 1e-6 r_nb^2, which are counted (hard_mismatches == 0);
   * a and jdot: relative error per particle |d a| / |a| <= 1e-6 at the
