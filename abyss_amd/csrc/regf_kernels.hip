@@ -414,6 +414,83 @@ __device__ __forceinline__ void jl_tile(JLLane &L, const double2 *__restrict__ p
     }
 }
 
+// The common tile of term mode 0 (all kTile records valid, no lane's own
+// target in the tile).  G consecutive steps share one neighbour test: the
+// minimum of their r^2 (v_min3 + v_min) against r_nb^2, one compare and one
+// branch instead of G of each.  r^2 is a sum of squares of finite
+// differences, so the minimum is below r_nb^2 exactly when one of the G
+// tests is.  Inside the rare branch each step is tested and listed in step
+// order, so lists stay ascending.  Every pair is evaluated with the same
+// instructions and summed in the same order as in jl_tile, so the results
+// are bitwise those of jl_tile.
+template <int JL, int G>
+__device__ __forceinline__ void jl_tile_common(JLLane &L, const double2 *__restrict__ pos,
+                                               const float *__restrict__ mass, int jt)
+{
+    constexpr int steps = kTile / JL;
+    static_assert(steps % G == 0, "the group divides the tile");
+#pragma unroll 1
+    for (int st = 0; st < steps; st += G) {
+        float dx[G], dy[G], dz[G], r2[G], rinv[G];
+        double vxj[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int r = (st + g) * JL + L.sub;
+            const double2 p0 = pos[3 * r + 0];
+            const double2 p1 = pos[3 * r + 1];
+            dx[g] = (float)(p0.x - L.xi);
+            dy[g] = (float)(p0.y - L.yi);
+            dz[g] = (float)(p1.x - L.zi);
+            vxj[g] = p1.y;
+            r2[g] = fmaf(dz[g], dz[g], fmaf(dy[g], dy[g], dx[g] * dx[g]));
+            rinv[g] = __builtin_amdgcn_rsqf(r2[g]);
+        }
+        float mn = r2[0];
+#pragma unroll
+        for (int g = 1; g < G; ++g) mn = fminf(mn, r2[g]);
+        if (__any(!(mn > L.rnbf))) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const bool near = !(r2[g] > L.rnbf);
+                const unsigned long long nb = __ballot(near);
+                if (nb) {
+                    // The lane offset passes through an empty asm so that the
+                    // list index is formed here, in the rare branch, rather
+                    // than kept in registers for every step of the tile.
+                    int sub = L.sub;
+                    asm volatile("" : "+v"(sub));
+                    if (near) asm volatile("v_mov_b32 %0, 0" : "=v"(rinv[g]));
+                    const unsigned long long gb = nb & L.gmask;
+                    if (near) {
+                        const int p = L.cnt + __popcll(gb & L.below);
+                        if (p < L.cap) L.lst[p] = jt + (st + g) * JL + sub;
+                    }
+                    L.cnt += __popcll(gb);
+                }
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int r = (st + g) * JL + L.sub;
+            const double2 p2 = pos[3 * r + 2];
+            const float mj = mass[r];
+            const float dvx = (float)(vxj[g] - L.vxi);
+            const float dvy = (float)(p2.x - L.vyi);
+            const float dvz = (float)(p2.y - L.vzi);
+            const float rinv2 = rinv[g] * rinv[g];
+            const float mr3 = mj * rinv[g] * rinv2;
+            const float rv = fmaf(dz[g], dvz, fmaf(dy[g], dvy, dx[g] * dvx));
+            const float al = -3.0f * rv * rinv2;
+            L.pa[0] = fmaf(mr3, dx[g], L.pa[0]);
+            L.pa[1] = fmaf(mr3, dy[g], L.pa[1]);
+            L.pa[2] = fmaf(mr3, dz[g], L.pa[2]);
+            L.pa[3] = fmaf(mr3, fmaf(al, dx[g], dvx), L.pa[3]);
+            L.pa[4] = fmaf(mr3, fmaf(al, dy[g], dvy), L.pa[4]);
+            L.pa[5] = fmaf(mr3, fmaf(al, dz[g], dvz), L.pa[5]);
+        }
+    }
+}
+
 template <int JL, int TM>
 __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__restrict__ tgt, int nt,
                                              int jspan, const QueryOut &out, int cap,
@@ -470,8 +547,12 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
         const int count = min(kTile, j1 - jt);
         if (__any((unsigned)(L.idx - jt) < (unsigned)count))
             jl_tile<JL, TM, false, true>(L, s_pos[b], s_m[b], count, jt);
-        else if (count == kTile)
-            jl_tile<JL, TM, true, false>(L, s_pos[b], s_m[b], count, jt);
+        else if (count == kTile) {
+            if constexpr (TM == 0)
+                jl_tile_common<JL, 4>(L, s_pos[b], s_m[b], jt);
+            else
+                jl_tile<JL, TM, true, false>(L, s_pos[b], s_m[b], count, jt);
+        }
         else
             jl_tile<JL, TM, false, false>(L, s_pos[b], s_m[b], count, jt);
         if constexpr (TM == 0) {
