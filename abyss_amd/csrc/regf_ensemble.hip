@@ -467,6 +467,7 @@ struct Request {
     int cluster = 0;
     std::vector<int> targets;
     Clock::time_point t_submit;
+    double work = 0.0;   // pair-interactions: nt * (N - 1)
     // progress through the cluster's target blocks (worker thread only)
     int nblocks = 0, next_block = 0;
     double block_cost = 0.0;   // pairs per block
@@ -506,6 +507,13 @@ struct abyss_srv_s {
     std::vector<double> latencies;
     double interactions = 0.0;
     long long batches = 0, completed = 0;
+
+    // request admission: submitted, not yet completed work (pairs) and its
+    // limit (0: unlimited); rejections and queue high-water marks
+    double max_outstanding = 0.0;
+    double outstanding = 0.0;
+    double peak_outstanding = 0.0;
+    long long rejected = 0, max_pending = 0;
 };
 
 namespace {
@@ -643,6 +651,8 @@ void complete_loop(abyss_srv_t s)
             r->latency_ms = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
             r->done = true;
             s->busy[r->cluster] = 0;
+            s->outstanding -= r->work;
+            if (s->outstanding < 0.5) s->outstanding = 0.0;   // sums of integers
             s->latencies.push_back(r->latency_ms);
             s->completed += 1;
             s->interactions += inter;
@@ -799,10 +809,22 @@ int abyss_srv_submit(abyss_srv_t s, int cluster, int nt, const int *targets, lon
         if (nt > n) return ABYSS_ERR_INVALID_ARG;
         for (int k = 0; k < nt; ++k)
             if (targets[k] < 0 || targets[k] >= n) return ABYSS_ERR_TARGET_RANGE;
+        r->work = (double)nt * (double)(n - 1);
+        // Admission: with a limit set, a request that would take the
+        // outstanding work past it is turned away, unless nothing is
+        // outstanding (so a request larger than the limit still runs alone).
+        if (s->max_outstanding > 0.0 && s->outstanding > 0.0 &&
+            s->outstanding + r->work > s->max_outstanding) {
+            s->rejected += 1;
+            return ABYSS_ERR_BUSY;
+        }
+        s->outstanding += r->work;
+        s->peak_outstanding = std::max(s->peak_outstanding, s->outstanding);
         r->id = s->next_id++;
         r->t_submit = Clock::now();
         s->reqs[r->id] = r;
         s->pending.push_back(r);
+        s->max_pending = std::max<long long>(s->max_pending, (long long)s->pending.size());
         *ticket = r->id;
     }
     s->cv_work.notify_one();
@@ -851,6 +873,27 @@ int abyss_srv_reset_stats(abyss_srv_t s)
     s->latencies.clear();
     s->interactions = 0.0;
     s->batches = s->completed = 0;
+    s->rejected = s->max_pending = 0;
+    s->peak_outstanding = s->outstanding;
+    return ABYSS_OK;
+}
+
+int abyss_srv_set_admission(abyss_srv_t s, double max_outstanding_pairs)
+{
+    if (!s || !(max_outstanding_pairs >= 0.0)) return ABYSS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->max_outstanding = max_outstanding_pairs;
+    return ABYSS_OK;
+}
+
+int abyss_srv_queue_stats(abyss_srv_t s, long long *rejected, long long *max_pending,
+                          double *peak_outstanding_pairs)
+{
+    if (!s) return ABYSS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (rejected) *rejected = s->rejected;
+    if (max_pending) *max_pending = s->max_pending;
+    if (peak_outstanding_pairs) *peak_outstanding_pairs = s->peak_outstanding;
     return ABYSS_OK;
 }
 

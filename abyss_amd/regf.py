@@ -56,7 +56,8 @@ EXPORTS = [
     "abyss_ens_set_block_range", "abyss_autotune", "abyss_autotune_load", "abyss_autotune_clear",
     "abyss_ctx_upload_state", "abyss_ctx_update_state", "abyss_ctx_predict",
     "abyss_ctx_keep_lists", "abyss_ctx_irregular", "abyss_ctx_regular_step",
-    "abyss_ctx_derivs23", "abyss_ctx_update_radius",
+    "abyss_ctx_derivs23", "abyss_ctx_update_radius", "abyss_srv_set_admission",
+    "abyss_srv_queue_stats",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -136,6 +137,10 @@ def load():
                                            ctypes.POINTER(ctypes.c_double),
                                            ctypes.POINTER(ctypes.c_double)]),
         "abyss_srv_reset_stats": (ctypes.c_int, [_vp]),
+        "abyss_srv_set_admission": (ctypes.c_int, [_vp, ctypes.c_double]),
+        "abyss_srv_queue_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong),
+                                                 ctypes.POINTER(ctypes.c_longlong),
+                                                 ctypes.POINTER(ctypes.c_double)]),
         "abyss_srv_cluster_footprint": (ctypes.c_ulonglong, [_vp, ctypes.c_int]),
         "abyss_srv_set_batch_budget": (ctypes.c_int, [_vp, ctypes.c_double]),
         "abyss_ens_blocks": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
@@ -619,10 +624,21 @@ class Server:
         if rc != OK:
             raise AbyssError(rc)
 
+    def set_admission(self, max_outstanding_pairs):
+        """Limit on submitted, not yet completed work in pair-interactions
+        (0: unlimited); requests past it are turned away by submit."""
+        rc = self._lib.abyss_srv_set_admission(self.h, float(max_outstanding_pairs))
+        if rc != OK:
+            raise AbyssError(rc)
+
     def submit(self, cluster, targets):
+        """Queues one regular step; returns (ticket, nt), or (None, nt) when
+        admission turns the request away."""
         t = np.ascontiguousarray(targets, dtype=np.int32)
         tk = ctypes.c_longlong()
         rc = self._lib.abyss_srv_submit(self.h, cluster, t.size, _i(t), ctypes.byref(tk))
+        if rc == ERR_BUSY:
+            return None, t.size
         if rc != OK:
             raise AbyssError(rc)
         return tk.value, t.size
@@ -652,8 +668,12 @@ class Server:
         i, p50, p99 = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         self._lib.abyss_srv_stats(self.h, ctypes.byref(c), ctypes.byref(b), ctypes.byref(i),
                                   ctypes.byref(p50), ctypes.byref(p99))
+        rej, mp, peak = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double()
+        self._lib.abyss_srv_queue_stats(self.h, ctypes.byref(rej), ctypes.byref(mp),
+                                        ctypes.byref(peak))
         return {"completed": c.value, "batches": b.value, "interactions": i.value,
-                "p50_ms": p50.value, "p99_ms": p99.value}
+                "p50_ms": p50.value, "p99_ms": p99.value, "rejected": rej.value,
+                "max_pending": mp.value, "peak_outstanding_pairs": peak.value}
 
     def reset_stats(self):
         self._lib.abyss_srv_reset_stats(self.h)

@@ -287,10 +287,14 @@ def served_workload(count=64, lo=4096, hi=1048576, seed=5, k_nb=32):
     return out
 
 
+ADMIT_PAIRS = 3.0e11
+
+
 def serve_stream(srv, ids, sizes, rate, window_s, seed=9):
     """Poisson request stream at `rate` req/s for `window_s` seconds; each
     request is one regular step of a random cluster with an active block
-    log-uniform in [1, N].  Returns (latencies_ms, interactions, wall_s)."""
+    log-uniform in [1, N].  Returns (latencies_ms, interactions, wall_s,
+    rejected) over the requests the server admitted."""
     import threading
     import numpy as np
     rng = np.random.default_rng(seed)
@@ -311,7 +315,9 @@ def serve_stream(srv, ids, sizes, rate, window_s, seed=9):
         while q < len(reqs):
             while tickets[q] is None:
                 time.sleep(0.0002)
-            lat[q] = srv.wait(tickets[q][0], tickets[q][1], copy=False)
+            tk, nt = tickets[q]
+            if tk is not None:   # None: turned away by admission
+                lat[q] = srv.wait(tk, nt, copy=False)
             q += 1
 
     wt = threading.Thread(target=waiter)
@@ -325,8 +331,9 @@ def serve_stream(srv, ids, sizes, rate, window_s, seed=9):
     done_sub.set()
     wt.join()
     wall = time.perf_counter() - t0
-    inter = float(sum(len(t) * (n - 1) for (_, t, n) in reqs))
-    return np.array(lat), inter, wall
+    ok = [q for q in range(len(reqs)) if tickets[q][0] is not None]
+    inter = float(sum(len(reqs[q][1]) * (reqs[q][2] - 1) for q in ok))
+    return np.array([lat[q] for q in ok]), inter, wall, len(reqs) - len(ok)
 
 
 def bench_neighbour(args, reps=5):
@@ -413,11 +420,20 @@ def bench_serve(args, rate=2000.0, window_s=1.0):
         tk, nt = srv.submit(c, np.arange(8, dtype=np.int32))
         srv.wait(tk, nt, copy=False)
     out = {}
-    for name, r, w in (("nominal", rate, window_s), ("sustained", args.serve_rate, 4.0)):
+    # nominal: every request queues (unbounded); nominal_admission: requests
+    # that would take the outstanding work past ADMIT_PAIRS (~0.25 s of the
+    # chip's work) are turned away; sustained: a rate the chip keeps up with
+    legs = (("nominal", rate, window_s, 0.0), ("nominal_admission", rate, window_s, ADMIT_PAIRS),
+            ("sustained", args.serve_rate, 4.0, 0.0))
+    for name, r, w, admit in legs:
+        srv.set_admission(admit)
         srv.reset_stats()
-        lat, inter, wall = serve_stream(srv, ids, sizes, r, w)
+        lat, inter, wall, rejected = serve_stream(srv, ids, sizes, r, w)
         st = srv.stats()
-        out[name] = {"rate_req_s": r, "window_s": w, "requests": int(len(lat)),
+        out[name] = {"rate_req_s": r, "window_s": w, "admission_pairs": admit or None,
+                     "requests": int(len(lat)) + rejected, "admitted": int(len(lat)),
+                     "rejected": rejected, "max_queued": st["max_pending"],
+                     "peak_outstanding_pairs": st["peak_outstanding_pairs"],
                      "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                      "interactions_per_s": inter / wall, "wall_s": wall, "batches": st["batches"],
                      "mean_batch_size": st["completed"] / max(1, st["batches"])}

@@ -349,6 +349,17 @@ int abyss_srv_wait(abyss_srv_t srv, long long ticket, double *acc, double *adot,
 int abyss_srv_stats(abyss_srv_t srv, long long *completed, long long *batches,
                     double *interactions, double *p50_ms, double *p99_ms);
 int abyss_srv_reset_stats(abyss_srv_t srv);
+/* Request admission: limit on the outstanding work (pair-interactions,
+ * nt * (N - 1) summed over submitted requests not yet completed); 0 (the
+ * default) means unlimited.  With a limit, abyss_srv_submit returns
+ * ABYSS_ERR_BUSY (no ticket) for a request that would take the outstanding
+ * work past it, unless nothing is outstanding. */
+int abyss_srv_set_admission(abyss_srv_t srv, double max_outstanding_pairs);
+/* Requests turned away by admission, the most requests waiting to start at
+ * once, and the peak outstanding work (pairs) since creation or the last
+ * reset. */
+int abyss_srv_queue_stats(abyss_srv_t srv, long long *rejected, long long *max_pending,
+                          double *peak_outstanding_pairs);
 /* Estimated device bytes a cluster of n particles needs (admission rule). */
 unsigned long long abyss_srv_cluster_footprint(abyss_srv_t srv, int n);
 

@@ -196,3 +196,36 @@ def test_server_admission_by_budget(clusters):
         tk, nt = srv.submit(b, np.arange(10, dtype=np.int32))
         res = srv.wait(tk, nt)
         check_vs_oracle(clusters[6], np.arange(10, dtype=np.int32), res[:4], "admitted")
+
+
+def test_server_request_admission(clusters):
+    say("case: server request admission (outstanding-work limit)")
+    x, v, m, r2 = clusters[10]             # n = 65536
+    n = len(m)
+    big = np.arange(n, dtype=np.int32)     # 65536 x 65535 pairs
+    small = np.arange(16, dtype=np.int32)
+    with regf.Server(device=0, cap=128, max_clusters=2, hbm_budget=8 << 30) as srv:
+        c = srv.add_cluster(m, x, v, r2)
+        srv.set_batch_budget(2e7)          # the big request spans ~200 batches (ms)
+        srv.set_admission(1.0)             # only a lone request is admitted
+        tk_big, nt_big = srv.submit(c, big)
+        assert tk_big is not None          # nothing outstanding: admitted
+        tk2, _ = srv.submit(c, small)      # microseconds later the big one still runs
+        srv.wait(tk_big, nt_big, copy=False)
+        st = srv.stats()
+        say(f"  stats {st}")
+        assert tk2 is None and st["rejected"] == 1
+        assert st["peak_outstanding_pairs"] == float(n) * (n - 1)
+        # the queue has drained: the next request is admitted and correct
+        tk3, nt3 = srv.submit(c, small)
+        assert tk3 is not None
+        check_vs_oracle(clusters[10], small, srv.wait(tk3, nt3)[:4], "admitted after drain")
+        srv.set_admission(0)               # unlimited again: a burst all queues
+        tks = [srv.submit(c, small) for _ in range(5)]
+        assert all(t is not None for t, _ in tks)
+        for t, nt in tks:
+            srv.wait(t, nt, copy=False)
+        st = srv.stats()
+        assert st["rejected"] == 1 and st["completed"] == 7 and st["max_pending"] >= 1
+        srv.reset_stats()
+        assert srv.stats()["rejected"] == 0
