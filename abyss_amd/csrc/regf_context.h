@@ -14,7 +14,7 @@ struct abyss_ctx_s {
     int precision = ABYSS_PREC_SPLIT;
     int force_ipl = 0, force_splits = 0, force_unroll = 0, force_jl = 0;
     bool batch_member = false;   // an ensemble's cluster: lane-group plans only
-    unsigned long long scratch_limit = 1ull << 30;
+    unsigned long long scratch_limit = 2ull << 30;   // split-j scratch bound
 
     // j-set: device SoA [x|y|z|vx|vy|vz|m|rnb2] (stride n) + FP32 mass
     int n = 0;

@@ -167,11 +167,13 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
     p.grid_x = (nt + tpb - 1) / tpb;
     int S = c->force_splits;
     if (S == 0) {
-        // Workgroups per CU over the whole launch (or the tuned count):
-        // ~48 for the 1- and 4-lane kernels (tail of the last wave of
-        // workgroups; measured best at m = 16K..256K, N = 256K), 16 for the
-        // 16/64-lane ones.
-        const int want_wg = cu * (tuned ? te.wg_per_cu : (jl <= 4 ? 48 : 16));
+        // Workgroups per CU over the whole launch (or the tuned count), which
+        // sets the tail of the last round of workgroups: for the 1- and
+        // 4-lane kernels 96 from 65536 targets up and 48 below (at N = 256K:
+        // m = 65536..262144 run 1.0-1.3 % faster with 96 than with 48,
+        // m = 4096 and 16384 best at 48), 16 for the 16/64-lane ones.
+        const int wg_rule = jl <= 4 ? (nt >= 65536 ? 96 : 48) : 16;
+        const int want_wg = cu * (tuned ? te.wg_per_cu : wg_rule);
         S = (want_wg + p.grid_x - 1) / p.grid_x;
         // every split keeps at least 4 tiles (1024 j): finer splits cost
         // more in finalize than they gain (m = 1 at N = 262144: S = 256

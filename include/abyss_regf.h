@@ -144,7 +144,8 @@ int abyss_ctx_set_precision(abyss_ctx_t ctx, int mode);
  * or 64; > 1 implies one target per lane group); 0 = automatic for each. */
 int abyss_ctx_set_tuning(abyss_ctx_t ctx, int i_per_lane, int j_splits, int j_unroll,
                          int lanes_per_target);
-/* Upper bound on the split-j scratch (bytes; default 1 GiB). */
+/* Upper bound on the split-j scratch, S * m * (52 + 4 * cap) bytes for S
+ * j-splits of m targets (bytes; default 2 GiB).  A smaller bound caps S. */
 int abyss_ctx_set_scratch_limit(abyss_ctx_t ctx, unsigned long long bytes);
 
 /* Upload a j-set: x, v as [n][3], m and r2 as [n] (FP64, host memory). */
