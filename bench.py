@@ -236,6 +236,19 @@ def bench_ensemble(args):
         times.append((time.perf_counter() - w0) * 1e3)
     ms_plain = float(np.median(times))
     log(f"per-cluster device-resident launches: {ms_plain:.3f} ms/step")
+    # (2b) the same launches with the plain kernel shape: one target per
+    # lane, no j-splits (the reference kernel's decomposition), same precision
+    for c, (_, _, _, _, t) in zip(ctxs, cl):
+        c.set_tuning(1, 1, 2, 1)
+        c.set_targets(t)
+        c.launch()
+        c.sync()
+    w0 = time.perf_counter()
+    for c in ctxs:
+        c.launch()
+        c.sync()
+    ms_unplanned = (time.perf_counter() - w0) * 1e3
+    log(f"per-cluster launches, one target per lane, no j-splits: {ms_unplanned:.1f} ms/step")
     for c in ctxs:
         c.close()
     # (3) per-cluster legacy ABI calls (upload + query + results on the host)
@@ -258,9 +271,11 @@ def bench_ensemble(args):
         "batched_ms_per_step": ms_batched,
         "interactions_per_s": inter / (ms_batched * 1e-3),
         "per_cluster_device_launches_ms": ms_plain,
+        "per_cluster_one_target_per_lane_unsplit_ms": ms_unplanned,
         "per_cluster_legacy_calls_ms": ms_legacy,
         "batched_with_fetch_ms": ms_batched_fetch,
         "speedup_vs_device_launches": ms_plain / ms_batched,
+        "speedup_vs_one_target_per_lane_unsplit": ms_unplanned / ms_batched,
         "speedup_vs_legacy_calls": ms_legacy / ms_batched_fetch,
         "accuracy": {"worst_normalised_error": worst, "list_hard_mismatches": hard},
     }
