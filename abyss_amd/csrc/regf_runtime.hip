@@ -145,6 +145,24 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
         else if (nt >= 16) jl = 16;
         else jl = 64;
     }
+    // A lane group wide enough to fill the chip: the j-split count is capped
+    // (>= 4 tiles per split, kMaxSplits, scratch), so on a small j-set the
+    // 4-lane kernel may get too few workgroups.  Move to 16 lanes below 4
+    // workgroups per CU, and from 16 to 64 below one per CU (Plummer +
+    // Kroupa N = 65536: m = 256 53.3 -> 35.7 us, m = 16 26.0 -> 19.9 us;
+    // King N = 262144 keeps its choices).
+    if (jl > 1 && c->force_jl == 0 && !tuned && c->force_splits == 0 && !c->batch_member &&
+        c->precision != ABYSS_PREC_FP64) {
+        int s_max = std::min(std::max(1, n / (4 * abyss::kTile)), abyss::kMaxSplits);
+        const double per = (double)nt * ((double)c->cap * 4.0 + 6 * 8.0 + 4.0);
+        s_max = std::min(s_max, (int)std::max(1.0, std::floor((double)c->scratch_limit / per)));
+        auto groups = [&](int l) {
+            const int tpb = abyss::kBlock / l;
+            return (double)((nt + tpb - 1) / tpb) * s_max;
+        };
+        if (jl == 4 && groups(4) < 4.0 * cu) jl = 16;
+        if (jl == 16 && groups(16) < 1.0 * cu) jl = 64;
+    }
     if (c->precision == ABYSS_PREC_FP64) jl = 1;
     // the FP64-term modes exist only in the lanes-per-target kernel
     if ((c->precision == ABYSS_PREC_MIXED || c->precision == ABYSS_PREC_REFINED ||

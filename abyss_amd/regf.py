@@ -443,14 +443,25 @@ class Context:
             allow_overflow=True)
         return acc, adot, nnb, lst
 
-    def query(self, targets):
-        """Synchronous query: (acc, adot, nnb, lists[nt, cap])."""
+    def query(self, targets, out=None):
+        """Synchronous query: (acc, adot, nnb, lists[nt, cap]).  `out` may hold
+        preallocated (acc[nt,3], adot[nt,3], nnb[nt], lists[nt,cap]) arrays
+        (C-contiguous float64 / int32), which are filled and returned."""
         t = np.ascontiguousarray(targets, dtype=np.int32)
         nt = t.size
-        acc = np.zeros((max(nt, 1), 3))
-        adot = np.zeros((max(nt, 1), 3))
-        nnb = np.zeros(max(nt, 1), dtype=np.int32)
-        lst = np.full((max(nt, 1), self.cap), -1, dtype=np.int32)
+        if out is not None:
+            acc, adot, nnb, lst = out
+            if (acc.shape != (nt, 3) or adot.shape != (nt, 3) or nnb.shape != (nt,)
+                    or lst.shape != (nt, self.cap) or acc.dtype != np.float64
+                    or adot.dtype != np.float64 or nnb.dtype != np.int32
+                    or lst.dtype != np.int32 or not all(a.flags.c_contiguous for a in out)):
+                raise ValueError("query: out arrays must be C-contiguous acc/adot float64 "
+                                 "[nt,3], nnb int32 [nt], lists int32 [nt,cap]")
+        else:
+            acc = np.zeros((max(nt, 1), 3))
+            adot = np.zeros((max(nt, 1), 3))
+            nnb = np.zeros(max(nt, 1), dtype=np.int32)
+            lst = np.full((max(nt, 1), self.cap), -1, dtype=np.int32)
         self.last_status = self._chk(
             self._lib.abyss_ctx_query(self.h, nt, _i(t) if nt else None, _d(acc), _d(adot),
                                       _i(nnb), _i(lst)), allow_overflow=True)

@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3
 FLOP_PER_INTERACTION = 60
+HBM_PEAK_BPS = 8.0e12
 
 
 def log(*a):
@@ -351,6 +352,77 @@ def serve_stream(srv, ids, sizes, rate, window_s, seed=9):
     return np.array([lat[q] for q in ok]), inter, wall, len(reqs) - len(ok)
 
 
+def bench_blocks(args):
+    """Active-block sweep on Plummer + Kroupa N = 65536 (cap 256, ~100
+    neighbours): per block size m, the device-resident launch (HIP events on
+    the context's stream, inputs in HBM) and the synchronous query (targets
+    to the device, launch, results into the caller's preallocated host
+    arrays; wall clock, median), with interactions/s, the
+    FP32 VALU fraction and the HBM fraction of the algorithmic bytes
+    68 B * N (j-set) + 64 B * m (targets) + (52 + 4 n_nb) B * m (results)."""
+    import numpy as np
+    from abyss_amd import regf, workloads
+    js = workloads.plummer_kroupa(65536)
+    n = js.n
+    ctx = regf.Context(device=0, cap=js.cap)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    regf.set_num_neighbor_max(js.cap)
+    regf.OpenDevice(0)
+    ev = HipEvents()
+    rng = np.random.default_rng(7)
+    rows = []
+    for m in (1, 4, 16, 64, 256, 1024, 4096, 16384, 65536):
+        t = np.sort(rng.permutation(n)[:m]).astype(np.int32)
+        inter = float(m) * (n - 1)
+        reps = int(min(2000, max(10, 0.2 / max(inter / 1.2e12, 1e-5))))
+        ctx.set_targets(t)
+        ctx.launch()
+        ctx.sync()
+        e0, e1 = ev.create(), ev.create()
+        ev.record(e0, ctx.stream)
+        for _ in range(reps):
+            ctx.launch()
+        ev.record(e1, ctx.stream)
+        ms = ev.elapsed_ms(e0, e1) / reps
+        ctx.sync()
+        nnb = ctx.fetch()[2]
+        out = (np.zeros((m, 3)), np.zeros((m, 3)), np.zeros(m, np.int32),
+               np.zeros((m, js.cap), np.int32))     # the caller's result arrays
+        ctx.query(t, out=out)
+        walls = []
+        for _ in range(min(reps, 200)):
+            w0 = time.perf_counter()
+            ctx.query(t, out=out)
+            walls.append((time.perf_counter() - w0) * 1e3)
+        q_ms = float(np.median(walls))
+        byts = 68.0 * n + 64.0 * m + (52.0 + 4.0 * float(np.abs(nnb).mean())) * m
+        # the reference's calling pattern: the whole j-set uploaded, then the
+        # block queried, through the legacy entry points
+        regf.SendToDevice(js.m, js.x, js.v, js.r2)
+        regf.CalculateAccelerationOnDevice(t, out=out)
+        lwalls = []
+        for _ in range(min(reps, 50)):
+            w0 = time.perf_counter()
+            regf.SendToDevice(js.m, js.x, js.v, js.r2)
+            regf.CalculateAccelerationOnDevice(t, out=out)
+            lwalls.append((time.perf_counter() - w0) * 1e3)
+        l_ms = float(np.median(lwalls))
+        row = {"m": m, "plan": ctx.plan(), "launch_us": ms * 1e3, "query_us": q_ms * 1e3,
+               "legacy_upload_and_call_us": l_ms * 1e3,
+               "interactions_per_s": inter / (ms * 1e-3),
+               "valu_frac": FLOP_PER_INTERACTION * inter / (ms * 1e-3) / (FP32_PEAK_TFLOPS * 1e12),
+               "hbm_bytes": byts, "hbm_frac": byts / (ms * 1e-3) / HBM_PEAK_BPS}
+        log(f"m={m:6d}: launch {row['launch_us']:9.1f} us, query {row['query_us']:9.1f} us, "
+            f"legacy upload+call {row['legacy_upload_and_call_us']:9.1f} us, "
+            f"{row['interactions_per_s']:.3e} int/s, VALU {100 * row['valu_frac']:5.1f} %, "
+            f"HBM {100 * row['hbm_frac']:5.2f} %, plan {row['plan']}")
+        rows.append(row)
+    ctx.close()
+    regf.CloseDevice()
+    regf.set_num_neighbor_max(1000)
+    return {"workload": "plummer_kroupa_n65536_cap256", "rows": rows}
+
+
 def bench_neighbour(args, reps=5):
     """Irregular force over resident lists and the fused regular step
     (neighbour-set correction + corrector) on the King N=262144 workload,
@@ -475,7 +547,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration)")
-    ap.add_argument("--workload", choices=["king262144", "ensemble", "serve", "neighbour"],
+    ap.add_argument("--workload", choices=["king262144", "ensemble", "serve", "neighbour",
+                                           "blocks"],
                     default="king262144")
     ap.add_argument("--serve-rate", type=float, default=100.0,
                     help="sustained request rate of the served-ensemble leg (req/s)")
@@ -486,6 +559,10 @@ def main():
     if args.workload == "ensemble":
         print(json.dumps({"metric": "batched ensemble regular-force pair-interactions/s",
                           "batched_ensemble": bench_ensemble(args)}), flush=True)
+        return
+    if args.workload == "blocks":
+        print(json.dumps({"metric": "regular-force active-block sweep (latency and rates)",
+                          "blocks": bench_blocks(args)}), flush=True)
         return
     if args.workload == "neighbour":
         print(json.dumps({"metric": "irregular force and fused regular step (wall-clock)",
