@@ -28,16 +28,16 @@ struct abyss_ctx_s {
     bool counts_valid = false;   // d_nnb holds the counts of the current targets
     size_t t_capacity = 0;  // targets the output buffers can hold
     int *d_tgt = nullptr, *h_tgt = nullptr;
-    // packed per-target results (device and pinned host, same layout):
-    // acc [nt][3] | adot [nt][3] | nnb [nt] | stats [2]; the pointers below
-    // are views of d_out / h_out for the current nt
+    // per-target results (device block and pinned host mirror, same
+    // layout): acc [nt][3] | adot [nt][3] | nnb [nt] | pad to 256 B | list
+    // rows [nt][cap]; the pointers below are views of d_out / h_out for the
+    // current nt (the host mirror holds rows only up to the one-copy size)
     char *d_out = nullptr, *h_out = nullptr;
     double *d_acc = nullptr, *d_adot = nullptr;
-    int *d_nnb = nullptr, *d_stats = nullptr;
+    int *d_nnb = nullptr, *d_list = nullptr;
     double *h_res = nullptr;  // acc then adot
-    int *h_nnb = nullptr, *h_stats = nullptr;
-    int *d_list = nullptr;
-    int *h_list = nullptr;    // pinned, rows of width <= cap
+    int *h_nnb = nullptr, *h_rows = nullptr;
+    int *h_list = nullptr;    // pinned, populated columns of larger results
     size_t h_list_bytes = 0;
 
     // device-resident particle state (regf_state.hip): SoA x,v,a,adot (3n

@@ -111,9 +111,4 @@ hipError_t launch_finalize(const QueryOut &part, int S, int nt, int cap,
                            double *acc, double *adot, int *nnb, int *list,
                            hipStream_t st);
 
-// Largest per-target count (clipped to cap) and number of overflowed
-// targets, for sizing the device->host list copy.
-hipError_t launch_count_stats(const int *nnb, int nt, int cap, int *d_stats,
-                              hipStream_t st);
-
 }  // namespace abyss

@@ -687,36 +687,6 @@ __global__ __launch_bounds__(256) void finalize_kernel(QueryOut part, int S, int
     finalize_body(part, S, nt, cap, q, q, q < nt, s_off_all + w * (S + 1), acc, adot, nnb, list);
 }
 
-// Single workgroup: d_stats[0] = max over targets of min(|nnb|, cap),
-// d_stats[1] = number of overflowed targets (nnb < 0).
-__global__ __launch_bounds__(1024) void count_stats_kernel(const int *__restrict__ nnb, int nt,
-                                                           int cap, int *__restrict__ d_stats)
-{
-    __shared__ int s_max[1024];
-    __shared__ int s_ovf[1024];
-    int mx = 0, ov = 0;
-    for (int q = threadIdx.x; q < nt; q += blockDim.x) {
-        const int c = nnb[q];
-        const int a = c < 0 ? -c : c;
-        mx = max(mx, min(a, cap));
-        ov += c < 0 ? 1 : 0;
-    }
-    s_max[threadIdx.x] = mx;
-    s_ovf[threadIdx.x] = ov;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            s_max[threadIdx.x] = max(s_max[threadIdx.x], s_max[threadIdx.x + w]);
-            s_ovf[threadIdx.x] += s_ovf[threadIdx.x + w];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        d_stats[0] = s_max[0];
-        d_stats[1] = s_ovf[0];
-    }
-}
-
 // ---- batched ensembles ---------------------------------------------------
 
 __global__ void batch_reset_kernel(int *counters)
@@ -906,12 +876,6 @@ hipError_t launch_batch_finalize(const ClusterDesc *cd, const int *fin_cl, const
 {
     hipLaunchKernelGGL(batch_finalize_kernel, dim3(grid), dim3(256), 0, st, cd, fin_cl, fin_off,
                        hdr, cap);
-    return hipGetLastError();
-}
-
-hipError_t launch_count_stats(const int *nnb, int nt, int cap, int *d_stats, hipStream_t st)
-{
-    hipLaunchKernelGGL(count_stats_kernel, dim3(1), dim3(1024), 0, st, nnb, nt, cap, d_stats);
     return hipGetLastError();
 }
 

@@ -85,24 +85,27 @@ void bind_jstate(abyss_ctx_t c, char *base, int n)
 }
 
 // Bytes of the packed per-target results of nt targets:
-// acc [nt][3] | adot [nt][3] (FP64) | nnb [nt] | stats [2] (int32).
+// acc [nt][3] | adot [nt][3] (FP64) | nnb [nt] (int32), padded to 256 B.
+// The list rows [nt][cap] follow at that offset in the same device block.
 size_t packed_bytes(size_t nt)
 {
-    return nt * (6 * sizeof(double) + sizeof(int)) + 2 * sizeof(int);
+    const size_t b = nt * (6 * sizeof(double) + sizeof(int));
+    return (b + 255) & ~(size_t)255;
 }
 
-// Views of the packed result blocks for the current target count, so that
-// one copy moves every per-target result (and the list statistics).
+// Views of the output block for the current target count: acc, adot, nnb,
+// then the list rows, so that one copy can move every result of a small
+// query.  The host block mirrors the layout up to kOneTripListBytes of rows.
 void set_output_views(abyss_ctx_t c, int nt)
 {
     const size_t n = (size_t)nt;
     c->d_acc = reinterpret_cast<double *>(c->d_out);
     c->d_adot = c->d_acc + 3 * n;
     c->d_nnb = reinterpret_cast<int *>(c->d_adot + 3 * n);
-    c->d_stats = c->d_nnb + n;
+    c->d_list = reinterpret_cast<int *>(c->d_out + packed_bytes(n));
     c->h_res = reinterpret_cast<double *>(c->h_out);
     c->h_nnb = reinterpret_cast<int *>(c->h_res + 6 * n);
-    c->h_stats = c->h_nnb + n;
+    c->h_rows = reinterpret_cast<int *>(c->h_out + packed_bytes(n));
 }
 
 int ensure_outputs(abyss_ctx_t c, int nt)
@@ -112,14 +115,16 @@ int ensure_outputs(abyss_ctx_t c, int nt)
         return ABYSS_OK;
     }
     size_t cap_t = grow((size_t)nt);
-    dfree(c->d_tgt); dfree(c->d_out); dfree(c->d_list);
+    dfree(c->d_tgt); dfree(c->d_out);
     hfree(c->h_tgt); hfree(c->h_out);
     c->t_capacity = 0;
+    c->d_list = nullptr;
+    const size_t rows = cap_t * (size_t)c->cap * sizeof(int);
     HIPCHK(c, hipMalloc(&c->d_tgt, cap_t * sizeof(int)));
-    HIPCHK(c, hipMalloc(&c->d_out, packed_bytes(cap_t)));
-    HIPCHK(c, hipMalloc(&c->d_list, cap_t * (size_t)c->cap * sizeof(int)));
+    HIPCHK(c, hipMalloc(&c->d_out, packed_bytes(cap_t) + rows));
     HIPCHK(c, hipHostMalloc(&c->h_tgt, cap_t * sizeof(int), hipHostMallocDefault));
-    HIPCHK(c, hipHostMalloc(&c->h_out, packed_bytes(cap_t), hipHostMallocDefault));
+    HIPCHK(c, hipHostMalloc(&c->h_out, packed_bytes(cap_t) + std::min(rows, kOneTripListBytes),
+                            hipHostMallocDefault));
     c->t_capacity = cap_t;
     set_output_views(c, nt);
     return ABYSS_OK;
@@ -256,7 +261,8 @@ void release_all(abyss_ctx_t c)
     if (c->device >= 0) (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dfree(c->d_j); hfree(c->h_j);
-    dfree(c->d_tgt); dfree(c->d_out); dfree(c->d_list);
+    dfree(c->d_tgt); dfree(c->d_out);
+    c->d_list = nullptr;
     dfree(c->d_scratch); dfree(c->d_state); dfree(c->d_upd);
     dfree(c->d_rlists); dfree(c->d_irr); dfree(c->d_aos); dfree(c->d_drv); dfree(c->d_rad);
     hfree(c->h_drv);
@@ -347,7 +353,7 @@ unsigned long long ctx_device_bytes(abyss_ctx_t c)
     return (unsigned long long)c->j_bytes + c->scratch_bytes + c->state_bytes + c->upd_bytes +
            c->rlist_bytes + c->irr_bytes + c->aos_bytes + c->drv_bytes + c->rad_bytes +
            (unsigned long long)c->t_capacity * (sizeof(int) * 2 + 6 * sizeof(double) +
-                                                (size_t)c->cap * sizeof(int));
+                                                (size_t)c->cap * sizeof(int)) + 256;
 }
 
 }  // namespace abyss
@@ -582,30 +588,29 @@ int ctx_fetch_on_stream(abyss_ctx_t c, hipStream_t st, double *acc, double *adot
     HIPCHK(c, hipSetDevice(c->device));
     const int nt = c->nt;
     const int cap = c->cap;
-    HIPCHK(c, abyss::launch_count_stats(c->d_nnb, nt, cap, c->d_stats, st));
-    // acc, adot, counts and list statistics are one packed block: one copy
-    HIPCHK(c, hipMemcpyAsync(c->h_out, c->d_out, packed_bytes((size_t)nt), hipMemcpyDeviceToHost,
-                             st));
-    // small results: copy the full list rows in the same round trip rather
-    // than waiting for the populated width
+    // Small results: acc, adot, counts and the full list rows are one
+    // contiguous block, moved by one copy.  Larger ones: the counts first,
+    // then only the populated list columns.
     const size_t full_bytes = (size_t)nt * cap * sizeof(int);
     const bool one_trip = full_bytes <= kOneTripListBytes;
-    if (one_trip) {
-        if (full_bytes > c->h_list_bytes) {
-            hfree(c->h_list);
-            c->h_list_bytes = 0;
-            HIPCHK(c, hipHostMalloc(&c->h_list, grow(full_bytes), hipHostMallocDefault));
-            c->h_list_bytes = grow(full_bytes);
-        }
-        HIPCHK(c, hipMemcpyAsync(c->h_list, c->d_list, full_bytes, hipMemcpyDeviceToHost, st));
-    }
+    HIPCHK(c, hipMemcpyAsync(c->h_out, c->d_out,
+                             one_trip ? packed_bytes((size_t)nt) + full_bytes
+                                      : (size_t)nt * (6 * sizeof(double) + sizeof(int)),
+                             hipMemcpyDeviceToHost, st));
     HIPCHK(c, hipStreamSynchronize(st));
     if (st == c->stream) {
         int rc = collect_timing(c);
         if (rc != ABYSS_OK) return rc;
     }
-    const int overflowed = c->h_stats[1];
-    const int width = one_trip ? cap : c->h_stats[0];  // row stride of h_list
+    // populated width and overflow count, from the counts on the host
+    int width = 0, overflowed = 0;
+    for (int k = 0; k < nt; ++k) {
+        const int cnt = c->h_nnb[k];
+        overflowed += cnt < 0;
+        width = std::max(width, std::min(cnt < 0 ? -cnt : cnt, cap));
+    }
+    const int *rows = c->h_rows;
+    int stride = cap;   // row stride of `rows`
     if (!one_trip && width > 0) {
         const size_t need = (size_t)nt * width * sizeof(int);
         if (need > c->h_list_bytes) {
@@ -618,6 +623,8 @@ int ctx_fetch_on_stream(abyss_ctx_t c, hipStream_t st, double *acc, double *adot
                                    (size_t)cap * sizeof(int), (size_t)width * sizeof(int), nt,
                                    hipMemcpyDeviceToHost, st));
         HIPCHK(c, hipStreamSynchronize(st));
+        rows = c->h_list;
+        stride = width;
     }
     memcpy(acc, c->h_res, (size_t)nt * 3 * sizeof(double));
     memcpy(adot, c->h_res + (size_t)nt * 3, (size_t)nt * 3 * sizeof(double));
@@ -626,7 +633,7 @@ int ctx_fetch_on_stream(abyss_ctx_t c, hipStream_t st, double *acc, double *adot
         const int cnt = c->h_nnb[k];
         const int ncopy = std::min(cnt < 0 ? -cnt : cnt, cap);
         if (ncopy > 0)
-            memcpy(list + (size_t)k * cap, c->h_list + (size_t)k * width,
+            memcpy(list + (size_t)k * cap, rows + (size_t)k * stride,
                    (size_t)ncopy * sizeof(int));
     }
     if (overflowed > 0)
