@@ -28,6 +28,10 @@ struct abyss_ctx_s {
     bool counts_valid = false;   // d_nnb holds the counts of the current targets
     size_t t_capacity = 0;  // targets the output buffers can hold
     int *d_tgt = nullptr, *h_tgt = nullptr;
+    // h_tgt is mapped, coherent pinned memory; h_tgt_dev is its device
+    // address.  `tgt` is the target array the kernels read: h_tgt_dev for
+    // small blocks (no copy), d_tgt otherwise.
+    const int *h_tgt_dev = nullptr, *tgt = nullptr;
     // per-target results (device block and pinned host mirror, same
     // layout): acc [nt][3] | adot [nt][3] | nnb [nt] | pad to 256 B | list
     // rows [nt][cap]; the pointers below are views of d_out / h_out for the

@@ -263,7 +263,7 @@ int abyss_ctx_update_radius(abyss_ctx_t c, int fix, int nmax, double *r2_out)
     double *st_r2 = (c->state_n == c->n && c->d_state) ? c->d_state + 14 * (size_t)c->n : nullptr;
     double *d_r2 = reinterpret_cast<double *>(c->d_rad);
     hipLaunchKernelGGL(update_radius_kernel, dim3((nt + 255) / 256), dim3(256), 0, c->stream,
-                       c->d_tgt, nt, c->d_nnb, fix, nmax, st_r2, j_r2, d_r2);
+                       c->tgt, nt, c->d_nnb, fix, nmax, st_r2, j_r2, d_r2);
     if (hipGetLastError() != hipSuccess) return ABYSS_ERR_HIP;
     if (hipMemcpyAsync(r2_out, d_r2, (size_t)nt * sizeof(double), hipMemcpyDeviceToHost,
                        c->stream) != hipSuccess ||

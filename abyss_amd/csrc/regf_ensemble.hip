@@ -374,7 +374,7 @@ int abyss_ens_launch(abyss_ens_t e)
         const int tpb = abyss::targets_per_block(c->plan);
         ClusterDesc &d = e->h_cd[a];
         d.js = c->js;
-        d.tgt = c->d_tgt;
+        d.tgt = c->tgt;
         d.nt = c->nt;
         d.jl = c->plan.jl;
         d.splits = en.splits;

@@ -311,7 +311,7 @@ int keep_launch_lists(abyss_ctx_t c)
     int *rnnb = reinterpret_cast<int *>(c->d_rlists);
     int *rlist = rnnb + (((size_t)c->n + 63) & ~size_t(63));
     hipLaunchKernelGGL(keep_lists_kernel, dim3((c->nt + 3) / 4), dim3(256), 0, c->stream,
-                       c->d_tgt, c->nt, c->d_nnb, c->d_list, c->cap, rnnb, rlist);
+                       c->tgt, c->nt, c->d_nnb, c->d_list, c->cap, rnnb, rlist);
     return hipGetLastError() == hipSuccess ? ABYSS_OK : ABYSS_ERR_HIP;
 }
 
@@ -436,7 +436,7 @@ int abyss_ctx_regular_step(abyss_ctx_t c, int nt, const int *targets, const doub
     rc = ensure_aos(c);
     if (rc != ABYSS_OK) return rc;
     hipLaunchKernelGGL(correct_kernel, dim3((nt + 3) / 4), dim3(256), 0, c->stream,
-                       reinterpret_cast<const JRec *>(c->d_aos), c->d_tgt, nt, c->d_nnb,
+                       reinterpret_cast<const JRec *>(c->d_aos), c->tgt, nt, c->d_nnb,
                        c->d_list, c->d_acc, c->d_adot, c->cap, rnnb, rlist, d_in, d_in + nt,
                        d_out);
     if (hipGetLastError() != hipSuccess) return ABYSS_ERR_HIP;

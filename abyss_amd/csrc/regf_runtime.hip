@@ -27,6 +27,9 @@ namespace {
 // Below this many bytes of full-width list rows, fetch copies them with the
 // other results in one round trip instead of first reading the widest row.
 constexpr size_t kOneTripListBytes = 1u << 20;
+// Up to this many targets the kernels read the target list straight from
+// mapped host memory instead of a device copy (saves one copy per query).
+constexpr int kMappedTargets = 64;
 
 int set_detail(abyss_ctx_t c, int code, const char *fmt, ...)
 {
@@ -119,10 +122,15 @@ int ensure_outputs(abyss_ctx_t c, int nt)
     hfree(c->h_tgt); hfree(c->h_out);
     c->t_capacity = 0;
     c->d_list = nullptr;
+    c->h_tgt_dev = c->tgt = nullptr;
     const size_t rows = cap_t * (size_t)c->cap * sizeof(int);
     HIPCHK(c, hipMalloc(&c->d_tgt, cap_t * sizeof(int)));
     HIPCHK(c, hipMalloc(&c->d_out, packed_bytes(cap_t) + rows));
-    HIPCHK(c, hipHostMalloc(&c->h_tgt, cap_t * sizeof(int), hipHostMallocDefault));
+    HIPCHK(c, hipHostMalloc(&c->h_tgt, cap_t * sizeof(int),
+                            hipHostMallocMapped | hipHostMallocCoherent));
+    void *dev = nullptr;
+    HIPCHK(c, hipHostGetDevicePointer(&dev, c->h_tgt, 0));
+    c->h_tgt_dev = static_cast<const int *>(dev);
     HIPCHK(c, hipHostMalloc(&c->h_out, packed_bytes(cap_t) + std::min(rows, kOneTripListBytes),
                             hipHostMallocDefault));
     c->t_capacity = cap_t;
@@ -263,6 +271,7 @@ void release_all(abyss_ctx_t c)
     dfree(c->d_j); hfree(c->h_j);
     dfree(c->d_tgt); dfree(c->d_out);
     c->d_list = nullptr;
+    c->h_tgt_dev = c->tgt = nullptr;
     dfree(c->d_scratch); dfree(c->d_state); dfree(c->d_upd);
     dfree(c->d_rlists); dfree(c->d_irr); dfree(c->d_aos); dfree(c->d_drv); dfree(c->d_rad);
     hfree(c->h_drv);
@@ -491,8 +500,15 @@ int abyss_ctx_set_targets(abyss_ctx_t c, int nt, const int *targets)
     int rc = ensure_outputs(c, nt);
     if (rc != ABYSS_OK) return rc;
     memcpy(c->h_tgt, targets, (size_t)nt * sizeof(int));
-    HIPCHK(c, hipMemcpyAsync(c->d_tgt, c->h_tgt, (size_t)nt * sizeof(int), hipMemcpyHostToDevice,
-                             c->stream));
+    if (nt <= kMappedTargets && !c->batch_member) {
+        // small block: kernels read the (coherent) pinned copy directly; the
+        // stream is idle, so no launch still reads the previous list
+        c->tgt = c->h_tgt_dev;
+    } else {
+        HIPCHK(c, hipMemcpyAsync(c->d_tgt, c->h_tgt, (size_t)nt * sizeof(int),
+                                 hipMemcpyHostToDevice, c->stream));
+        c->tgt = c->d_tgt;
+    }
     c->nt = nt;
     c->counts_valid = false;   // counts belong to the next launch
     c->plan = make_plan(c, nt, c->n);
@@ -536,10 +552,10 @@ int ctx_launch_force(abyss_ctx_t c)
     QueryOut fin{c->d_acc, c->d_adot, c->d_nnb, c->d_list};
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (p.splits == 1) {
-        HIPCHK(c, abyss::launch_regf(c->js, c->d_tgt, c->nt, p, fin, c->cap, c->precision,
+        HIPCHK(c, abyss::launch_regf(c->js, c->tgt, c->nt, p, fin, c->cap, c->precision,
                                      c->stream));
     } else {
-        HIPCHK(c, abyss::launch_regf(c->js, c->d_tgt, c->nt, p, c->part, c->cap, c->precision,
+        HIPCHK(c, abyss::launch_regf(c->js, c->tgt, c->nt, p, c->part, c->cap, c->precision,
                                      c->stream));
         HIPCHK(c, abyss::launch_finalize(c->part, p.splits, c->nt, c->cap, c->d_acc, c->d_adot,
                                          c->d_nnb, c->d_list, c->stream));
