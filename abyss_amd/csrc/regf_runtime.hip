@@ -8,7 +8,9 @@
 // design: validation happens before any HIP call and is reported through
 // status codes; device buffers are sized from the actual N and targets
 // (grow-only) instead of next-pow2 staging of GridDimY*NNB_per_block*N ints;
-// only the populated columns of the neighbour lists cross PCIe.
+// small results (all list rows <= 1 MB) come back in one copy, larger ones
+// move only the populated columns of the neighbour lists; small target lists
+// are read by the kernels from mapped host memory.
 #include "regf_context.h"
 
 #include <math.h>
