@@ -41,6 +41,12 @@ struct abyss_ctx_s {
     int *d_nnb = nullptr, *d_list = nullptr;
     double *h_res = nullptr;  // acc then adot
     int *h_nnb = nullptr, *h_rows = nullptr;
+    // h_out is mapped, coherent pinned memory; for small split launches the
+    // finalize also writes the results there (device view `mirror`), and
+    // mirror_valid tells fetch that no copy is needed
+    char *h_out_dev = nullptr;
+    abyss::FinalMirror mirror{};
+    bool mirror_valid = false;
     int *h_list = nullptr;    // pinned, populated columns of larger results
     size_t h_list_bytes = 0;
 
@@ -82,8 +88,10 @@ struct abyss_ctx_s {
     abyss::Plan plan{};
     bool plan_valid = false;
 
-    // timing / counters (ProfileDevice)
+    // timing / counters (ProfileDevice); events are recorded around the
+    // force kernels only when `timed` (the legacy entry points' context)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
     bool timing_pending = false;
     long long n_send = 0, n_grav = 0, sum_ni = 0;
     double interactions = 0.0, t_grav_ms = 0.0;

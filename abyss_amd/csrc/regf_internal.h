@@ -26,6 +26,15 @@ struct QueryOut {
     int *list;      // [S][nt][cap]
 };
 
+// Optional second copy of the final results, written by the finalize into
+// mapped host memory (device addresses; all null when unused).
+struct FinalMirror {
+    double *acc = nullptr;   // [nt][3]
+    double *adot = nullptr;  // [nt][3]
+    int *nnb = nullptr;      // [nt]
+    int *list = nullptr;     // [nt][cap]
+};
+
 struct Plan {
     int ipl = 1;     // targets per lane
     int splits = 1;  // j-splits (grid.y)
@@ -109,6 +118,6 @@ hipError_t launch_regf(const JState &js, const int *tgt, int nt, const Plan &pla
 // sub-lists in split (= ascending j) order into the final outputs.
 hipError_t launch_finalize(const QueryOut &part, int S, int nt, int cap,
                            double *acc, double *adot, int *nnb, int *list,
-                           hipStream_t st);
+                           const FinalMirror &mirror, hipStream_t st);
 
 }  // namespace abyss

@@ -620,7 +620,8 @@ __device__ __forceinline__ void finalize_body(const QueryOut &part, int S, int n
                                               int q_rel, int q, bool active, int *off,
                                               double *__restrict__ acc,
                                               double *__restrict__ adot,
-                                              int *__restrict__ nnb, int *__restrict__ list)
+                                              int *__restrict__ nnb, int *__restrict__ list,
+                                              const FinalMirror &mir = FinalMirror{})
 {
     const int lane = threadIdx.x & 63;
 
@@ -658,8 +659,14 @@ __device__ __forceinline__ void finalize_body(const QueryOut &part, int S, int n
     __syncthreads();
 
     if (!active) return;
-    if (lane < 3) acc[(size_t)q * 3 + lane] = a[lane];
-    else if (lane < 6) adot[(size_t)q * 3 + lane - 3] = a[lane];
+    const bool mirror = mir.acc != nullptr;
+    if (lane < 3) {
+        acc[(size_t)q * 3 + lane] = a[lane];
+        if (mirror) mir.acc[(size_t)q * 3 + lane] = a[lane];
+    } else if (lane < 6) {
+        adot[(size_t)q * 3 + lane - 3] = a[lane];
+        if (mirror) mir.adot[(size_t)q * 3 + lane - 3] = a[lane];
+    }
     const int total = running;
     const int ncopy = min(total, cap);
     int *dst = list + (size_t)q * cap;
@@ -670,21 +677,27 @@ __device__ __forceinline__ void finalize_body(const QueryOut &part, int S, int n
             const int mid = (lo + hi + 1) >> 1;
             if (off[mid] <= e) lo = mid; else hi = mid - 1;
         }
-        dst[e] = part.list[((size_t)lo * nt_stride + q_rel) * (size_t)cap + (e - off[lo])];
+        const int j = part.list[((size_t)lo * nt_stride + q_rel) * (size_t)cap + (e - off[lo])];
+        dst[e] = j;
+        if (mirror) mir.list[(size_t)q * cap + e] = j;
     }
-    if (lane == 0) nnb[q] = total <= cap ? total : -total;
+    if (lane == 0) {
+        nnb[q] = total <= cap ? total : -total;
+        if (mirror) mir.nnb[q] = total <= cap ? total : -total;
+    }
 }
 
 __global__ __launch_bounds__(256) void finalize_kernel(QueryOut part, int S, int nt, int cap,
                                                        double *__restrict__ acc,
                                                        double *__restrict__ adot,
                                                        int *__restrict__ nnb,
-                                                       int *__restrict__ list)
+                                                       int *__restrict__ list, FinalMirror mir)
 {
     extern __shared__ int s_off_all[];   // [4 waves][S + 1]
     const int w = threadIdx.x >> 6;
     const int q = blockIdx.x * 4 + w;
-    finalize_body(part, S, nt, cap, q, q, q < nt, s_off_all + w * (S + 1), acc, adot, nnb, list);
+    finalize_body(part, S, nt, cap, q, q, q < nt, s_off_all + w * (S + 1), acc, adot, nnb, list,
+                  mir);
 }
 
 // ---- batched ensembles ---------------------------------------------------
@@ -848,12 +861,13 @@ hipError_t launch_regf(const JState &js, const int *tgt, int nt, const Plan &p,
 }
 
 hipError_t launch_finalize(const QueryOut &part, int S, int nt, int cap, double *acc,
-                           double *adot, int *nnb, int *list, hipStream_t st)
+                           double *adot, int *nnb, int *list, const FinalMirror &mirror,
+                           hipStream_t st)
 {
     dim3 grid((nt + 3) / 4);
     const size_t lds = 4 * (size_t)(S + 1) * sizeof(int);
     hipLaunchKernelGGL(finalize_kernel, grid, dim3(256), lds, st, part, S, nt, cap, acc, adot, nnb,
-                       list);
+                       list, mirror);
     return hipGetLastError();
 }
 

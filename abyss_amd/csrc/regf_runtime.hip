@@ -111,6 +111,11 @@ void set_output_views(abyss_ctx_t c, int nt)
     c->h_res = reinterpret_cast<double *>(c->h_out);
     c->h_nnb = reinterpret_cast<int *>(c->h_res + 6 * n);
     c->h_rows = reinterpret_cast<int *>(c->h_out + packed_bytes(n));
+    double *m = reinterpret_cast<double *>(c->h_out_dev);
+    c->mirror.acc = m;
+    c->mirror.adot = m + 3 * n;
+    c->mirror.nnb = reinterpret_cast<int *>(m + 6 * n);
+    c->mirror.list = reinterpret_cast<int *>(c->h_out_dev + packed_bytes(n));
 }
 
 int ensure_outputs(abyss_ctx_t c, int nt)
@@ -125,6 +130,9 @@ int ensure_outputs(abyss_ctx_t c, int nt)
     c->t_capacity = 0;
     c->d_list = nullptr;
     c->h_tgt_dev = c->tgt = nullptr;
+    c->h_out_dev = nullptr;
+    c->mirror = abyss::FinalMirror{};
+    c->mirror_valid = false;
     const size_t rows = cap_t * (size_t)c->cap * sizeof(int);
     HIPCHK(c, hipMalloc(&c->d_tgt, cap_t * sizeof(int)));
     HIPCHK(c, hipMalloc(&c->d_out, packed_bytes(cap_t) + rows));
@@ -134,7 +142,10 @@ int ensure_outputs(abyss_ctx_t c, int nt)
     HIPCHK(c, hipHostGetDevicePointer(&dev, c->h_tgt, 0));
     c->h_tgt_dev = static_cast<const int *>(dev);
     HIPCHK(c, hipHostMalloc(&c->h_out, packed_bytes(cap_t) + std::min(rows, kOneTripListBytes),
-                            hipHostMallocDefault));
+                            hipHostMallocMapped | hipHostMallocCoherent));
+    void *out_dev = nullptr;
+    HIPCHK(c, hipHostGetDevicePointer(&out_dev, c->h_out, 0));
+    c->h_out_dev = static_cast<char *>(out_dev);
     c->t_capacity = cap_t;
     set_output_views(c, nt);
     return ABYSS_OK;
@@ -513,6 +524,7 @@ int abyss_ctx_set_targets(abyss_ctx_t c, int nt, const int *targets)
     }
     c->nt = nt;
     c->counts_valid = false;   // counts belong to the next launch
+    c->mirror_valid = false;
     c->plan = make_plan(c, nt, c->n);
     rc = ensure_scratch(c, c->plan, nt);
     if (rc != ABYSS_OK) return rc;
@@ -552,18 +564,27 @@ int ctx_launch_force(abyss_ctx_t c)
     }
     const Plan &p = c->plan;
     QueryOut fin{c->d_acc, c->d_adot, c->d_nnb, c->d_list};
-    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    // GPU time for ProfileDevice: only the legacy context records events
+    // (two event records and their readback cost a small query ~7 us)
+    c->mirror_valid = false;
+    if (c->timed) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (p.splits == 1) {
         HIPCHK(c, abyss::launch_regf(c->js, c->tgt, c->nt, p, fin, c->cap, c->precision,
                                      c->stream));
     } else {
         HIPCHK(c, abyss::launch_regf(c->js, c->tgt, c->nt, p, c->part, c->cap, c->precision,
                                      c->stream));
+        // small results also go straight into the mapped host block, so a
+        // following fetch needs no copy
+        const bool mirror = c->nt <= kMappedTargets && !c->batch_member &&
+                            (size_t)c->nt * c->cap * sizeof(int) <= kOneTripListBytes;
         HIPCHK(c, abyss::launch_finalize(c->part, p.splits, c->nt, c->cap, c->d_acc, c->d_adot,
-                                         c->d_nnb, c->d_list, c->stream));
+                                         c->d_nnb, c->d_list,
+                                         mirror ? c->mirror : abyss::FinalMirror{}, c->stream));
+        c->mirror_valid = mirror;
     }
-    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    c->timing_pending = true;
+    if (c->timed) HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->timing_pending = c->timed;
     c->n_grav += 1;
     c->counts_valid = true;
     c->sum_ni += c->nt;
@@ -611,11 +632,16 @@ int ctx_fetch_on_stream(abyss_ctx_t c, hipStream_t st, double *acc, double *adot
     // then only the populated list columns.
     const size_t full_bytes = (size_t)nt * cap * sizeof(int);
     const bool one_trip = full_bytes <= kOneTripListBytes;
-    HIPCHK(c, hipMemcpyAsync(c->h_out, c->d_out,
-                             one_trip ? packed_bytes((size_t)nt) + full_bytes
-                                      : (size_t)nt * (6 * sizeof(double) + sizeof(int)),
-                             hipMemcpyDeviceToHost, st));
-    HIPCHK(c, hipStreamSynchronize(st));
+    if (c->mirror_valid && st == c->stream) {
+        // the launch's finalize already wrote every result into h_out
+        HIPCHK(c, hipStreamSynchronize(st));
+    } else {
+        HIPCHK(c, hipMemcpyAsync(c->h_out, c->d_out,
+                                 one_trip ? packed_bytes((size_t)nt) + full_bytes
+                                          : (size_t)nt * (6 * sizeof(double) + sizeof(int)),
+                                 hipMemcpyDeviceToHost, st));
+        HIPCHK(c, hipStreamSynchronize(st));
+    }
     if (st == c->stream) {
         int rc = collect_timing(c);
         if (rc != ABYSS_OK) return rc;
@@ -726,6 +752,7 @@ int legacy_init(int irank)
         return rc;
     }
     g_init_detail[0] = 0;
+    g_ctx->timed = true;
     abyss_ctx_set_precision(g_ctx, g_prec);
     return ABYSS_OK;
 }
