@@ -319,6 +319,7 @@ int abyss_ens_launch(abyss_ens_t e)
     // work item is about kItemPairs pairs whatever N is.  The split depends
     // only on the cluster's own N and targets per block.
     constexpr double kItemPairs = 4194304.0;
+    constexpr int kItemTilesSmall = 16;
     struct Entry {
         int cluster, first, nblocks, splits, jspan, nt_slice;
         double item_cost;
@@ -338,6 +339,12 @@ int abyss_ens_launch(abyss_ens_t e)
         const int tile = abyss::kTile;
         const int nround = ((c->n + tile - 1) / tile) * tile;
         int jspan = (int)std::ceil(kItemPairs / eff / tile) * tile;
+        // 16/64-lane groups do only 16 or 4 steps per tile, too little to
+        // hide the next tile's load: an item's time is ~1-2 us per tile
+        // whatever its pairs.  Cap their items at kItemTilesSmall tiles so a
+        // small block of a large cluster spreads over many short items
+        // (8 targets of a 500K-particle cluster: one 2 ms item before).
+        if (c->plan.jl >= 16) jspan = std::min(jspan, kItemTilesSmall * tile);
         jspan = std::max(tile, std::min(jspan, nround));
         int S = (c->n + jspan - 1) / jspan;
         if (S > abyss::kMaxSplitsBatch) {
