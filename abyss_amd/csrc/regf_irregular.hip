@@ -174,6 +174,10 @@ __global__ __launch_bounds__(256) void irregular_kernel(const JRec *__restrict__
 //   a2 = -6 da_dt2 - 2 adot_dt - 2 j0/dt,  a3 = (12 da_dt2 + 6 adot_dt)/dt
 //   x_new = x + a2 dt^4/24 + a3 dt^5/120,  v_new = v + a2 dt^3/6 + a3 dt^4/24
 // out row (18): x_new, v_new, a2, a3, a_irr, adot_irr.
+// STAGED: both lists of the wave's target are first copied into LDS
+// (dynamic, 2 * cap ints per wave), so the membership binary searches read
+// LDS instead of paying global-memory latency at every step.
+template <bool STAGED>
 __global__ __launch_bounds__(256) void correct_kernel(
     const JRec *__restrict__ jr, const int *__restrict__ tgt, int nt, const int *__restrict__ new_nnb,
     const int *__restrict__ new_list, const double *__restrict__ new_acc,
@@ -181,14 +185,25 @@ __global__ __launch_bounds__(256) void correct_kernel(
     const int *__restrict__ rlist, const double *__restrict__ dts,
     const double *__restrict__ a_old, double *__restrict__ out)
 {
+    extern __shared__ int s_lists[];   // STAGED: [4 waves][new cap | old cap]
     const int lane = threadIdx.x & 63;
     const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (q >= nt) return;   // wave-uniform; no barriers below
-    const int i = tgt[q];
-    const int cn = list_len(new_nnb[q], cap);
-    const int co = list_len(rnnb[i], cap);
+    const bool active = q < nt;
+    const int i = active ? tgt[q] : 0;
+    const int cn = active ? list_len(new_nnb[q], cap) : 0;
+    const int co = active ? list_len(rnnb[i], cap) : 0;
     const int *ln = new_list + (size_t)q * cap;
     const int *lo = rlist + (size_t)i * cap;
+    if constexpr (STAGED) {
+        int *sn = s_lists + (threadIdx.x >> 6) * 2 * cap;
+        int *so = sn + cap;
+        for (int e = lane; e < cn; e += 64) sn[e] = ln[e];
+        for (int e = lane; e < co; e += 64) so[e] = lo[e];
+        __syncthreads();   // every wave of the block reaches it
+        ln = sn;
+        lo = so;
+    }
+    if (!active) return;   // wave-uniform; no barriers below
     const JRec ri = jr[i];
     const double xi[3] = {ri.x, ri.y, ri.z};
     const double vi[3] = {ri.vx, ri.vy, ri.vz};
@@ -435,10 +450,19 @@ int abyss_ctx_regular_step(abyss_ctx_t c, int nt, const int *targets, const doub
     const int *rlist = rnnb + (((size_t)c->n + 63) & ~size_t(63));
     rc = ensure_aos(c);
     if (rc != ABYSS_OK) return rc;
-    hipLaunchKernelGGL(correct_kernel, dim3((nt + 3) / 4), dim3(256), 0, c->stream,
-                       reinterpret_cast<const JRec *>(c->d_aos), c->tgt, nt, c->d_nnb,
-                       c->d_list, c->d_acc, c->d_adot, c->cap, rnnb, rlist, d_in, d_in + nt,
-                       d_out);
+    // lists staged in LDS when both fit (2 * cap ints per wave, 64 KB per
+    // workgroup at most)
+    const size_t lds = 4 * 2 * (size_t)c->cap * sizeof(int);
+    if (lds <= 65536)
+        hipLaunchKernelGGL(correct_kernel<true>, dim3((nt + 3) / 4), dim3(256), lds, c->stream,
+                           reinterpret_cast<const JRec *>(c->d_aos), c->tgt, nt, c->d_nnb,
+                           c->d_list, c->d_acc, c->d_adot, c->cap, rnnb, rlist, d_in, d_in + nt,
+                           d_out);
+    else
+        hipLaunchKernelGGL(correct_kernel<false>, dim3((nt + 3) / 4), dim3(256), 0, c->stream,
+                           reinterpret_cast<const JRec *>(c->d_aos), c->tgt, nt, c->d_nnb,
+                           c->d_list, c->d_acc, c->d_adot, c->cap, rnnb, rlist, d_in, d_in + nt,
+                           d_out);
     if (hipGetLastError() != hipSuccess) return ABYSS_ERR_HIP;
     // the resident lists become this step's (after the correction read them)
     rc = abyss::keep_launch_lists(c);
