@@ -128,7 +128,7 @@ const char *abyss_last_error_detail(void);
 /* Legacy list stride; setter only allowed while the library is closed. */
 int abyss_num_neighbor_max(void);
 int abyss_set_num_neighbor_max(int cap);
-/* Legacy API precision (ABYSS_PREC_SPLIT or ABYSS_PREC_FP64). */
+/* Legacy API precision: any ABYSS_PREC_* mode (default ABYSS_PREC_SPLIT). */
 int abyss_set_precision(int mode);
 /* Number of HIP devices visible (0 without a GPU; never fails). */
 int abyss_device_count(void);
