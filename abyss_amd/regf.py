@@ -466,6 +466,8 @@ class Context:
             self._lib.abyss_ctx_query(self.h, nt, _i(t) if nt else None, _d(acc), _d(adot),
                                       _i(nnb), _i(lst)), allow_overflow=True)
         self.nt = nt
+        if out is not None:
+            return acc, adot, nnb, lst
         return acc[:nt], adot[:nt], nnb[:nt], lst[:nt]
 
     def plan(self):

@@ -93,6 +93,39 @@ def test_plummer_kroupa_65536_blocks(ctx, size):
     err, _ = check(js, t, ctx.query(t), label=f"kroupa65536/{size}")
 
 
+def test_switching_block_sizes_on_one_context(ctx):
+    """Small blocks (<= 64 targets) take the mapped target list and the
+    finalize's host copy; larger ones the device copies.  Alternating sizes
+    on one context, through query, launch + fetch and preallocated outputs,
+    must give every block its own correct results."""
+    say("case: switching block sizes on one context")
+    js = workloads.plummer_kroupa(65536)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    rng = np.random.default_rng(77)
+    first = None
+    for k, size in enumerate([1, 64, 65, 3, 300, 2, 64, 5000, 7, 1]):
+        t = rng.permutation(js.n)[:size].astype(np.int32)
+        if k % 3 == 0:
+            res = ctx.query(t)
+        elif k % 3 == 1:
+            ctx.set_targets(t)
+            ctx.launch()
+            ctx.launch()               # a second launch rewrites the same results
+            res = ctx.fetch()
+        else:
+            out = (np.zeros((size, 3)), np.zeros((size, 3)), np.zeros(size, np.int32),
+                   np.zeros((size, ctx.cap), np.int32))
+            res = ctx.query(t, out=out)
+            assert res[0] is out[0] and res[3] is out[3]
+        check(js, t, res, label=f"switch/{k}/{size}")
+        if first is None:
+            first = (t, [np.copy(a) for a in res])
+    # the first block again: bitwise what it gave the first time
+    again = ctx.query(first[0])
+    for a, b in zip(first[1], again):
+        assert np.array_equal(a, b)
+
+
 def test_fp64_kernel(ctx):
     say("case: fp64 kernel plummer_kroupa N=65536 block=4096")
     js = workloads.plummer_kroupa(65536)
