@@ -325,7 +325,7 @@ const char *abyss_error_string(int code)
         case ABYSS_ERR_OUT_OF_MEMORY: return "out of device memory";
         case ABYSS_ERR_NEIGHBOR_OVERFLOW: return "neighbour list overflow";
         case ABYSS_ERR_ADMISSION: return "admission refused (memory budget)";
-        case ABYSS_ERR_BUSY: return "no free slot";
+        case ABYSS_ERR_BUSY: return "busy: no free slot, or request turned away by admission";
         default: return "unknown error";
     }
 }

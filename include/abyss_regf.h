@@ -69,7 +69,7 @@ extern "C" {
 #define ABYSS_ERR_OUT_OF_MEMORY     8
 #define ABYSS_ERR_NEIGHBOR_OVERFLOW 9  /* results valid; some NumNeighbor < 0 */
 #define ABYSS_ERR_ADMISSION        10  /* served ensemble: HBM budget exhausted */
-#define ABYSS_ERR_BUSY             11  /* served ensemble: no free slot */
+#define ABYSS_ERR_BUSY             11  /* served ensemble: no free slot, or request turned away */
 
 #define ABYSS_PREC_SPLIT   0
 #define ABYSS_PREC_FP64    1
