@@ -104,7 +104,9 @@ __device__ __forceinline__ void run_tile(Targets<IPL, F64> &T, const double2 *__
 #pragma unroll
                 for (int c = 0; c < 6; ++c) pa[k][c] = 0.0f;
             const int fe = FULL ? kFold : min(kFold, count - f);
-#pragma unroll U
+            // partial tiles: a run-time trip count, which the neighbour
+            // branch keeps from being unrolled with a remainder loop
+#pragma unroll(FULL ? U : 1)
             for (int jj = 0; jj < fe; ++jj) {
                 const int r = f + jj;
                 const double2 p0 = pos[3 * r + 0];   // x, y
@@ -353,7 +355,7 @@ __device__ __forceinline__ void jl_tile(JLLane &L, const double2 *__restrict__ p
                                         int count, int jt)
 {
     const int steps = FULL ? kTile / JL : (count + JL - 1) / JL;
-#pragma unroll 2
+#pragma unroll(FULL ? 2 : 1)
     for (int st = 0; st < steps; ++st) {
         const int r = st * JL + L.sub;
         const double2 p0 = pos[3 * r + 0];
