@@ -106,7 +106,8 @@ __device__ __forceinline__ void run_tile(Targets<IPL, F64> &T, const double2 *__
             const int fe = FULL ? kFold : min(kFold, count - f);
             // partial tiles: a run-time trip count, which the neighbour
             // branch keeps from being unrolled with a remainder loop
-#pragma unroll(FULL ? U : 1)
+            constexpr int kUnroll = FULL ? U : 1;
+#pragma unroll kUnroll
             for (int jj = 0; jj < fe; ++jj) {
                 const int r = f + jj;
                 const double2 p0 = pos[3 * r + 0];   // x, y
@@ -355,7 +356,8 @@ __device__ __forceinline__ void jl_tile(JLLane &L, const double2 *__restrict__ p
                                         int count, int jt)
 {
     const int steps = FULL ? kTile / JL : (count + JL - 1) / JL;
-#pragma unroll(FULL ? 2 : 1)
+    constexpr int kUnroll = FULL ? 2 : 1;   // partial tiles: run-time trip count
+#pragma unroll kUnroll
     for (int st = 0; st < steps; ++st) {
         const int r = st * JL + L.sub;
         const double2 p0 = pos[3 * r + 0];
