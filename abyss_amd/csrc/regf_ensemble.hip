@@ -183,8 +183,9 @@ int abyss_ens_create(int device, int cap, int max_clusters, abyss_ens_t *out)
         }
         if (hipDeviceGetAttribute(&e->cu_count, hipDeviceAttributeMultiprocessorCount, device) !=
             hipSuccess) { rc = ABYSS_ERR_HIP; break; }
-        // four 256-thread workgroups per CU: the batch kernel's occupancy
-        e->grid = std::max(1, e->cu_count) * 4;
+        // as many 256-thread workgroups per CU as the batch kernel keeps
+        // resident (5 at its 96 VGPRs): one persistent wave of workgroups
+        e->grid = std::max(1, e->cu_count) * abyss::batch_blocks_per_cu();
         e->fin_grid = std::max(1, e->cu_count) * 8;
         if (hipMalloc(&e->d_cd, (size_t)max_clusters * sizeof(ClusterDesc)) != hipSuccess ||
             hipHostMalloc(&e->h_cd, (size_t)max_clusters * sizeof(ClusterDesc),

@@ -101,7 +101,9 @@ constexpr int kMaxSplitsBatch = 512;   // j-splits of one cluster in a batch
 // counters[0..1] := 0 (a kernel, so that graph replays need no memset node)
 hipError_t launch_batch_reset(int *counters, hipStream_t st);
 // Persistent pairwise kernel over the work queue; exits when the queue index
-// passes hdr->nitems.
+// passes hdr->nitems.  batch_blocks_per_cu(): its resident workgroups per CU
+// (the grid is that many per CU).
+int batch_blocks_per_cu();
 hipError_t launch_batch(const ClusterDesc *cd, const WorkItem *items, const BatchHeader *hdr,
                         int *counter, int cap, int grid, hipStream_t st);
 // Persistent finalize over the targets of split slices: fin_off[k] (k <=

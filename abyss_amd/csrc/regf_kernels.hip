@@ -881,6 +881,16 @@ hipError_t launch_batch_reset(int *counters, hipStream_t st)
     return hipGetLastError();
 }
 
+int batch_blocks_per_cu()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel, kBlock, 0) !=
+            hipSuccess ||
+        nb < 1)
+        nb = 4;
+    return nb;
+}
+
 hipError_t launch_batch(const ClusterDesc *cd, const WorkItem *items, const BatchHeader *hdr,
                         int *counter, int cap, int grid, hipStream_t st)
 {
