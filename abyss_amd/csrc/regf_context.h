@@ -85,6 +85,20 @@ struct abyss_ctx_s {
     char *d_scratch = nullptr;
     abyss::QueryOut part{};
 
+    // checked precision: device block of spread [t][2] | spread partials
+    // [S][t][2] | counters [2] | flagged positions [t] | their j-indices [t];
+    // the recomputation's split partials; pinned readback of the count
+    char *d_chk = nullptr;
+    size_t chk_bytes = 0;
+    float *chk_spread = nullptr, *chk_pspread = nullptr;
+    int *chk_counters = nullptr, *chk_fq = nullptr, *chk_fi = nullptr;
+    char *d_rchk = nullptr;
+    size_t rchk_bytes = 0;
+    int *h_chk = nullptr;
+    double check_tol = 1.0e-6;
+    int n_rechecked = 0;        // targets recomputed by the last launch
+    bool spread_valid = false;  // chk_spread holds the last launch's spread
+
     abyss::Plan plan{};
     bool plan_valid = false;
 

@@ -24,6 +24,9 @@ struct QueryOut {
     double *adot;   // [S][nt][3]
     int *cnt;       // [S][nt]  true neighbour count of the split's j-range
     int *list;      // [S][nt][cap]
+    // [S][nt][2] sums over the FP32 runs of |a run partial|^2 and |jdot run
+    // partial|^2 (checked precision only; null otherwise)
+    float *spread = nullptr;
 };
 
 // Optional second copy of the final results, written by the finalize into
@@ -112,14 +115,35 @@ hipError_t launch_batch_finalize(const ClusterDesc *cd, const int *fin_cl, const
                                  const BatchHeader *hdr, int cap, int grid, hipStream_t st);
 
 // Launch the pairwise kernel for nt targets against js (all j), split into
-// plan.splits contiguous j-ranges.  prec: 0 split precision, 1 FP64.
+// plan.splits contiguous j-ranges.  prec: an ABYSS_PREC_* mode, or
+// kPrecRecheck (the recomputation of checked precision).
 hipError_t launch_regf(const JState &js, const int *tgt, int nt, const Plan &plan,
                        const QueryOut &out, int cap, int prec, hipStream_t st);
 
+// Internal mode of launch_regf: the neighbour test of split precision (FP32
+// r^2 of the rounded differences, so the regular set is the same) with FP64
+// terms and a Newton-refined rsqrt, accumulated in FP64.  Used with cap = 0
+// (no list is written).
+constexpr int kPrecRecheck = 100;
+
 // Sum the S partial results in split order and concatenate the neighbour
-// sub-lists in split (= ascending j) order into the final outputs.
+// sub-lists in split (= ascending j) order into the final outputs; with
+// `spread` non-null also the S spread partials.
 hipError_t launch_finalize(const QueryOut &part, int S, int nt, int cap,
                            double *acc, double *adot, int *nnb, int *list,
-                           const FinalMirror &mirror, hipStream_t st);
+                           const FinalMirror &mirror, float *spread, hipStream_t st);
+
+// Checked precision.  counters[0] must be 0.  Appends to (fq, fi) the
+// position and j-index of every target q whose a or jdot may be off by
+// more than the tolerance: spread[q][0] * k2 > |acc[q]|^2 or spread[q][1] *
+// k2 > |adot[q]|^2.  counters[0] ends as their number (order of the
+// entries unspecified).
+hipError_t launch_check_flag(const double *acc, const double *adot, const float *spread,
+                             const int *tgt, int nt, double k2, int *counters, int *fq, int *fi,
+                             hipStream_t st);
+// Sums the recomputation's S partials of each of the m flagged targets (in
+// split order) and writes them over acc/adot at fq[k].
+hipError_t launch_recheck_finalize(const QueryOut &part, int S, int m, const int *fq,
+                                   double *acc, double *adot, hipStream_t st);
 
 }  // namespace abyss

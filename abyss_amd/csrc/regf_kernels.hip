@@ -308,8 +308,12 @@ __global__ __launch_bounds__(kBlock) void regf_kernel(JState js, const int *__re
 //      neighbour test compares FP64 r^2 with FP64 r_nb^2
 //   2  as 1, with one Newton step y += y (1 - r^2 y^2) / 2 in FP64 on the
 //      rsqrt (relative error ~1e-14)
-// FP64 vector instructions issue at the FP32 rate on MI355X, so modes 1 and
-// 2 trade the six FP64->FP32 conversions for FP64 arithmetic.
+//   3  terms as 2, but the neighbour test is that of mode 0 (FP32 r^2 of
+//      the differences rounded to FP32 against FP32 r_nb^2), so a pair is
+//      regular exactly when mode 0 finds it so; the recomputation of
+//      checked precision
+// FP64 vector instructions issue at the FP32 rate on MI355X, so modes 1 to
+// 3 trade the six FP64->FP32 conversions for FP64 arithmetic.
 template <int TM>
 struct TermMass { using type = double; };
 template <>
@@ -396,11 +400,19 @@ __device__ __forceinline__ void jl_tile(JLLane &L, const double2 *__restrict__ p
             const double dvz = p2.y - L.vzi;
             const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
             float rf = __builtin_amdgcn_rsqf((float)r2);
-            const bool near = !(r2 > L.rnbd);
+            bool near;
+            if constexpr (TM == 3) {
+                // the same expression as mode 0, hence the same bits
+                const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+                const float r2f = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+                near = !(r2f > L.rnbf);
+            } else {
+                near = !(r2 > L.rnbd);
+            }
             const unsigned long long nb = __ballot(near);
             if (nb) jl_near<FULL, SELF>(L, near, nb, r, count, jt, rf);
             double rinv = (double)rf;
-            if constexpr (TM == 2) {
+            if constexpr (TM >= 2) {
                 const double h = fma(-(r2 * rinv), rinv, 1.0);
                 rinv = fma(0.5 * rinv, h, rinv);
             }
@@ -495,7 +507,13 @@ __device__ __forceinline__ void jl_tile_common(JLLane &L, const double2 *__restr
     }
 }
 
-template <int JL, int TM>
+// EST (mode 0 only): also sum, over the lane's FP32 runs, the squared norms
+// of each run's acceleration and jdot partials, and write the target's two
+// totals to out.spread.  The rounding error of each sum scales with the
+// root of its total (the largest terms dominate both), which the checked
+// precision compares with |a| and |jdot|.  The pair arithmetic and the sums
+// are those of EST = false.
+template <int JL, int TM, bool EST = false>
 __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__restrict__ tgt, int nt,
                                              int jspan, const QueryOut &out, int cap,
                                              int final_counts, int bx, int s, int q_base,
@@ -536,6 +554,13 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
         L.pa[c] = 0.0f;
     }
     int since_fold = 0;
+    float spa = 0.0f, spj = 0.0f;   // EST: sums of squared a / jdot run partials
+    auto run_spread = [&]() {
+        if constexpr (EST) {
+            spa = fmaf(L.pa[2], L.pa[2], fmaf(L.pa[1], L.pa[1], fmaf(L.pa[0], L.pa[0], spa)));
+            spj = fmaf(L.pa[5], L.pa[5], fmaf(L.pa[4], L.pa[4], fmaf(L.pa[3], L.pa[3], spj)));
+        }
+    };
 
     const int ntile = (j1 - j0 + kTile - 1) / kTile;
     TileLoader<MT> ld;
@@ -564,6 +589,7 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
             // (every tile for JL = 4, every 4 / 16 tiles for JL = 16 / 64)
             since_fold += (count + JL - 1) / JL;
             if (since_fold >= kFold) {
+                run_spread();
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
                     L.acc[c] += (double)L.pa[c];
@@ -583,6 +609,19 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
     for (int d = JL / 2; d >= 1; d >>= 1)
 #pragma unroll
         for (int c = 0; c < 6; ++c) acc[c] += __shfl_xor(acc[c], d, 64);
+    if constexpr (EST) {
+        run_spread();
+#pragma unroll
+        for (int d = JL / 2; d >= 1; d >>= 1) {
+            spa += __shfl_xor(spa, d, 64);
+            spj += __shfl_xor(spj, d, 64);
+        }
+        if (q < nt && L.sub == 0) {
+            const size_t o = (size_t)s * nt_stride + (q - q_base);
+            out.spread[2 * o + 0] = spa;
+            out.spread[2 * o + 1] = spj;
+        }
+    }
     if (q < nt && L.sub == 0) {
         const size_t o = (size_t)s * nt_stride + (q - q_base);
         out.acc[3 * o + 0] = acc[0];
@@ -597,15 +636,15 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
 
 // 5 waves per SIMD (<= 96 VGPRs); the FP64-term modes would otherwise take
 // 98 registers and drop to 4.
-template <int JL, int TM>
+template <int JL, int TM, bool EST>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
 void regf_kernel_jl(JState js, const int *__restrict__ tgt, int nt, int jspan, QueryOut out,
                     int cap, int final_counts)
 {
     __shared__ double2 s_pos[2][kTile * 3];
     __shared__ typename TermMass<TM>::type s_m[2][kTile];
-    regf_body_jl<JL, TM>(js, tgt, nt, jspan, out, cap, final_counts, blockIdx.x, blockIdx.y, 0,
-                         nt, s_pos, s_m);
+    regf_body_jl<JL, TM, EST>(js, tgt, nt, jspan, out, cap, final_counts, blockIdx.x,
+                              blockIdx.y, 0, nt, s_pos, s_m);
 }
 
 // One wave per target.  Partial sums: lane l adds splits l, l+64, ... in
@@ -620,16 +659,15 @@ void regf_kernel_jl(JState js, const int *__restrict__ tgt, int nt, int jspan, Q
 // prefix array of S + 1 ints.
 // Partials of this target are read at s*nt_stride + q_rel; the final
 // outputs are written at q (absolute target position).
-__device__ __forceinline__ void finalize_body(const QueryOut &part, int S, int nt_stride, int cap,
-                                              int q_rel, int q, bool active, int *off,
-                                              double *__restrict__ acc,
-                                              double *__restrict__ adot,
-                                              int *__restrict__ nnb, int *__restrict__ list,
-                                              const FinalMirror &mir = FinalMirror{})
+// The S split partials of one target summed by one wave: lane l adds splits
+// l, l+64, ... in order, then a fixed xor-butterfly (every lane ends with
+// the total).
+__device__ __forceinline__ void sum_splits(const QueryOut &part, int S, int nt_stride, int q_rel,
+                                           bool active, double (&a)[6])
 {
     const int lane = threadIdx.x & 63;
-
-    double a[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 6; ++c) a[c] = 0.0;
     if (active) {
         for (int s = lane; s < S; s += 64) {
             const size_t o = (size_t)s * nt_stride + q_rel;
@@ -645,6 +683,38 @@ __device__ __forceinline__ void finalize_body(const QueryOut &part, int S, int n
     for (int d = 32; d >= 1; d >>= 1)
 #pragma unroll
         for (int c = 0; c < 6; ++c) a[c] += __shfl_xor(a[c], d, 64);
+}
+
+__device__ __forceinline__ void finalize_body(const QueryOut &part, int S, int nt_stride, int cap,
+                                              int q_rel, int q, bool active, int *off,
+                                              double *__restrict__ acc,
+                                              double *__restrict__ adot,
+                                              int *__restrict__ nnb, int *__restrict__ list,
+                                              const FinalMirror &mir = FinalMirror{},
+                                              float *__restrict__ spread = nullptr)
+{
+    const int lane = threadIdx.x & 63;
+
+    double a[6];
+    sum_splits(part, S, nt_stride, q_rel, active, a);
+    if (spread) {
+        float ea = 0.0f, ej = 0.0f;
+        if (active)
+            for (int s = lane; s < S; s += 64) {
+                const size_t o = (size_t)s * nt_stride + q_rel;
+                ea += part.spread[2 * o + 0];
+                ej += part.spread[2 * o + 1];
+            }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            ea += __shfl_xor(ea, d, 64);
+            ej += __shfl_xor(ej, d, 64);
+        }
+        if (active && lane == 0) {
+            spread[2 * (size_t)q + 0] = ea;
+            spread[2 * (size_t)q + 1] = ej;
+        }
+    }
 
     int running = 0;
     for (int base = 0; base < S; base += 64) {
@@ -695,13 +765,70 @@ __global__ __launch_bounds__(256) void finalize_kernel(QueryOut part, int S, int
                                                        double *__restrict__ acc,
                                                        double *__restrict__ adot,
                                                        int *__restrict__ nnb,
-                                                       int *__restrict__ list, FinalMirror mir)
+                                                       int *__restrict__ list, FinalMirror mir,
+                                                       float *__restrict__ spread)
 {
     extern __shared__ int s_off_all[];   // [4 waves][S + 1]
     const int w = threadIdx.x >> 6;
     const int q = blockIdx.x * 4 + w;
     finalize_body(part, S, nt, cap, q, q, q < nt, s_off_all + w * (S + 1), acc, adot, nnb, list,
-                  mir);
+                  mir, spread);
+}
+
+// ---- checked precision ---------------------------------------------------
+
+__device__ __forceinline__ double norm2(const double *__restrict__ v, size_t q)
+{
+    const double x = v[3 * q + 0], y = v[3 * q + 1], z = v[3 * q + 2];
+    return x * x + y * y + z * z;
+}
+
+// One thread per target.  A wave appends its flagged targets with one
+// atomic add (of its flag count) and their ranks within the wave.
+__global__ __launch_bounds__(256) void check_flag_kernel(const double *__restrict__ acc,
+                                                         const double *__restrict__ adot,
+                                                         const float *__restrict__ spread,
+                                                         const int *__restrict__ tgt, int nt,
+                                                         double k2, int *__restrict__ counter,
+                                                         int *__restrict__ fq,
+                                                         int *__restrict__ fi)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    bool flag = false;
+    if (q < nt) {
+        flag = (double)spread[2 * (size_t)q + 0] * k2 > norm2(acc, q) ||
+               (double)spread[2 * (size_t)q + 1] * k2 > norm2(adot, q);
+    }
+    const unsigned long long b = __ballot(flag);
+    if (b == 0) return;   // the whole wave
+    const int first = __ffsll((long long)b) - 1;
+    int base = 0;
+    if (lane == first) base = atomicAdd(counter, __popcll(b));
+    base = __shfl(base, first, 64);
+    if (flag) {
+        const int k = base + __popcll(b & ((1ull << lane) - 1ull));
+        fq[k] = q;
+        fi[k] = tgt[q];
+    }
+}
+
+// One wave per flagged target k < m.
+__global__ __launch_bounds__(256) void recheck_finalize_kernel(QueryOut part, int S, int m,
+                                                               const int *__restrict__ fq,
+                                                               double *__restrict__ acc,
+                                                               double *__restrict__ adot)
+{
+    const int lane = threadIdx.x & 63;
+    const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (k >= m) return;   // the whole wave
+    double a[6];
+    sum_splits(part, S, m, k, true, a);
+    const size_t q = (size_t)fq[k];
+    if (lane < 3)
+        acc[q * 3 + lane] = a[lane];
+    else if (lane < 6)
+        adot[q * 3 + lane - 3] = a[lane];
 }
 
 // ---- batched ensembles ---------------------------------------------------
@@ -797,21 +924,28 @@ hipError_t launch_t(const JState &js, const int *tgt, int nt, const Plan &p, con
     return hipGetLastError();
 }
 
+// tm: term mode; est: the spread estimate of checked precision (mode 0)
 template <int JL>
 hipError_t launch_jl(const JState &js, const int *tgt, int nt, const Plan &p, const QueryOut &out,
-                     int cap, int tm, hipStream_t st)
+                     int cap, int tm, bool est, hipStream_t st)
 {
     dim3 grid(p.grid_x, p.splits);
     const int fc = p.splits == 1 ? 1 : 0;
-    if (tm == 0)
-        hipLaunchKernelGGL((regf_kernel_jl<JL, 0>), grid, dim3(kBlock), 0, st, js, tgt, nt,
+    if (tm == 0 && est)
+        hipLaunchKernelGGL((regf_kernel_jl<JL, 0, true>), grid, dim3(kBlock), 0, st, js, tgt, nt,
                            p.jspan, out, cap, fc);
+    else if (tm == 0)
+        hipLaunchKernelGGL((regf_kernel_jl<JL, 0, false>), grid, dim3(kBlock), 0, st, js, tgt,
+                           nt, p.jspan, out, cap, fc);
     else if (tm == 1)
-        hipLaunchKernelGGL((regf_kernel_jl<JL, 1>), grid, dim3(kBlock), 0, st, js, tgt, nt,
-                           p.jspan, out, cap, fc);
+        hipLaunchKernelGGL((regf_kernel_jl<JL, 1, false>), grid, dim3(kBlock), 0, st, js, tgt,
+                           nt, p.jspan, out, cap, fc);
+    else if (tm == 2)
+        hipLaunchKernelGGL((regf_kernel_jl<JL, 2, false>), grid, dim3(kBlock), 0, st, js, tgt,
+                           nt, p.jspan, out, cap, fc);
     else
-        hipLaunchKernelGGL((regf_kernel_jl<JL, 2>), grid, dim3(kBlock), 0, st, js, tgt, nt,
-                           p.jspan, out, cap, fc);
+        hipLaunchKernelGGL((regf_kernel_jl<JL, 3, false>), grid, dim3(kBlock), 0, st, js, tgt,
+                           nt, p.jspan, out, cap, fc);
     return hipGetLastError();
 }
 
@@ -821,6 +955,7 @@ int term_mode(int prec)
     switch (prec) {
         case ABYSS_PREC_MIXED: return 1;
         case ABYSS_PREC_REFINED: return 2;
+        case kPrecRecheck: return 3;
         default: return 0;
     }
 }
@@ -829,7 +964,8 @@ int term_mode(int prec)
 
 bool variant_supported(int prec, int ipl, int unroll, int jl)
 {
-    if (prec == ABYSS_PREC_MIXED || prec == ABYSS_PREC_REFINED)
+    if (prec == ABYSS_PREC_MIXED || prec == ABYSS_PREC_REFINED || prec == ABYSS_PREC_CHECKED ||
+        prec == kPrecRecheck)
         return jl == 4 || jl == 16 || jl == 64;
     if (jl != 1) return prec == ABYSS_PREC_SPLIT && (jl == 4 || jl == 16 || jl == 64);
     if (prec == ABYSS_PREC_FP64) return (ipl == 1 || ipl == 2) && unroll == 2;
@@ -846,9 +982,11 @@ hipError_t launch_regf(const JState &js, const int *tgt, int nt, const Plan &p,
 {
     if (!variant_supported(prec, p.ipl, p.unroll, p.jl)) return hipErrorInvalidValue;
     const int tm = term_mode(prec);
-    if (p.jl == 4) return launch_jl<4>(js, tgt, nt, p, out, cap, tm, st);
-    if (p.jl == 16) return launch_jl<16>(js, tgt, nt, p, out, cap, tm, st);
-    if (p.jl == 64) return launch_jl<64>(js, tgt, nt, p, out, cap, tm, st);
+    const bool est = prec == ABYSS_PREC_CHECKED;
+    if (est && !out.spread) return hipErrorInvalidValue;
+    if (p.jl == 4) return launch_jl<4>(js, tgt, nt, p, out, cap, tm, est, st);
+    if (p.jl == 16) return launch_jl<16>(js, tgt, nt, p, out, cap, tm, est, st);
+    if (p.jl == 64) return launch_jl<64>(js, tgt, nt, p, out, cap, tm, est, st);
     if (prec == ABYSS_PREC_FP64) {
         if (p.ipl == 1) return launch_t<1, 2, true>(js, tgt, nt, p, out, cap, st);
         return launch_t<2, 2, true>(js, tgt, nt, p, out, cap, st);
@@ -866,12 +1004,30 @@ hipError_t launch_regf(const JState &js, const int *tgt, int nt, const Plan &p,
 
 hipError_t launch_finalize(const QueryOut &part, int S, int nt, int cap, double *acc,
                            double *adot, int *nnb, int *list, const FinalMirror &mirror,
-                           hipStream_t st)
+                           float *spread, hipStream_t st)
 {
+    if (spread && !part.spread) return hipErrorInvalidValue;
     dim3 grid((nt + 3) / 4);
     const size_t lds = 4 * (size_t)(S + 1) * sizeof(int);
     hipLaunchKernelGGL(finalize_kernel, grid, dim3(256), lds, st, part, S, nt, cap, acc, adot, nnb,
-                       list, mirror);
+                       list, mirror, spread);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_flag(const double *acc, const double *adot, const float *spread,
+                             const int *tgt, int nt, double k2, int *counters, int *fq, int *fi,
+                             hipStream_t st)
+{
+    hipLaunchKernelGGL(check_flag_kernel, dim3((nt + 255) / 256), dim3(256), 0, st, acc, adot,
+                       spread, tgt, nt, k2, counters, fq, fi);
+    return hipGetLastError();
+}
+
+hipError_t launch_recheck_finalize(const QueryOut &part, int S, int m, const int *fq,
+                                   double *acc, double *adot, hipStream_t st)
+{
+    hipLaunchKernelGGL(recheck_finalize_kernel, dim3((m + 3) / 4), dim3(256), 0, st, part, S, m,
+                       fq, acc, adot);
     return hipGetLastError();
 }
 

@@ -151,7 +151,11 @@ int ensure_outputs(abyss_ctx_t c, int nt)
     return ABYSS_OK;
 }
 
-Plan make_plan(abyss_ctx_t c, int nt, int n)
+// Plan for nt targets against n j under precision `prec`; list rows of
+// `cap` ints bound the split count through the scratch limit.  `forced`:
+// honour the context's forced tuning and the tuned table (the context's own
+// queries), or not (the recomputation of checked precision).
+Plan plan_for(abyss_ctx_t c, int nt, int n, int prec, int cap, bool forced)
 {
     Plan p;
     p.tile = abyss::kTile;
@@ -162,11 +166,15 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
     // peak at N = m = 262144); smaller blocks spread a target over 16 lanes
     // (16..255 targets) or 64 (below 16), measured best at N = 262144.
     abyss::TuneEntry te{0, 0, 16};
-    const bool tuned = c->precision == ABYSS_PREC_SPLIT && abyss::tuned_plan(c->device, nt, &te);
-    int jl = c->force_jl;
+    const bool tuned = forced &&
+                       (prec == ABYSS_PREC_SPLIT || prec == ABYSS_PREC_CHECKED) &&
+                       abyss::tuned_plan(c->device, nt, &te);
+    const int force_jl = forced ? c->force_jl : 0;
+    const int force_splits = forced ? c->force_splits : 0;
+    int jl = force_jl;
     if (jl == 0) {
         if (tuned) jl = te.jl;
-        else if (c->precision == ABYSS_PREC_FP64) jl = 1;
+        else if (prec == ABYSS_PREC_FP64) jl = 1;
         else if (nt >= 256) jl = 4;
         else if (nt >= 16) jl = 16;
         else jl = 64;
@@ -177,10 +185,10 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
     // workgroups per CU, and from 16 to 64 below one per CU (Plummer +
     // Kroupa N = 65536: m = 256 53.3 -> 35.7 us, m = 16 26.0 -> 19.9 us;
     // King N = 262144 keeps its choices).
-    if (jl > 1 && c->force_jl == 0 && !tuned && c->force_splits == 0 && !c->batch_member &&
-        c->precision != ABYSS_PREC_FP64) {
+    if (jl > 1 && force_jl == 0 && !tuned && force_splits == 0 && !c->batch_member &&
+        prec != ABYSS_PREC_FP64) {
         int s_max = std::min(std::max(1, n / (4 * abyss::kTile)), abyss::kMaxSplits);
-        const double per = (double)nt * ((double)c->cap * 4.0 + 6 * 8.0 + 4.0);
+        const double per = (double)nt * ((double)cap * 4.0 + 6 * 8.0 + 4.0);
         s_max = std::min(s_max, (int)std::max(1.0, std::floor((double)c->scratch_limit / per)));
         auto groups = [&](int l) {
             const int tpb = abyss::kBlock / l;
@@ -189,17 +197,17 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
         if (jl == 4 && groups(4) < 4.0 * cu) jl = 16;
         if (jl == 16 && groups(16) < 1.0 * cu) jl = 64;
     }
-    if (c->precision == ABYSS_PREC_FP64) jl = 1;
+    if (prec == ABYSS_PREC_FP64) jl = 1;
     // the FP64-term modes exist only in the lanes-per-target kernel
-    if ((c->precision == ABYSS_PREC_MIXED || c->precision == ABYSS_PREC_REFINED ||
-         c->batch_member) && jl == 1)
+    if ((prec == ABYSS_PREC_MIXED || prec == ABYSS_PREC_REFINED ||
+         prec == ABYSS_PREC_CHECKED || c->batch_member) && jl == 1)
         jl = 4;
     // One target per lane (ipl = 1, unroll 2) measured fastest at every
     // block size on MI355X; ipl 2/4 keep more targets in registers but cut
     // occupancy to 3 and 1 waves per SIMD.
-    int ipl = c->force_ipl ? c->force_ipl : 1;
-    int unroll = c->force_unroll ? c->force_unroll : (tuned ? te.unroll : 2);
-    if (c->precision == ABYSS_PREC_FP64) {
+    int ipl = forced && c->force_ipl ? c->force_ipl : 1;
+    int unroll = forced && c->force_unroll ? c->force_unroll : (tuned ? te.unroll : 2);
+    if (prec == ABYSS_PREC_FP64) {
         ipl = std::min(ipl, 2);
         unroll = 2;
     }
@@ -209,7 +217,7 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
     p.unroll = unroll;
     const int tpb = abyss::targets_per_block(p);
     p.grid_x = (nt + tpb - 1) / tpb;
-    int S = c->force_splits;
+    int S = force_splits;
     if (S == 0) {
         // Workgroups per CU over the whole launch (or the tuned count), which
         // sets the tail of the last round of workgroups: for the 1- and
@@ -226,7 +234,7 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
         S = std::min(S, std::min(max_by_n, abyss::kMaxSplits));
     }
     if (S > 1) {
-        const double per = (double)nt * ((double)c->cap * 4.0 + 6 * 8.0 + 4.0);
+        const double per = (double)nt * ((double)cap * 4.0 + 6 * 8.0 + 4.0);
         const int max_by_mem = (int)std::max(1.0, std::floor((double)c->scratch_limit / per));
         S = std::min(S, max_by_mem);
     }
@@ -236,6 +244,11 @@ Plan make_plan(abyss_ctx_t c, int nt, int n)
     p.jspan = jspan;
     p.splits = (n + jspan - 1) / jspan;
     return p;
+}
+
+Plan make_plan(abyss_ctx_t c, int nt, int n)
+{
+    return plan_for(c, nt, n, c->precision, c->cap, true);
 }
 
 int ensure_scratch(abyss_ctx_t c, const Plan &p, int nt)
@@ -267,6 +280,78 @@ int ensure_scratch(abyss_ctx_t c, const Plan &p, int nt)
     return ABYSS_OK;
 }
 
+// Checked precision: the spread arrays, counters and flagged-target arrays
+// for nt targets and S splits.
+int ensure_check(abyss_ctx_t c, int nt, int S)
+{
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t t = (size_t)nt;
+    const size_t b_sp = al(2 * t * sizeof(float));
+    const size_t b_ps = al(2 * (size_t)S * t * sizeof(float));
+    const size_t b_ix = al(t * sizeof(int));
+    const size_t need = b_sp + b_ps + 256 + 2 * b_ix;
+    if (need > c->chk_bytes) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        dfree(c->d_chk);
+        c->chk_bytes = 0;
+        HIPCHK(c, hipMalloc(&c->d_chk, grow(need)));
+        c->chk_bytes = grow(need);
+    }
+    if (!c->h_chk) HIPCHK(c, hipHostMalloc(&c->h_chk, 256, hipHostMallocDefault));
+    char *b = c->d_chk;
+    c->chk_spread = reinterpret_cast<float *>(b);
+    b += b_sp;
+    c->chk_pspread = reinterpret_cast<float *>(b);
+    b += b_ps;
+    c->chk_counters = reinterpret_cast<int *>(b);
+    b += 256;
+    c->chk_fq = reinterpret_cast<int *>(b);
+    b += b_ix;
+    c->chk_fi = reinterpret_cast<int *>(b);
+    return ABYSS_OK;
+}
+
+// After a checked launch's force and finalize: flags the targets whose
+// jerk may miss the tolerance, reads their count back (a host wait), and
+// recomputes them with FP64 terms over the same regular set.
+int recheck(abyss_ctx_t c)
+{
+    const double ratio = ABYSS_CHECK_ERR_SCALE / c->check_tol;
+    HIPCHK(c, abyss::launch_batch_reset(c->chk_counters, c->stream));
+    HIPCHK(c, abyss::launch_check_flag(c->d_acc, c->d_adot, c->chk_spread, c->tgt, c->nt,
+                                       ratio * ratio, c->chk_counters, c->chk_fq, c->chk_fi,
+                                       c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_chk, c->chk_counters, sizeof(int), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int m = c->h_chk[0];
+    c->n_rechecked = m;
+    if (m <= 0) return ABYSS_OK;
+    if (m > c->nt) return set_detail(c, ABYSS_ERR_HIP, "check: %d flagged of %d", m, c->nt);
+    const Plan rp = plan_for(c, m, c->n, ABYSS_PREC_REFINED, 0, false);
+    const size_t S = (size_t)rp.splits;
+    const size_t b_acc = (S * m * 3 * sizeof(double) + 255) & ~size_t(255);
+    const size_t b_cnt = (S * m * sizeof(int) + 255) & ~size_t(255);
+    const size_t need = 2 * b_acc + b_cnt;
+    if (need > c->rchk_bytes) {
+        // the stream is idle (synchronised above)
+        dfree(c->d_rchk);
+        c->rchk_bytes = 0;
+        HIPCHK(c, hipMalloc(&c->d_rchk, grow(need)));
+        c->rchk_bytes = grow(need);
+    }
+    QueryOut rpart{};
+    rpart.acc = reinterpret_cast<double *>(c->d_rchk);
+    rpart.adot = reinterpret_cast<double *>(c->d_rchk + b_acc);
+    rpart.cnt = reinterpret_cast<int *>(c->d_rchk + 2 * b_acc);
+    rpart.list = nullptr;   // cap 0: nothing is listed
+    HIPCHK(c, abyss::launch_regf(c->js, c->chk_fi, m, rp, rpart, 0, abyss::kPrecRecheck,
+                                 c->stream));
+    HIPCHK(c, abyss::launch_recheck_finalize(rpart, rp.splits, m, c->chk_fq, c->d_acc, c->d_adot,
+                                             c->stream));
+    return ABYSS_OK;
+}
+
 int collect_timing(abyss_ctx_t c)
 {
     if (!c->timing_pending) return ABYSS_OK;
@@ -286,6 +371,7 @@ void release_all(abyss_ctx_t c)
     c->d_list = nullptr;
     c->h_tgt_dev = c->tgt = nullptr;
     dfree(c->d_scratch); dfree(c->d_state); dfree(c->d_upd);
+    dfree(c->d_chk); dfree(c->d_rchk); hfree(c->h_chk);
     dfree(c->d_rlists); dfree(c->d_irr); dfree(c->d_aos); dfree(c->d_drv); dfree(c->d_rad);
     hfree(c->h_drv);
     hfree(c->h_state); hfree(c->h_upd); hfree(c->h_irr);
@@ -397,7 +483,7 @@ int abyss_ctx_destroy(abyss_ctx_t c)
 
 int abyss_ctx_set_precision(abyss_ctx_t c, int mode)
 {
-    if (!c || mode < ABYSS_PREC_SPLIT || mode > ABYSS_PREC_REFINED) return ABYSS_ERR_INVALID_ARG;
+    if (!c || mode < ABYSS_PREC_SPLIT || mode > ABYSS_PREC_CHECKED) return ABYSS_ERR_INVALID_ARG;
     c->precision = mode;
     c->plan_valid = false;
     return ABYSS_OK;
@@ -432,6 +518,35 @@ int abyss_ctx_set_scratch_limit(abyss_ctx_t c, unsigned long long bytes)
 }
 
 const char *abyss_ctx_error_detail(abyss_ctx_t c) { return c ? c->detail : ""; }
+
+int abyss_ctx_set_check_tolerance(abyss_ctx_t c, double tol)
+{
+    if (!c) return ABYSS_ERR_INVALID_ARG;
+    if (!(tol > 0.0) || !isfinite(tol))
+        return set_detail(c, ABYSS_ERR_INVALID_ARG, "check tolerance must be finite and > 0");
+    c->check_tol = tol;
+    return ABYSS_OK;
+}
+
+int abyss_ctx_last_recheck(abyss_ctx_t c, int *count)
+{
+    if (!c || !count) return ABYSS_ERR_INVALID_ARG;
+    *count = c->n_rechecked;
+    return ABYSS_OK;
+}
+
+int abyss_ctx_fetch_spread(abyss_ctx_t c, float *spread)
+{
+    if (!c || !spread) return ABYSS_ERR_INVALID_ARG;
+    if (!c->spread_valid)
+        return set_detail(c, ABYSS_ERR_INVALID_ARG, "no checked-precision launch to read");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpyAsync(spread, c->chk_spread, 2 * (size_t)c->nt * sizeof(float),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 2 * c->nt; ++k) spread[k] = sqrtf(spread[k]);
+    return ABYSS_OK;
+}
 
 void *abyss_ctx_stream(abyss_ctx_t c) { return c ? (void *)c->stream : nullptr; }
 
@@ -525,6 +640,7 @@ int abyss_ctx_set_targets(abyss_ctx_t c, int nt, const int *targets)
     c->nt = nt;
     c->counts_valid = false;   // counts belong to the next launch
     c->mirror_valid = false;
+    c->spread_valid = false;
     c->plan = make_plan(c, nt, c->n);
     rc = ensure_scratch(c, c->plan, nt);
     if (rc != ABYSS_OK) return rc;
@@ -563,7 +679,19 @@ int ctx_launch_force(abyss_ctx_t c)
         c->plan_valid = true;
     }
     const Plan &p = c->plan;
+    const bool checked = c->precision == ABYSS_PREC_CHECKED;
+    c->spread_valid = false;
+    c->n_rechecked = 0;
+    if (checked) {
+        int rc = ensure_check(c, c->nt, p.splits);
+        if (rc != ABYSS_OK) return rc;
+    }
     QueryOut fin{c->d_acc, c->d_adot, c->d_nnb, c->d_list};
+    QueryOut part = c->part;
+    if (checked) {
+        fin.spread = c->chk_spread;
+        part.spread = c->chk_pspread;
+    }
     // GPU time for ProfileDevice: only the legacy context records events
     // (two event records and their readback cost a small query ~7 us)
     c->mirror_valid = false;
@@ -572,16 +700,23 @@ int ctx_launch_force(abyss_ctx_t c)
         HIPCHK(c, abyss::launch_regf(c->js, c->tgt, c->nt, p, fin, c->cap, c->precision,
                                      c->stream));
     } else {
-        HIPCHK(c, abyss::launch_regf(c->js, c->tgt, c->nt, p, c->part, c->cap, c->precision,
+        HIPCHK(c, abyss::launch_regf(c->js, c->tgt, c->nt, p, part, c->cap, c->precision,
                                      c->stream));
         // small results also go straight into the mapped host block, so a
-        // following fetch needs no copy
-        const bool mirror = c->nt <= kMappedTargets && !c->batch_member &&
+        // following fetch needs no copy (not under checked precision, whose
+        // recomputation rewrites results after the finalize)
+        const bool mirror = c->nt <= kMappedTargets && !c->batch_member && !checked &&
                             (size_t)c->nt * c->cap * sizeof(int) <= kOneTripListBytes;
-        HIPCHK(c, abyss::launch_finalize(c->part, p.splits, c->nt, c->cap, c->d_acc, c->d_adot,
+        HIPCHK(c, abyss::launch_finalize(part, p.splits, c->nt, c->cap, c->d_acc, c->d_adot,
                                          c->d_nnb, c->d_list,
-                                         mirror ? c->mirror : abyss::FinalMirror{}, c->stream));
+                                         mirror ? c->mirror : abyss::FinalMirror{},
+                                         fin.spread, c->stream));
         c->mirror_valid = mirror;
+    }
+    if (checked) {
+        int rc = recheck(c);
+        if (rc != ABYSS_OK) return rc;
+        c->spread_valid = true;
     }
     if (c->timed) HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->timing_pending = c->timed;
@@ -774,7 +909,7 @@ int abyss_set_num_neighbor_max(int cap)
 
 int abyss_set_precision(int mode)
 {
-    if (mode < ABYSS_PREC_SPLIT || mode > ABYSS_PREC_REFINED) return ABYSS_ERR_INVALID_ARG;
+    if (mode < ABYSS_PREC_SPLIT || mode > ABYSS_PREC_CHECKED) return ABYSS_ERR_INVALID_ARG;
     g_prec = mode;
     if (g_ctx) abyss_ctx_set_precision(g_ctx, mode);
     return ABYSS_OK;

@@ -35,6 +35,10 @@ PREC_SPLIT = 0
 PREC_FP64 = 1
 PREC_MIXED = 2
 PREC_REFINED = 3
+PREC_CHECKED = 4
+# ABYSS_CHECK_ERR_SCALE: the checked precision recomputes a target when
+# CHECK_ERR_SCALE * spread > tol * |a| (or |adot|) for either spread
+CHECK_ERR_SCALE = 1.0e-6
 
 # Every exported symbol of include/abyss_regf.h (checked by the CPU tests).
 EXPORTS = [
@@ -57,7 +61,8 @@ EXPORTS = [
     "abyss_ctx_upload_state", "abyss_ctx_update_state", "abyss_ctx_predict",
     "abyss_ctx_keep_lists", "abyss_ctx_irregular", "abyss_ctx_regular_step",
     "abyss_ctx_derivs23", "abyss_ctx_update_radius", "abyss_srv_set_admission",
-    "abyss_srv_queue_stats",
+    "abyss_srv_queue_stats", "abyss_ctx_set_check_tolerance", "abyss_ctx_last_recheck",
+    "abyss_ctx_fetch_spread",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -110,6 +115,9 @@ def load():
         "abyss_ctx_stream": (_vp, [_vp]),
         "abyss_ctx_plan": (ctypes.c_int, [_vp, _ip, _ip, _ip, _ip, _ip, _ip]),
         "abyss_ctx_error_detail": (ctypes.c_char_p, [_vp]),
+        "abyss_ctx_set_check_tolerance": (ctypes.c_int, [_vp, ctypes.c_double]),
+        "abyss_ctx_last_recheck": (ctypes.c_int, [_vp, _ip]),
+        "abyss_ctx_fetch_spread": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
         "abyss_ens_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.POINTER(_vp)]),
         "abyss_ens_destroy": (ctypes.c_int, [_vp]),
@@ -340,6 +348,24 @@ class Context:
 
     def set_precision(self, mode):
         self._chk(self._lib.abyss_ctx_set_precision(self.h, int(mode)))
+
+    def set_check_tolerance(self, tol):
+        """Relative jdot tolerance of the checked precision (default 1e-6)."""
+        self._chk(self._lib.abyss_ctx_set_check_tolerance(self.h, float(tol)))
+
+    def last_recheck(self):
+        """Targets the last checked-precision launch recomputed."""
+        n = ctypes.c_int(0)
+        self._chk(self._lib.abyss_ctx_last_recheck(self.h, ctypes.byref(n)))
+        return n.value
+
+    def spread(self):
+        """Per target of the last checked-precision launch: the root sums of
+        squares of its FP32 run partials of a and of jdot (float32 [nt, 2])."""
+        out = np.empty((max(self.nt, 1), 2), dtype=np.float32)
+        self._chk(self._lib.abyss_ctx_fetch_spread(
+            self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out[:self.nt]
 
     def set_tuning(self, i_per_lane=0, j_splits=0, j_unroll=0, lanes_per_target=0):
         self._chk(self._lib.abyss_ctx_set_tuning(self.h, int(i_per_lane), int(j_splits),

@@ -33,7 +33,12 @@
  * test and the terms in FP64 as well (sums in FP64); only rsqrt is the FP32
  * hardware estimate (~1e-7 relative).  ABYSS_PREC_REFINED adds one FP64
  * Newton step to that estimate (~1e-14).  ABYSS_PREC_FP64 evaluates
- * everything in FP64 with a correctly rounded rsqrt.
+ * everything in FP64 with a correctly rounded rsqrt.  ABYSS_PREC_CHECKED is
+ * ABYSS_PREC_SPLIT plus a per-target check: targets whose jerk sum cancels
+ * so far that its FP32 rounding may exceed the check tolerance (default
+ * 1e-6 relative) are recomputed with FP64 terms (as ABYSS_PREC_REFINED) over
+ * the same regular set; every other result is bitwise that of
+ * ABYSS_PREC_SPLIT, and the lists always are.
  *
  * Threading: the legacy entry points share one process-wide default context
  * and are not re-entrant (as in the reference, whose state is file-static,
@@ -75,6 +80,7 @@ extern "C" {
 #define ABYSS_PREC_FP64    1
 #define ABYSS_PREC_MIXED   2
 #define ABYSS_PREC_REFINED 3
+#define ABYSS_PREC_CHECKED 4
 
 /* ===================================================================== */
 /* Part 1: legacy entry points (src_mpi/cuda/cuda_functions.h:3-10,      */
@@ -171,6 +177,23 @@ int abyss_ctx_plan(abyss_ctx_t ctx, int *i_per_lane, int *j_splits,
                    int *grid_x, int *tile, int *j_unroll, int *lanes_per_target);
 /* Detail string of the context's last error. */
 const char *abyss_ctx_error_detail(abyss_ctx_t ctx);
+
+/* Checked precision (ABYSS_PREC_CHECKED).  A launch sums, for each target,
+ * the squared norms of its FP32 run partials of a and of jdot (Sa, Sj; the
+ * root of each is the scale of that sum's rounding error, since the largest
+ * terms dominate both) and recomputes with FP64 terms every target with
+ *   ABYSS_CHECK_ERR_SCALE * sqrt(Sa) > tol * |a|  or
+ *   ABYSS_CHECK_ERR_SCALE * sqrt(Sj) > tol * |adot|,
+ * before the results are fetched.  No counterpart in the reference (whose
+ * FP32 build has no accuracy control).
+ *   set_check_tolerance: tol (> 0; default 1e-6);
+ *   last_recheck: targets recomputed by the last launch;
+ *   fetch_spread: [nt][2] = sqrt(Sa), sqrt(Sj) per target of the last
+ *     launch. */
+#define ABYSS_CHECK_ERR_SCALE 1.0e-6
+int abyss_ctx_set_check_tolerance(abyss_ctx_t ctx, double tol);
+int abyss_ctx_last_recheck(abyss_ctx_t ctx, int *count);
+int abyss_ctx_fetch_spread(abyss_ctx_t ctx, float *spread);
 
 /* Device-resident particle state: j prediction on the device instead of
  * the caller's O(N) predict + pack + copy before every regular step

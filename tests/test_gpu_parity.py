@@ -155,6 +155,100 @@ def test_other_precision_modes(ctx, prec, size):
         ctx.set_precision(regf.PREC_SPLIT)
 
 
+def _same_set_rows(nnb, lst, rn, rl):
+    """Rows whose neighbour set equals the reference's (a pair in the 1e-6
+    band may fall on the other side in FP64; such a row sums a different
+    regular set and is compared through its list instead)."""
+    w = lst.shape[1]
+    col = np.arange(w)[None, :]
+    valid = col < np.minimum(np.abs(nnb), w)[:, None]
+    return (nnb == rn) & np.all((lst == rl[:, :w]) | ~valid, axis=1)
+
+
+CHECK_TOL = 1e-6
+
+
+@pytest.mark.parametrize("wl,size", [("plummer1024", 1024), ("kroupa", 1), ("kroupa", 100),
+                                     ("kroupa", 4096), ("kroupa", 65536)])
+def test_checked_precision(ctx, wl, size):
+    """Checked precision: every result within 1e-6 relative of the FP64
+    oracle (a and jdot, per particle, maximum), lists and counts those of
+    split precision, unflagged targets bitwise split precision, recomputed
+    targets FP64-accurate, and the recomputed set the one the flag rule
+    names."""
+    say(f"case: checked precision {wl} block={size}")
+    js = workloads.plummer_equal(1024) if wl == "plummer1024" else workloads.plummer_kroupa(65536)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    t = np.random.default_rng(size + 29).permutation(js.n)[:size].astype(np.int32)
+    split = ctx.query(t)
+    ctx.set_precision(regf.PREC_CHECKED)
+    try:
+        chk = ctx.query(t)
+        nre = ctx.last_recheck()
+        spr = ctx.spread().astype(np.float64)
+    finally:
+        ctx.set_precision(regf.PREC_SPLIT)
+    assert np.array_equal(chk[2], split[2]) and np.array_equal(chk[3], split[3])
+    changed = np.any(chk[0] != split[0], axis=1) | np.any(chk[1] != split[1], axis=1)
+    assert changed.sum() <= nre <= size
+    # the flag rule, evaluated on the host from the split results (a 1 %
+    # margin each way for the FP32 spread's rounding)
+    na = np.linalg.norm(split[0], axis=1)
+    nj = np.linalg.norm(split[1], axis=1)
+    score = np.maximum(spr[:, 0] / np.where(na > 0, na, 1e-300),
+                       spr[:, 1] / np.where(nj > 0, nj, 1e-300)) * regf.CHECK_ERR_SCALE / CHECK_TOL
+    assert np.all(changed[score > 1.01]) and not np.any(changed[score < 0.99])
+    ra, rj, rn, rl = oracle.regf(js.x, js.v, js.m, js.r2, t, js.cap)
+    rows = _same_set_rows(chk[2], chk[3], rn, rl)
+    err = force_errors(chk[0][rows], chk[1][rows], ra[rows], rj[rows])
+    sp_err = force_errors(split[0][rows], split[1][rows], ra[rows], rj[rows])
+    say(f"[checked {wl}/{size}] recomputed={nre} band_rows={(~rows).sum()} "
+        f"a_rel={err['a_rel']} jdot_rel={err['jdot_rel']} (split: jdot_rel={sp_err['jdot_rel']})")
+    assert err["a_rel"]["max"] <= CHECK_TOL and err["jdot_rel"]["max"] <= CHECK_TOL
+    if changed[rows].any():
+        rerr = force_errors(chk[0][rows & changed], chk[1][rows & changed], ra[rows & changed],
+                            rj[rows & changed])
+        assert rerr["a_rel"]["max"] <= 1e-11 and rerr["jdot_rel"]["max"] <= 1e-11
+
+
+def test_checked_precision_tolerance_and_legacy(ctx):
+    """A tighter tolerance recomputes more targets; the legacy entry points
+    give the context's checked results bit for bit."""
+    say("case: checked precision tolerance + legacy")
+    js = workloads.plummer_equal(1024)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    t = np.arange(js.n, dtype=np.int32)
+    ctx.set_precision(regf.PREC_CHECKED)
+    try:
+        ref = ctx.query(t)
+        n_default = ctx.last_recheck()
+        ctx.set_check_tolerance(1e-7)
+        ctx.query(t)
+        n_tight = ctx.last_recheck()
+        ctx.set_check_tolerance(1.0)
+        loose = ctx.query(t)
+        assert ctx.last_recheck() == 0
+        with pytest.raises(regf.AbyssError):
+            ctx.set_check_tolerance(0.0)
+        ctx.set_check_tolerance(CHECK_TOL)
+    finally:
+        ctx.set_precision(regf.PREC_SPLIT)
+    split = ctx.query(t)
+    say(f"[checked tolerance] recomputed default={n_default} tight={n_tight}")
+    assert n_tight > n_default > 0
+    assert np.array_equal(loose[0], split[0]) and np.array_equal(loose[1], split[1])
+    regf.OpenDevice(0)
+    try:
+        regf.set_precision(regf.PREC_CHECKED)
+        regf.SendToDevice(js.m, js.x, js.v, js.r2)
+        acc, adot, nnb, lst = regf.CalculateAccelerationOnDevice(t)
+    finally:
+        regf.set_precision(regf.PREC_SPLIT)
+        regf.CloseDevice()
+    assert np.array_equal(acc, ref[0]) and np.array_equal(adot, ref[1])
+    assert np.array_equal(nnb, ref[2])
+
+
 def test_tuning_variants_agree(ctx):
     """Every kernel variant classifies every pair identically (same FP32 r^2),
     so lists are equal; one-target-per-lane variants share FP32 partial groups
@@ -330,6 +424,30 @@ def test_king_262144_full_properties(ctx):
     sample = np.random.default_rng(4).permutation(js.n)[:1024].astype(np.int32)
     sub = tuple(a[sample] for a in split)
     check(js, sample, sub, label="king262144/sample")
+
+
+def test_king_262144_checked_every_target(ctx):
+    """Checked precision on the full King workload: every target within 1e-6
+    relative (a and jdot) of the FP64 kernel, over the rows whose regular
+    set the FP64 test agrees with (the rest differ by a band pair)."""
+    say("case: king_w6_binaries N=262144 checked, every target")
+    js = workloads.king_binaries(262144)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    t = np.arange(js.n, dtype=np.int32)
+    ctx.set_precision(regf.PREC_CHECKED)
+    try:
+        chk = ctx.query(t)
+        nre = ctx.last_recheck()
+        ctx.set_precision(regf.PREC_FP64)
+        f64 = ctx.query(t)
+    finally:
+        ctx.set_precision(regf.PREC_SPLIT)
+    rows = _same_set_rows(chk[2], chk[3], f64[2], f64[3])
+    err = force_errors(chk[0][rows], chk[1][rows], f64[0][rows], f64[1][rows])
+    say(f"[king262144 checked vs fp64] recomputed={nre} band_rows={(~rows).sum()} "
+        f"a_rel={err['a_rel']} jdot_rel={err['jdot_rel']}")
+    assert (~rows).sum() <= 16
+    assert err["a_rel"]["max"] <= CHECK_TOL and err["jdot_rel"]["max"] <= CHECK_TOL
 
 
 @pytest.mark.parametrize("name", ["plummer_n64_seed1.npz", "plummer_n1024_seed1.npz"])
