@@ -11,7 +11,12 @@ Prints ONE JSON line.  Progress goes to stderr.
   roofline  = FP32 VALU roof: 60 flop per interaction (the reference's
               convention) / kernel time vs 157.3 TFLOP/s
   cpu_baseline = the oracle's FP64 C loop (OpenMP over targets) on a bounded
-              sample of the same workload on this host's cores
+              sample of the same workload on this host's cores; its results on
+              256 of those targets also check the GPU results
+  checked_precision = the same step under the checked precision (targets whose
+              a or jdot rounding may exceed 1e-6 recomputed with FP64 terms)
+  accuracy  = per-particle errors over every target against the FP64 kernel
+              (split and checked precision), plus the oracle sample
 """
 import argparse
 import ctypes
@@ -101,8 +106,11 @@ def profiled_traffic(name="regf_kernel_king262144_fullN_traffic.md"):
     return None
 
 
-def cpu_baseline(js, target_seconds=10.0):
-    """Time the oracle's FP64 loop (all host cores) on a bounded target sample."""
+def cpu_baseline(js, target_seconds=10.0, gpu_results=None, nsample=256):
+    """Time the oracle's FP64 loop (all host cores) on a bounded target sample.
+    With gpu_results ({name: (acc, adot, nnb, lists)} over all targets), the
+    oracle's results on the first `nsample` targets of that sample also check
+    each GPU result set (returned as the second value)."""
     import numpy as np
     import oracle
     threads = oracle.max_threads()
@@ -124,26 +132,70 @@ def cpu_baseline(js, target_seconds=10.0):
     oracle.regf(js.x, js.v, js.m, js.r2, sample, js.cap, nthreads=threads)
     dt = time.perf_counter() - t0
     inter = float(count) * (js.n - 1)
-    return {"value": inter / dt, "unit": "interactions/s", "cores": threads, "kind": "port",
+    info = {"value": inter / dt, "unit": "interactions/s", "cores": threads, "kind": "port",
             "sample": f"{count} random targets of the same N={js.n} j-set "
                       f"({inter:.3e} interactions, {dt:.1f} s, FP64 C loop, OpenMP {threads} threads)"}
+    checks = None
+    if gpu_results:
+        from oracle.compare import compare_lists, force_errors
+        t = sample[:nsample]
+        ra, rj, rn, rl, sa, sj = oracle.regf(js.x, js.v, js.m, js.r2, t, js.cap, scales=True)
+        checks = {}
+        for name, (acc, adot, nnb, lst) in gpu_results.items():
+            err = force_errors(acc[t], adot[t], ra, rj, sa, sj)
+            lists = compare_lists(nnb[t], lst[t], rn, rl, js.x, js.r2, t)
+            checks[name] = {
+                "sample_targets": int(len(t)), "a_rel_max": err["a_rel"]["max"],
+                "a_rel_p99": err["a_rel"]["p99"], "jdot_rel_max": err["jdot_rel"]["max"],
+                "jdot_rel_p99": err["jdot_rel"]["p99"], "a_norm_max": err["a_norm"]["max"],
+                "jdot_norm_max": err["jdot_norm"]["max"],
+                "list_hard_mismatches": lists["hard_mismatches"],
+                "list_band_pairs_flipped": lists["band_pairs_flipped"]}
+    return info, checks
 
 
-def accuracy_sample(js, res, nsample=256):
+def full_n_errors(res, ref):
+    """Per-particle relative errors of a and jdot over every target against the
+    FP64 kernel's results, on the rows whose neighbour set (hence regular set)
+    is the same; the other rows differ by a pair within the 1e-6 band."""
     import numpy as np
-    import oracle
-    from oracle.compare import compare_lists, force_errors
     acc, adot, nnb, lst = res
-    t = np.random.default_rng(11).permutation(js.n)[:nsample].astype(np.int32)
-    ra, rj, rn, rl, sa, sj = oracle.regf(js.x, js.v, js.m, js.r2, t, js.cap, scales=True)
-    err = force_errors(acc[t], adot[t], ra, rj, sa, sj)
-    lists = compare_lists(nnb[t], lst[t], rn, rl, js.x, js.r2, t)
-    return {"sample_targets": nsample, "a_rel_max": err["a_rel"]["max"],
-            "a_rel_p99": err["a_rel"]["p99"], "jdot_rel_max": err["jdot_rel"]["max"],
-            "jdot_rel_p99": err["jdot_rel"]["p99"], "a_norm_max": err["a_norm"]["max"],
-            "jdot_norm_max": err["jdot_norm"]["max"],
-            "list_hard_mismatches": lists["hard_mismatches"],
-            "list_band_pairs_flipped": lists["band_pairs_flipped"]}
+    racc, radot, rnnb, rlst = ref
+    w = lst.shape[1]
+    valid = np.arange(w)[None, :] < np.minimum(np.abs(nnb), w)[:, None]
+    rows = (nnb == rnnb) & np.all((lst == rlst) | ~valid, axis=1)
+    ea = np.linalg.norm(acc - racc, axis=1)[rows] / np.linalg.norm(racc, axis=1)[rows]
+    ej = np.linalg.norm(adot - radot, axis=1)[rows] / np.linalg.norm(radot, axis=1)[rows]
+    return {"targets": int(rows.sum()), "band_rows": int((~rows).sum()),
+            "a_rel_max": float(ea.max()), "a_rel_p99": float(np.percentile(ea, 99)),
+            "jdot_rel_max": float(ej.max()), "jdot_rel_p99": float(np.percentile(ej, 99)),
+            "jdot_over_1e-6": int((ej > 1e-6).sum()), "a_over_1e-6": int((ea > 1e-6).sum())}
+
+
+def checked_leg(ctx, targets, inter, reps=5):
+    """Checked precision on the same context and targets: wall time per
+    synchronous launch (median of reps) and the recomputed target count."""
+    import numpy as np
+    from abyss_amd import regf
+    ctx.set_precision(regf.PREC_CHECKED)
+    ctx.set_targets(targets)
+    ctx.launch()
+    ctx.sync()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ctx.launch()
+        ctx.sync()
+        ts.append(time.perf_counter() - t0)
+    ms = float(np.median(ts)) * 1e3
+    info = {"ms_per_step": ms, "value": inter / (ms * 1e-3),
+            "pct_fp32_peak": 100.0 * FLOP_PER_INTERACTION * inter / (ms * 1e-3) / 1e12
+                             / FP32_PEAK_TFLOPS,
+            "recomputed_targets": ctx.last_recheck(), "tolerance": 1e-6,
+            "timing": f"wall clock per launch + sync, median of {reps}"}
+    res = ctx.fetch()
+    ctx.set_precision(regf.PREC_SPLIT)
+    return info, res
 
 
 def ensemble_workload(count=256, lo=1024, hi=65536, seed=1, k_nb=64):
@@ -614,17 +666,27 @@ def main():
     log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step (wall {wall:.3f} s), "
         f"{value:.4e} interactions/s, {achieved:.2f} TFLOP/s")
 
-    acc_info = None
+    acc_info = chk_info = None
+    gpu_results = {}
     if not args.no_check:
-        res = ctx.fetch()
-        acc_info = accuracy_sample(js, res)
+        gpu_results[args.precision] = ctx.fetch()
+        if args.precision == "split":
+            chk_info, gpu_results["checked"] = checked_leg(ctx, targets, inter)
+            log(f"checked precision {chk_info}")
+        ctx.set_precision(regf.PREC_FP64)
+        f64 = ctx.query(targets)
+        acc_info = {"full_n_vs_fp64_kernel": {k: full_n_errors(r, f64)
+                                              for k, r in gpu_results.items()}}
         log(f"accuracy {acc_info}")
     ctx.close()
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(js, args.cpu_seconds)
+        cpu, checks = cpu_baseline(js, args.cpu_seconds, gpu_results)
         log(f"cpu baseline {cpu}")
+        if checks:
+            acc_info["sample_vs_oracle"] = checks
+            log(f"oracle sample {checks}")
     ens_info = serve_info = None
     tuned = False
     if not args.no_ensemble:
@@ -667,6 +729,7 @@ def main():
                      "interactions_per_launch": inter},
         "cpu_baseline": cpu,
         "accuracy": acc_info,
+        "checked_precision": chk_info,
         "batched_ensemble": ens_info,
         "served_ensemble": serve_info,
     }
