@@ -1,6 +1,7 @@
 """C ABI checks that need no GPU: the library loads, exports every function
 declared in include/abyss_regf.h, and validates before touching the device."""
 import os
+import ctypes
 import re
 import subprocess
 
@@ -62,6 +63,20 @@ def test_calls_without_device_fail_cleanly():
     with pytest.raises(regf.AbyssError) as e:
         regf.Context()
     assert e.value.code == regf.ERR_NO_DEVICE
+
+
+def test_precision_setters_validate_without_a_context():
+    lib = regf.load()
+    assert lib.abyss_set_precision(regf.PREC_CHECKED) == regf.OK
+    assert lib.abyss_set_precision(regf.PREC_CHECKED + 1) == regf.ERR_INVALID_ARG
+    assert lib.abyss_set_precision(-1) == regf.ERR_INVALID_ARG
+    assert lib.abyss_set_precision(regf.PREC_SPLIT) == regf.OK
+    assert lib.abyss_ctx_set_precision(None, regf.PREC_CHECKED) == regf.ERR_INVALID_ARG
+    assert lib.abyss_ctx_set_check_tolerance(None, 1e-6) == regf.ERR_INVALID_ARG
+    n = ctypes.c_int(7)
+    assert lib.abyss_ctx_last_recheck(None, ctypes.byref(n)) == regf.ERR_INVALID_ARG
+    buf = (ctypes.c_float * 2)()
+    assert lib.abyss_ctx_fetch_spread(None, buf) == regf.ERR_INVALID_ARG
 
 
 def test_neighbor_stride_setter_validates():
