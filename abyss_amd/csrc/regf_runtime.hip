@@ -311,8 +311,8 @@ int ensure_check(abyss_ctx_t c, int nt, int S)
     return ABYSS_OK;
 }
 
-// After a checked launch's force and finalize: flags the targets whose
-// jerk may miss the tolerance, reads their count back (a host wait), and
+// After a checked launch's force and finalize: flags the targets whose a or
+// jdot may miss the tolerance, reads their count back (a host wait), and
 // recomputes them with FP64 terms over the same regular set.
 int recheck(abyss_ctx_t c)
 {
