@@ -224,6 +224,29 @@ def test_regular_step_matches_oracle_kroupa_65536(ctx, size):
         assert gained > 0 and lost > 0
 
 
+def test_regular_step_checked_precision(ctx):
+    """The fused step under checked precision: its regular force is the
+    checked query's, bit for bit, and the correction matches the oracle."""
+    say("case: regular_step checked precision plummer_kroupa N=65536 block=4096")
+    js = workloads.plummer_kroupa(65536)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    rng = np.random.default_rng(4127)
+    t = rng.permutation(js.n)[:4096].astype(np.int32)
+    r2_new = js.r2 * np.where(rng.uniform(size=js.n) < 0.5, 1.4, 0.7)
+    dt = rng.uniform(1e-3, 1e-2, t.size)
+    a_old = rng.normal(size=(t.size, 6))
+    ctx.set_precision(regf.PREC_CHECKED)
+    try:
+        _step_case(ctx, js, t, r2_new, dt, a_old, "step/checked")
+        acc, adot, nn, _, _ = ctx.regular_step(t, dt, a_old)
+        n_step = ctx.last_recheck()
+        qa, qj, qn, _ = ctx.query(t)
+        assert ctx.last_recheck() == n_step > 0
+    finally:
+        ctx.set_precision(regf.PREC_SPLIT)
+    assert np.array_equal(acc, qa) and np.array_equal(adot, qj) and np.array_equal(nn, qn)
+
+
 def test_regular_step_first_step_and_overflow():
     say("case: regular_step without old lists, and with overflowed lists (cap=16)")
     js = workloads.plummer_equal(1024)
