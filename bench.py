@@ -135,6 +135,14 @@ def cpu_baseline(js, target_seconds=10.0, gpu_results=None, nsample=256):
     info = {"value": inter / dt, "unit": "interactions/s", "cores": threads, "kind": "port",
             "sample": f"{count} random targets of the same N={js.n} j-set "
                       f"({inter:.3e} interactions, {dt:.1f} s, FP64 C loop, OpenMP {threads} threads)"}
+    # one core on a smaller sample (the reference's CPU loop is serial per
+    # particle), sized to ~target_seconds / 2
+    n1 = int(min(js.n, max(16, rate / threads * target_seconds / 2)))
+    t0 = time.perf_counter()
+    oracle.regf(js.x, js.v, js.m, js.r2, order[:n1], js.cap, nthreads=1)
+    dt1 = time.perf_counter() - t0
+    info["single_core"] = {"value": float(n1) * (js.n - 1) / dt1, "cores": 1,
+                           "sample": f"{n1} targets, {dt1:.1f} s"}
     checks = None
     if gpu_results:
         from oracle.compare import compare_lists, force_errors
