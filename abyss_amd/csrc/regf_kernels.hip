@@ -399,14 +399,18 @@ __device__ __forceinline__ void jl_tile(JLLane &L, const double2 *__restrict__ p
             const double dvy = p2.x - L.vyi;
             const double dvz = p2.y - L.vzi;
             const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-            float rf = __builtin_amdgcn_rsqf((float)r2);
+            float rf;
             bool near;
             if constexpr (TM == 3) {
-                // the same expression as mode 0, hence the same bits
+                // the same expression as mode 0, hence the same bits; it also
+                // seeds the rsqrt (within 3.1e-7 of the FP64 r^2, which the
+                // Newton step below then corrects to ~1e-13)
                 const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
                 const float r2f = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+                rf = __builtin_amdgcn_rsqf(r2f);
                 near = !(r2f > L.rnbf);
             } else {
+                rf = __builtin_amdgcn_rsqf((float)r2);
                 near = !(r2 > L.rnbd);
             }
             const unsigned long long nb = __ballot(near);
