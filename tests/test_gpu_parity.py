@@ -315,6 +315,34 @@ def test_overflow(ctx):
         small.close()
 
 
+def test_checked_precision_overflowed_lists():
+    """Overflowed lists (cap 16, ~48 neighbours each): the recomputation
+    still sums exactly the regular set (its test does not read the lists),
+    so every result is within 1e-6 of the oracle and the truncated lists
+    and negative counts are those of split precision."""
+    say("case: checked precision, overflowed lists")
+    js = workloads.plummer_equal(1024)
+    small = regf.Context(device=0, cap=16)
+    try:
+        small.upload(js.m, js.x, js.v, js.r2)
+        t = np.arange(js.n, dtype=np.int32)
+        split = small.query(t)
+        small.set_precision(regf.PREC_CHECKED)
+        chk = small.query(t)
+        assert small.last_status == regf.ERR_NEIGHBOR_OVERFLOW
+        nre = small.last_recheck()
+    finally:
+        small.close()
+    assert nre > 0
+    assert np.array_equal(chk[2], split[2]) and np.all(chk[2] < 0)
+    assert np.array_equal(chk[3], split[3])
+    ra, rj, rn, rl = oracle.regf(js.x, js.v, js.m, js.r2, t, 16)
+    assert np.array_equal(chk[2], rn)
+    err = force_errors(chk[0], chk[1], ra, rj)
+    say(f"[checked overflow] recomputed={nre} a_rel={err['a_rel']} jdot_rel={err['jdot_rel']}")
+    assert err["a_rel"]["max"] <= CHECK_TOL and err["jdot_rel"]["max"] <= CHECK_TOL
+
+
 def test_edge_cases(ctx):
     say("case: edge cases")
     # single particle: zero force, no neighbours
