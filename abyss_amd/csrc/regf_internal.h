@@ -145,5 +145,14 @@ hipError_t launch_check_flag(const double *acc, const double *adot, const float 
 // split order) and writes them over acc/adot at fq[k].
 hipError_t launch_recheck_finalize(const QueryOut &part, int S, int m, const int *fq,
                                    double *acc, double *adot, hipStream_t st);
+// The same two steps with the flagged count read on the device (*count, as
+// left by launch_check_flag) and no host wait: plan p is for all nt targets
+// (partials [S][nt]); workgroups and waves past the count exit at once.
+// Writes the count to *m_report (host memory) and the results also to the
+// mirror when it is set.
+hipError_t launch_recheck_dev(const JState &js, const int *fi, const int *count, int nt,
+                              const Plan &p, const QueryOut &out, const int *fq, double *acc,
+                              double *adot, const FinalMirror &mirror, int *m_report,
+                              hipStream_t st);
 
 }  // namespace abyss

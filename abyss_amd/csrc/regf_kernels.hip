@@ -817,22 +817,50 @@ __global__ __launch_bounds__(256) void check_flag_kernel(const double *__restric
     }
 }
 
-// One wave per flagged target k < m.
-__global__ __launch_bounds__(256) void recheck_finalize_kernel(QueryOut part, int S, int m,
+// The recomputation with the flagged count read on the device (small
+// blocks: no host wait).  The grid covers every target; a workgroup whose
+// target block starts at or past the count exits before its first barrier.
+// Partials are indexed s * nt_stride + k.
+template <int JL>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+void recheck_kernel_dev(JState js, const int *__restrict__ tgt, const int *__restrict__ count,
+                        int nt_stride, int jspan, QueryOut out)
+{
+    __shared__ double2 s_pos[2][kTile * 3];
+    __shared__ double s_m[2][kTile];
+    const int m = *count;
+    if ((int)blockIdx.x * (4 * (64 / JL)) >= m) return;   // the whole workgroup
+    regf_body_jl<JL, 3>(js, tgt, m, jspan, out, 0, 0, blockIdx.x, blockIdx.y, 0, nt_stride, s_pos,
+                        s_m);
+}
+
+// One wave per flagged target k < m (m = *m_dev when m_dev is set; then
+// thread 0 also writes m to *m_report, host memory the caller reads after
+// the stream has drained).  The mirror, when set, receives the same values.
+__global__ __launch_bounds__(256) void recheck_finalize_kernel(QueryOut part, int S, int nt_stride,
+                                                               int m_host,
+                                                               const int *__restrict__ m_dev,
+                                                               int *__restrict__ m_report,
                                                                const int *__restrict__ fq,
                                                                double *__restrict__ acc,
-                                                               double *__restrict__ adot)
+                                                               double *__restrict__ adot,
+                                                               FinalMirror mir)
 {
     const int lane = threadIdx.x & 63;
+    const int m = m_dev ? *m_dev : m_host;
+    if (m_report && blockIdx.x == 0 && threadIdx.x == 0) *m_report = m;
     const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (k >= m) return;   // the whole wave
     double a[6];
-    sum_splits(part, S, m, k, true, a);
+    sum_splits(part, S, nt_stride, k, true, a);
     const size_t q = (size_t)fq[k];
-    if (lane < 3)
+    if (lane < 3) {
         acc[q * 3 + lane] = a[lane];
-    else if (lane < 6)
+        if (mir.acc) mir.acc[q * 3 + lane] = a[lane];
+    } else if (lane < 6) {
         adot[q * 3 + lane - 3] = a[lane];
+        if (mir.adot) mir.adot[q * 3 + lane - 3] = a[lane];
+    }
 }
 
 // ---- batched ensembles ---------------------------------------------------
@@ -1031,7 +1059,30 @@ hipError_t launch_recheck_finalize(const QueryOut &part, int S, int m, const int
                                    double *acc, double *adot, hipStream_t st)
 {
     hipLaunchKernelGGL(recheck_finalize_kernel, dim3((m + 3) / 4), dim3(256), 0, st, part, S, m,
-                       fq, acc, adot);
+                       m, (const int *)nullptr, (int *)nullptr, fq, acc, adot, FinalMirror{});
+    return hipGetLastError();
+}
+
+hipError_t launch_recheck_dev(const JState &js, const int *fi, const int *count, int nt,
+                              const Plan &p, const QueryOut &out, const int *fq, double *acc,
+                              double *adot, const FinalMirror &mirror, int *m_report,
+                              hipStream_t st)
+{
+    if (p.jl != 4 && p.jl != 16 && p.jl != 64) return hipErrorInvalidValue;
+    dim3 grid(p.grid_x, p.splits);
+    if (p.jl == 4)
+        hipLaunchKernelGGL(recheck_kernel_dev<4>, grid, dim3(kBlock), 0, st, js, fi, count, nt,
+                           p.jspan, out);
+    else if (p.jl == 16)
+        hipLaunchKernelGGL(recheck_kernel_dev<16>, grid, dim3(kBlock), 0, st, js, fi, count, nt,
+                           p.jspan, out);
+    else
+        hipLaunchKernelGGL(recheck_kernel_dev<64>, grid, dim3(kBlock), 0, st, js, fi, count, nt,
+                           p.jspan, out);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(recheck_finalize_kernel, dim3((nt + 3) / 4), dim3(256), 0, st, out,
+                       p.splits, nt, 0, count, m_report, fq, acc, adot, mirror);
     return hipGetLastError();
 }
 

@@ -297,7 +297,12 @@ int ensure_check(abyss_ctx_t c, int nt, int S)
         HIPCHK(c, hipMalloc(&c->d_chk, grow(need)));
         c->chk_bytes = grow(need);
     }
-    if (!c->h_chk) HIPCHK(c, hipHostMalloc(&c->h_chk, 256, hipHostMallocDefault));
+    if (!c->h_chk) {
+        HIPCHK(c, hipHostMalloc(&c->h_chk, 256, hipHostMallocMapped | hipHostMallocCoherent));
+        void *dev = nullptr;
+        HIPCHK(c, hipHostGetDevicePointer(&dev, c->h_chk, 0));
+        c->h_chk_dev = static_cast<int *>(dev);
+    }
     char *b = c->d_chk;
     c->chk_spread = reinterpret_cast<float *>(b);
     b += b_sp;
@@ -311,16 +316,55 @@ int ensure_check(abyss_ctx_t c, int nt, int S)
     return ABYSS_OK;
 }
 
+// Device buffer of the recomputation's split partials for `rows` (split,
+// target) rows.
+int ensure_recheck_scratch(abyss_ctx_t c, size_t rows, QueryOut *rpart)
+{
+    const size_t b_acc = (rows * 3 * sizeof(double) + 255) & ~size_t(255);
+    const size_t b_cnt = (rows * sizeof(int) + 255) & ~size_t(255);
+    const size_t need = 2 * b_acc + b_cnt;
+    if (need > c->rchk_bytes) {
+        // an earlier launch's recomputation may still read it
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        dfree(c->d_rchk);
+        c->rchk_bytes = 0;
+        HIPCHK(c, hipMalloc(&c->d_rchk, grow(need)));
+        c->rchk_bytes = grow(need);
+    }
+    *rpart = QueryOut{};
+    rpart->acc = reinterpret_cast<double *>(c->d_rchk);
+    rpart->adot = reinterpret_cast<double *>(c->d_rchk + b_acc);
+    rpart->cnt = reinterpret_cast<int *>(c->d_rchk + 2 * b_acc);
+    rpart->list = nullptr;   // cap 0: nothing is listed
+    return ABYSS_OK;
+}
+
 // After a checked launch's force and finalize: flags the targets whose a or
-// jdot may miss the tolerance, reads their count back (a host wait), and
-// recomputes them with FP64 terms over the same regular set.
-int recheck(abyss_ctx_t c)
+// jdot may miss the tolerance and recomputes them with FP64 terms over the
+// same regular set.  Blocks of up to kMappedTargets targets plan the
+// recomputation for all of them and read the flagged count on the device
+// (no host wait; the count reaches h_chk[1]); larger blocks read the count
+// back first (a host wait) and plan for it.  `mirror`: the finalize wrote
+// the mapped host copy, which the recomputed targets must update too.
+int recheck(abyss_ctx_t c, bool mirror)
 {
     const double ratio = ABYSS_CHECK_ERR_SCALE / c->check_tol;
     HIPCHK(c, abyss::launch_batch_reset(c->chk_counters, c->stream));
     HIPCHK(c, abyss::launch_check_flag(c->d_acc, c->d_adot, c->chk_spread, c->tgt, c->nt,
                                        ratio * ratio, c->chk_counters, c->chk_fq, c->chk_fi,
                                        c->stream));
+    if (c->nt <= kMappedTargets) {
+        const Plan rp = plan_for(c, c->nt, c->n, ABYSS_PREC_REFINED, 0, false);
+        QueryOut rpart;
+        int rc = ensure_recheck_scratch(c, (size_t)rp.splits * c->nt, &rpart);
+        if (rc != ABYSS_OK) return rc;
+        HIPCHK(c, abyss::launch_recheck_dev(c->js, c->chk_fi, c->chk_counters, c->nt, rp, rpart,
+                                            c->chk_fq, c->d_acc, c->d_adot,
+                                            mirror ? c->mirror : abyss::FinalMirror{},
+                                            c->h_chk_dev + 1, c->stream));
+        c->recheck_pending = true;
+        return ABYSS_OK;
+    }
     HIPCHK(c, hipMemcpyAsync(c->h_chk, c->chk_counters, sizeof(int), hipMemcpyDeviceToHost,
                              c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -329,22 +373,9 @@ int recheck(abyss_ctx_t c)
     if (m <= 0) return ABYSS_OK;
     if (m > c->nt) return set_detail(c, ABYSS_ERR_HIP, "check: %d flagged of %d", m, c->nt);
     const Plan rp = plan_for(c, m, c->n, ABYSS_PREC_REFINED, 0, false);
-    const size_t S = (size_t)rp.splits;
-    const size_t b_acc = (S * m * 3 * sizeof(double) + 255) & ~size_t(255);
-    const size_t b_cnt = (S * m * sizeof(int) + 255) & ~size_t(255);
-    const size_t need = 2 * b_acc + b_cnt;
-    if (need > c->rchk_bytes) {
-        // the stream is idle (synchronised above)
-        dfree(c->d_rchk);
-        c->rchk_bytes = 0;
-        HIPCHK(c, hipMalloc(&c->d_rchk, grow(need)));
-        c->rchk_bytes = grow(need);
-    }
-    QueryOut rpart{};
-    rpart.acc = reinterpret_cast<double *>(c->d_rchk);
-    rpart.adot = reinterpret_cast<double *>(c->d_rchk + b_acc);
-    rpart.cnt = reinterpret_cast<int *>(c->d_rchk + 2 * b_acc);
-    rpart.list = nullptr;   // cap 0: nothing is listed
+    QueryOut rpart;
+    int rc = ensure_recheck_scratch(c, (size_t)rp.splits * m, &rpart);
+    if (rc != ABYSS_OK) return rc;
     HIPCHK(c, abyss::launch_regf(c->js, c->chk_fi, m, rp, rpart, 0, abyss::kPrecRecheck,
                                  c->stream));
     HIPCHK(c, abyss::launch_recheck_finalize(rpart, rp.splits, m, c->chk_fq, c->d_acc, c->d_adot,
@@ -531,6 +562,13 @@ int abyss_ctx_set_check_tolerance(abyss_ctx_t c, double tol)
 int abyss_ctx_last_recheck(abyss_ctx_t c, int *count)
 {
     if (!c || !count) return ABYSS_ERR_INVALID_ARG;
+    if (c->recheck_pending) {
+        // the count of a small block arrives with the launch's last kernel
+        HIPCHK(c, hipSetDevice(c->device));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->n_rechecked = c->h_chk[1];
+        c->recheck_pending = false;
+    }
     *count = c->n_rechecked;
     return ABYSS_OK;
 }
@@ -682,6 +720,7 @@ int ctx_launch_force(abyss_ctx_t c)
     const bool checked = c->precision == ABYSS_PREC_CHECKED;
     c->spread_valid = false;
     c->n_rechecked = 0;
+    c->recheck_pending = false;
     if (checked) {
         int rc = ensure_check(c, c->nt, p.splits);
         if (rc != ABYSS_OK) return rc;
@@ -703,9 +742,9 @@ int ctx_launch_force(abyss_ctx_t c)
         HIPCHK(c, abyss::launch_regf(c->js, c->tgt, c->nt, p, part, c->cap, c->precision,
                                      c->stream));
         // small results also go straight into the mapped host block, so a
-        // following fetch needs no copy (not under checked precision, whose
-        // recomputation rewrites results after the finalize)
-        const bool mirror = c->nt <= kMappedTargets && !c->batch_member && !checked &&
+        // following fetch needs no copy (a checked recomputation updates it
+        // too)
+        const bool mirror = c->nt <= kMappedTargets && !c->batch_member &&
                             (size_t)c->nt * c->cap * sizeof(int) <= kOneTripListBytes;
         HIPCHK(c, abyss::launch_finalize(part, p.splits, c->nt, c->cap, c->d_acc, c->d_adot,
                                          c->d_nnb, c->d_list,
@@ -714,7 +753,7 @@ int ctx_launch_force(abyss_ctx_t c)
         c->mirror_valid = mirror;
     }
     if (checked) {
-        int rc = recheck(c);
+        int rc = recheck(c, c->mirror_valid);
         if (rc != ABYSS_OK) return rc;
         c->spread_valid = true;
     }

@@ -184,11 +184,13 @@ const char *abyss_ctx_error_detail(abyss_ctx_t ctx);
  * terms dominate both) and recomputes with FP64 terms every target with
  *   ABYSS_CHECK_ERR_SCALE * sqrt(Sa) > tol * |a|  or
  *   ABYSS_CHECK_ERR_SCALE * sqrt(Sj) > tol * |adot|,
- * before the results are fetched.  A launch under this mode waits once on
- * the host (for the flagged count), so it is not asynchronous.  No
- * counterpart in the reference (whose FP32 build has no accuracy control).
+ * before the results are fetched.  A launch of more than 64 targets under
+ * this mode waits once on the host (for the flagged count, to plan the
+ * recomputation); smaller blocks read the count on the device and do not
+ * wait.  No counterpart in the reference (whose FP32 build has no accuracy
+ * control).
  *   set_check_tolerance: tol (> 0; default 1e-6);
- *   last_recheck: targets recomputed by the last launch;
+ *   last_recheck: targets recomputed by the last launch (waits for it);
  *   fetch_spread: [nt][2] = sqrt(Sa), sqrt(Sj) per target of the last
  *     launch. */
 #define ABYSS_CHECK_ERR_SCALE 1.0e-6

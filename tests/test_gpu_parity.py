@@ -211,6 +211,43 @@ def test_checked_precision(ctx, wl, size):
         assert rerr["a_rel"]["max"] <= 1e-11 and rerr["jdot_rel"]["max"] <= 1e-11
 
 
+@pytest.mark.parametrize("size", [8, 64])
+def test_checked_precision_small_block_with_recomputation(ctx, size):
+    """Blocks of up to 64 targets read the flagged count on the device (no
+    host wait) and update the mapped copy of the results.  A block made of
+    targets that a full-N checked launch had to recompute exercises that
+    path with a non-zero count."""
+    say(f"case: checked precision, small block of {size} flagged-prone targets")
+    js = workloads.plummer_kroupa(65536)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    full = np.arange(js.n, dtype=np.int32)
+    split_full = ctx.query(full)
+    ctx.set_precision(regf.PREC_CHECKED)
+    try:
+        chk_full = ctx.query(full)
+        hot = np.flatnonzero(np.any(chk_full[1] != split_full[1], axis=1))
+        cold = np.setdiff1d(full, hot)
+        t = np.concatenate([hot[: size * 3 // 4], cold[: size - min(len(hot), size * 3 // 4)]])
+        t = np.random.default_rng(size).permutation(t).astype(np.int32)
+        chk = ctx.query(t)
+        nre = ctx.last_recheck()
+        spr = ctx.spread()
+    finally:
+        ctx.set_precision(regf.PREC_SPLIT)
+    split = ctx.query(t)
+    assert len(t) == size and ctx.plan()["j_splits"] > 1
+    changed = np.any(chk[0] != split[0], axis=1) | np.any(chk[1] != split[1], axis=1)
+    say(f"[checked small {size}] recomputed={nre} changed={changed.sum()}")
+    assert nre > 0 and changed.sum() <= nre
+    assert np.array_equal(chk[2], split[2]) and np.array_equal(chk[3], split[3])
+    assert spr.shape == (size, 2) and np.all(np.isfinite(spr))
+    ra, rj, rn, rl = oracle.regf(js.x, js.v, js.m, js.r2, t, js.cap)
+    err = force_errors(chk[0], chk[1], ra, rj)
+    assert err["a_rel"]["max"] <= CHECK_TOL and err["jdot_rel"]["max"] <= CHECK_TOL
+    rerr = force_errors(chk[0][changed], chk[1][changed], ra[changed], rj[changed])
+    assert rerr["a_rel"]["max"] <= 1e-11 and rerr["jdot_rel"]["max"] <= 1e-11
+
+
 def test_checked_precision_tolerance_and_legacy(ctx):
     """A tighter tolerance recomputes more targets; the legacy entry points
     give the context's checked results bit for bit."""
