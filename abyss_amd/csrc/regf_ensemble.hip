@@ -28,6 +28,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -63,6 +64,13 @@ struct abyss_ens_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int grid = 0, fin_grid = 0;
+
+    // checked precision: per-descriptor flagged counts and the prefix of the
+    // targets each descriptor checks ([max_clusters (+1)])
+    bool checked = false;
+    double check_tol = 1.0e-6;
+    int *d_ckcount = nullptr, *h_ckcount = nullptr;
+    int *d_ck_off = nullptr, *h_ck_off = nullptr;
 
     char detail[512] = {0};
 };
@@ -141,7 +149,7 @@ int capture_graph(abyss_ens_t e)
     ENSCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     hipError_t r1 = abyss::launch_batch_reset(e->d_counter, e->stream);
     hipError_t r2 = abyss::launch_batch(e->d_cd, e->d_items, e->d_hdr, e->d_counter, e->cap,
-                                        e->grid, e->stream);
+                                        e->grid, e->checked, e->stream);
     hipError_t r4 = abyss::launch_batch_finalize(e->d_cd, e->d_fin_cl, e->d_fin_off, e->d_hdr,
                                                  e->cap, e->fin_grid, e->stream);
     hipGraph_t g = nullptr;
@@ -198,6 +206,12 @@ int abyss_ens_create(int device, int cap, int max_clusters, abyss_ens_t *out)
                           hipHostMallocDefault) != hipSuccess ||
             hipMalloc(&e->d_fin_off, (size_t)(max_clusters + 1) * sizeof(int)) != hipSuccess ||
             hipHostMalloc(&e->h_fin_off, (size_t)(max_clusters + 1) * sizeof(int),
+                          hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&e->d_ckcount, (size_t)max_clusters * sizeof(int)) != hipSuccess ||
+            hipHostMalloc(&e->h_ckcount, (size_t)max_clusters * sizeof(int),
+                          hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&e->d_ck_off, (size_t)(max_clusters + 1) * sizeof(int)) != hipSuccess ||
+            hipHostMalloc(&e->h_ck_off, (size_t)(max_clusters + 1) * sizeof(int),
                           hipHostMallocDefault) != hipSuccess) {
             rc = ABYSS_ERR_OUT_OF_MEMORY; break;
         }
@@ -231,8 +245,46 @@ int abyss_ens_destroy(abyss_ens_t e)
     if (e->d_fin_off) (void)hipFree(e->d_fin_off);
     if (e->h_fin_off) (void)hipHostFree(e->h_fin_off);
     if (e->d_pool) (void)hipFree(e->d_pool);
+    if (e->d_ckcount) (void)hipFree(e->d_ckcount);
+    if (e->h_ckcount) (void)hipHostFree(e->h_ckcount);
+    if (e->d_ck_off) (void)hipFree(e->d_ck_off);
+    if (e->h_ck_off) (void)hipHostFree(e->h_ck_off);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
+    return ABYSS_OK;
+}
+
+int abyss_ens_set_precision(abyss_ens_t e, int mode)
+{
+    if (!e) return ABYSS_ERR_INVALID_ARG;
+    if (mode != ABYSS_PREC_SPLIT && mode != ABYSS_PREC_CHECKED)
+        return ens_detail(e, ABYSS_ERR_INVALID_ARG,
+                          "ensembles run split or checked precision (mode %d)", mode);
+    const bool checked = mode == ABYSS_PREC_CHECKED;
+    if (checked != e->checked) {
+        ENSCHK(e, hipSetDevice(e->device));
+        ENSCHK(e, hipStreamSynchronize(e->stream));
+        drop_graph(e);   // the captured batch kernel depends on the mode
+        e->checked = checked;
+    }
+    return ABYSS_OK;
+}
+
+int abyss_ens_set_check_tolerance(abyss_ens_t e, double tol)
+{
+    if (!e) return ABYSS_ERR_INVALID_ARG;
+    if (!(tol > 0.0) || !std::isfinite(tol))
+        return ens_detail(e, ABYSS_ERR_INVALID_ARG, "check tolerance must be finite and > 0");
+    e->check_tol = tol;
+    return ABYSS_OK;
+}
+
+int abyss_ens_last_recheck(abyss_ens_t e, int cluster, int *count)
+{
+    if (!e || !count || cluster < 0 || cluster >= e->max_clusters) return ABYSS_ERR_INVALID_ARG;
+    abyss_ctx_t c = e->cl[cluster];
+    if (!c) return ens_detail(e, ABYSS_ERR_NO_JSET, "cluster %d is empty", cluster);
+    *count = c->n_rechecked;
     return ABYSS_OK;
 }
 
@@ -327,7 +379,8 @@ int abyss_ens_launch(abyss_ens_t e)
     };
     std::vector<Entry> act;
     size_t nitems = 0, pool = 0;
-    const size_t per_target = (size_t)e->cap * sizeof(int) + 6 * sizeof(double) + sizeof(int);
+    const size_t per_target = (size_t)e->cap * sizeof(int) + 6 * sizeof(double) + sizeof(int) +
+                              (e->checked ? 2 * sizeof(float) : 0);
     for (int k = 0; k < e->max_clusters; ++k) {
         abyss_ctx_t c = e->cl[k];
         if (!c || !e->active[k] || c->nt <= 0 || !c->plan_valid) continue;
@@ -362,6 +415,15 @@ int abyss_ens_launch(abyss_ens_t e)
     std::sort(act.begin(), act.end(), [](const Entry &a, const Entry &b) {
         return a.item_cost != b.item_cost ? a.item_cost > b.item_cost : a.cluster < b.cluster;
     });
+    if (e->checked) {
+        for (const Entry &en : act) {
+            abyss_ctx_t c = e->cl[en.cluster];
+            int rc = abyss::ctx_ensure_check(c, c->nt, 1);
+            if (rc != ABYSS_OK) return from_ctx(e, c, rc);
+            c->spread_valid = false;
+            c->n_rechecked = 0;
+        }
+    }
     int rc = ensure_items(e, nitems);
     if (rc != ABYSS_OK) return rc;
     rc = ensure_pool(e, pool);
@@ -375,12 +437,20 @@ int abyss_ens_launch(abyss_ens_t e)
         bump += (bytes + 255) & ~size_t(255);
         return r;
     };
-    int nfc = 0, nfin = 0;
+    int nfc = 0, nfin = 0, nck = 0;
     for (size_t a = 0; a < act.size(); ++a) {
         const Entry &en = act[a];
         abyss_ctx_t c = e->cl[en.cluster];
         const int tpb = abyss::targets_per_block(c->plan);
         ClusterDesc &d = e->h_cd[a];
+        d.spread = e->checked ? c->chk_spread : nullptr;
+        d.fq = c->chk_fq;
+        d.fi = c->chk_fi;
+        d.count = e->d_ckcount + a;
+        d.chk_q0 = en.first * tpb;
+        d.chk_nq = en.nt_slice;
+        e->h_ck_off[a] = nck;
+        nck += en.nt_slice;
         d.js = c->js;
         d.tgt = c->tgt;
         d.nt = c->nt;
@@ -399,6 +469,8 @@ int abyss_ens_launch(abyss_ens_t e)
             d.part.adot = reinterpret_cast<double *>(take(m * 3 * sizeof(double)));
             d.part.cnt = reinterpret_cast<int *>(take(m * sizeof(int)));
             d.part.list = reinterpret_cast<int *>(take(m * (size_t)e->cap * sizeof(int)));
+            d.part.spread =
+                e->checked ? reinterpret_cast<float *>(take(m * 2 * sizeof(float))) : nullptr;
             e->h_fin_cl[nfc] = (int)a;
             e->h_fin_off[nfc] = nfin;
             ++nfc;
@@ -406,7 +478,7 @@ int abyss_ens_launch(abyss_ens_t e)
         } else {
             d.q_base = 0;
             d.nt_stride = c->nt;
-            d.part = abyss::QueryOut{nullptr, nullptr, nullptr, nullptr};
+            d.part = abyss::QueryOut{};
         }
         for (int b = 0; b < en.nblocks; ++b)
             for (int sp = 0; sp < en.splits; ++sp)
@@ -414,6 +486,7 @@ int abyss_ens_launch(abyss_ens_t e)
         c->n_grav += 1;
     }
     e->h_fin_off[nfc] = nfin;
+    e->h_ck_off[act.size()] = nck;
     e->h_hdr->nitems = (int)nitems;
     e->h_hdr->nfin = nfin;
     e->h_hdr->nfc = nfc;
@@ -427,11 +500,33 @@ int abyss_ens_launch(abyss_ens_t e)
                              hipMemcpyHostToDevice, e->stream));
     ENSCHK(e, hipMemcpyAsync(e->d_hdr, e->h_hdr, sizeof(BatchHeader), hipMemcpyHostToDevice,
                              e->stream));
+    if (e->checked) {
+        ENSCHK(e, hipMemcpyAsync(e->d_ck_off, e->h_ck_off, (act.size() + 1) * sizeof(int),
+                                 hipMemcpyHostToDevice, e->stream));
+        ENSCHK(e, abyss::launch_zero_ints(e->d_ckcount, (int)act.size(), e->stream));
+    }
     if (!e->exec) {
         rc = capture_graph(e);
         if (rc != ABYSS_OK) return rc;
     }
     ENSCHK(e, hipGraphLaunch(e->exec, e->stream));
+    if (e->checked) {
+        // flag every checked target of the batch, read the per-cluster
+        // counts back (one host wait), then recompute each cluster's
+        // flagged targets with its own plan
+        const double ratio = ABYSS_CHECK_ERR_SCALE / e->check_tol;
+        ENSCHK(e, abyss::launch_batch_check_flag(e->d_cd, e->d_ck_off, (int)act.size(), nck,
+                                                 ratio * ratio, e->stream));
+        ENSCHK(e, hipMemcpyAsync(e->h_ckcount, e->d_ckcount, act.size() * sizeof(int),
+                                 hipMemcpyDeviceToHost, e->stream));
+        ENSCHK(e, hipStreamSynchronize(e->stream));
+        for (size_t a = 0; a < act.size(); ++a) {
+            abyss_ctx_t c = e->cl[act[a].cluster];
+            rc = abyss::ctx_recompute_flagged(c, e->h_ckcount[a]);
+            if (rc != ABYSS_OK) return from_ctx(e, c, rc);
+            c->spread_valid = true;
+        }
+    }
     return ABYSS_OK;
 }
 

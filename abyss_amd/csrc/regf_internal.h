@@ -83,6 +83,12 @@ struct ClusterDesc {
     QueryOut part;
     double *acc, *adot;
     int *nnb, *list;
+    // checked precision (null when off): final spread [nt][2] (part.spread
+    // holds a split slice's partials); flagged positions / j-indices and
+    // their counter; the slice of targets this batch evaluates
+    float *spread;
+    int *fq, *fi, *count;
+    int chk_q0, chk_nq;
 };
 
 // (cluster, target block, j-split): one workgroup's worth of work.
@@ -108,7 +114,15 @@ hipError_t launch_batch_reset(int *counters, hipStream_t st);
 // (the grid is that many per CU).
 int batch_blocks_per_cu();
 hipError_t launch_batch(const ClusterDesc *cd, const WorkItem *items, const BatchHeader *hdr,
-                        int *counter, int cap, int grid, hipStream_t st);
+                        int *counter, int cap, int grid, bool est, hipStream_t st);
+// Checked precision of a batch: over the chk_nq targets of each of the ncd
+// descriptors (ck_off[k]: prefix of their counts, ck_off[ncd] = total),
+// appends the flagged ones to the descriptor's fq / fi and counts them in
+// *count (which must start at 0); the rule of launch_check_flag.
+hipError_t launch_batch_check_flag(const ClusterDesc *cd, const int *ck_off, int ncd, int total,
+                                   double k2, hipStream_t st);
+// p[0..n) := 0
+hipError_t launch_zero_ints(int *p, int n, hipStream_t st);
 // Persistent finalize over the targets of split slices: fin_off[k] (k <=
 // nfc) is the prefix of slice target counts, fin_cl[k] the descriptor index.
 hipError_t launch_batch_finalize(const ClusterDesc *cd, const int *fin_cl, const int *fin_off,

@@ -877,11 +877,11 @@ __global__ void batch_reset_kernel(int *counters)
 // Every workgroup leaves the loop once the queue index passes nitems.
 // Lanes-per-target bodies only (ensemble members never plan one target per
 // lane), so the kernel fits 5 waves per SIMD like regf_kernel_jl.
+// EST: checked precision (the spread estimate of regf_body_jl).
+template <bool EST>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
-void batch_kernel(const ClusterDesc *__restrict__ cd,
-                                                       const WorkItem *__restrict__ items,
-                                                       const BatchHeader *__restrict__ hdr,
-                                                       int *__restrict__ counter, int cap)
+void batch_kernel(const ClusterDesc *__restrict__ cd, const WorkItem *__restrict__ items,
+                  const BatchHeader *__restrict__ hdr, int *__restrict__ counter, int cap)
 {
     __shared__ double2 s_pos[2][kTile * 3];
     __shared__ float s_m[2][kTile];
@@ -896,19 +896,20 @@ void batch_kernel(const ClusterDesc *__restrict__ cd,
         const WorkItem w = items[it];
         const ClusterDesc &d = cd[w.cluster];
         const int fc = d.splits == 1 ? 1 : 0;
-        const QueryOut out = fc ? QueryOut{d.acc, d.adot, d.nnb, d.list} : d.part;
+        const QueryOut out =
+            fc ? QueryOut{d.acc, d.adot, d.nnb, d.list, d.spread} : d.part;
         switch (d.jl) {
             case 4:
-                regf_body_jl<4, 0>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock, w.split,
-                                d.q_base, d.nt_stride, s_pos, s_m);
+                regf_body_jl<4, 0, EST>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock,
+                                        w.split, d.q_base, d.nt_stride, s_pos, s_m);
                 break;
             case 16:
-                regf_body_jl<16, 0>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock, w.split,
-                                 d.q_base, d.nt_stride, s_pos, s_m);
+                regf_body_jl<16, 0, EST>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock,
+                                         w.split, d.q_base, d.nt_stride, s_pos, s_m);
                 break;
             default:
-                regf_body_jl<64, 0>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock, w.split,
-                                 d.q_base, d.nt_stride, s_pos, s_m);
+                regf_body_jl<64, 0, EST>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock,
+                                         w.split, d.q_base, d.nt_stride, s_pos, s_m);
                 break;
         }
     }
@@ -942,8 +943,41 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(const ClusterDesc *
         const ClusterDesc &d = cd[fin_cl[k]];
         const int q_rel = active ? g - fin_off[k] : 0;
         finalize_body(d.part, active ? d.splits : 1, d.nt_stride, cap, q_rel, d.q_base + q_rel,
-                      active, off, d.acc, d.adot, d.nnb, d.list);
+                      active, off, d.acc, d.adot, d.nnb, d.list, FinalMirror{},
+                      d.part.spread ? d.spread : nullptr);
     }
+}
+
+// Flat over the checked targets of every descriptor (descriptor k by binary
+// search over ck_off); one thread per target.  Lanes of a wave may belong
+// to different clusters, so each flagged lane appends with its own atomic.
+__global__ __launch_bounds__(256) void batch_check_flag_kernel(const ClusterDesc *__restrict__ cd,
+                                                               const int *__restrict__ ck_off,
+                                                               int ncd, int total, double k2)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    int lo = 0, hi = ncd - 1;   // largest k with ck_off[k] <= g
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ck_off[mid] <= g) lo = mid; else hi = mid - 1;
+    }
+    const ClusterDesc &d = cd[lo];
+    const size_t q = (size_t)d.chk_q0 + (g - ck_off[lo]);
+    const double ax = d.acc[3 * q], ay = d.acc[3 * q + 1], az = d.acc[3 * q + 2];
+    const double jx = d.adot[3 * q], jy = d.adot[3 * q + 1], jz = d.adot[3 * q + 2];
+    if ((double)d.spread[2 * q + 0] * k2 > ax * ax + ay * ay + az * az ||
+        (double)d.spread[2 * q + 1] * k2 > jx * jx + jy * jy + jz * jz) {
+        const int k = atomicAdd(d.count, 1);
+        d.fq[k] = (int)q;
+        d.fi[k] = d.tgt[q];
+    }
+}
+
+__global__ void zero_ints_kernel(int *__restrict__ p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 template <int IPL, int U, bool F64>
@@ -1095,7 +1129,7 @@ hipError_t launch_batch_reset(int *counters, hipStream_t st)
 int batch_blocks_per_cu()
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel, kBlock, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, batch_kernel<false>, kBlock, 0) !=
             hipSuccess ||
         nb < 1)
         nb = 4;
@@ -1103,10 +1137,30 @@ int batch_blocks_per_cu()
 }
 
 hipError_t launch_batch(const ClusterDesc *cd, const WorkItem *items, const BatchHeader *hdr,
-                        int *counter, int cap, int grid, hipStream_t st)
+                        int *counter, int cap, int grid, bool est, hipStream_t st)
 {
-    hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(kBlock), 0, st, cd, items, hdr, counter,
-                       cap);
+    if (est)
+        hipLaunchKernelGGL(batch_kernel<true>, dim3(grid), dim3(kBlock), 0, st, cd, items, hdr,
+                           counter, cap);
+    else
+        hipLaunchKernelGGL(batch_kernel<false>, dim3(grid), dim3(kBlock), 0, st, cd, items, hdr,
+                           counter, cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_check_flag(const ClusterDesc *cd, const int *ck_off, int ncd, int total,
+                                   double k2, hipStream_t st)
+{
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_check_flag_kernel, dim3((total + 255) / 256), dim3(256), 0, st, cd,
+                       ck_off, ncd, total, k2);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_ints(int *p, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
 

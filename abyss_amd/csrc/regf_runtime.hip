@@ -339,6 +339,24 @@ int ensure_recheck_scratch(abyss_ctx_t c, size_t rows, QueryOut *rpart)
     return ABYSS_OK;
 }
 
+// The recomputation of m flagged targets (c->chk_fq / chk_fi hold them),
+// planned for m; the stream need not be idle.
+int recompute_flagged(abyss_ctx_t c, int m)
+{
+    c->n_rechecked = m;
+    if (m <= 0) return ABYSS_OK;
+    if (m > c->nt) return set_detail(c, ABYSS_ERR_HIP, "check: %d flagged of %d", m, c->nt);
+    const Plan rp = plan_for(c, m, c->n, ABYSS_PREC_REFINED, 0, false);
+    QueryOut rpart;
+    int rc = ensure_recheck_scratch(c, (size_t)rp.splits * m, &rpart);
+    if (rc != ABYSS_OK) return rc;
+    HIPCHK(c, abyss::launch_regf(c->js, c->chk_fi, m, rp, rpart, 0, abyss::kPrecRecheck,
+                                 c->stream));
+    HIPCHK(c, abyss::launch_recheck_finalize(rpart, rp.splits, m, c->chk_fq, c->d_acc, c->d_adot,
+                                             c->stream));
+    return ABYSS_OK;
+}
+
 // After a checked launch's force and finalize: flags the targets whose a or
 // jdot may miss the tolerance and recomputes them with FP64 terms over the
 // same regular set.  Blocks of up to kMappedTargets targets plan the
@@ -368,19 +386,7 @@ int recheck(abyss_ctx_t c, bool mirror)
     HIPCHK(c, hipMemcpyAsync(c->h_chk, c->chk_counters, sizeof(int), hipMemcpyDeviceToHost,
                              c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    const int m = c->h_chk[0];
-    c->n_rechecked = m;
-    if (m <= 0) return ABYSS_OK;
-    if (m > c->nt) return set_detail(c, ABYSS_ERR_HIP, "check: %d flagged of %d", m, c->nt);
-    const Plan rp = plan_for(c, m, c->n, ABYSS_PREC_REFINED, 0, false);
-    QueryOut rpart;
-    int rc = ensure_recheck_scratch(c, (size_t)rp.splits * m, &rpart);
-    if (rc != ABYSS_OK) return rc;
-    HIPCHK(c, abyss::launch_regf(c->js, c->chk_fi, m, rp, rpart, 0, abyss::kPrecRecheck,
-                                 c->stream));
-    HIPCHK(c, abyss::launch_recheck_finalize(rpart, rp.splits, m, c->chk_fq, c->d_acc, c->d_adot,
-                                             c->stream));
-    return ABYSS_OK;
+    return recompute_flagged(c, c->h_chk[0]);
 }
 
 int collect_timing(abyss_ctx_t c)
@@ -703,6 +709,10 @@ int abyss_ctx_plan(abyss_ctx_t c, int *ipl, int *splits, int *grid_x, int *tile,
 }  // extern "C"
 
 namespace abyss {
+
+int ctx_ensure_check(abyss_ctx_t c, int nt, int S) { return ensure_check(c, nt, S); }
+
+int ctx_recompute_flagged(abyss_ctx_t c, int m) { return recompute_flagged(c, m); }
 
 int ctx_launch_force(abyss_ctx_t c)
 {

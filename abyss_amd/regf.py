@@ -62,7 +62,8 @@ EXPORTS = [
     "abyss_ctx_keep_lists", "abyss_ctx_irregular", "abyss_ctx_regular_step",
     "abyss_ctx_derivs23", "abyss_ctx_update_radius", "abyss_srv_set_admission",
     "abyss_srv_queue_stats", "abyss_ctx_set_check_tolerance", "abyss_ctx_last_recheck",
-    "abyss_ctx_fetch_spread",
+    "abyss_ctx_fetch_spread", "abyss_ens_set_precision", "abyss_ens_set_check_tolerance",
+    "abyss_ens_last_recheck",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -118,6 +119,9 @@ def load():
         "abyss_ctx_set_check_tolerance": (ctypes.c_int, [_vp, ctypes.c_double]),
         "abyss_ctx_last_recheck": (ctypes.c_int, [_vp, _ip]),
         "abyss_ctx_fetch_spread": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+        "abyss_ens_set_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "abyss_ens_set_check_tolerance": (ctypes.c_int, [_vp, ctypes.c_double]),
+        "abyss_ens_last_recheck": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
         "abyss_ens_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.POINTER(_vp)]),
         "abyss_ens_destroy": (ctypes.c_int, [_vp]),
@@ -548,6 +552,19 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+    def set_precision(self, mode):
+        """PREC_SPLIT (default) or PREC_CHECKED for every cluster."""
+        self._chk(self._lib.abyss_ens_set_precision(self.h, int(mode)))
+
+    def set_check_tolerance(self, tol):
+        self._chk(self._lib.abyss_ens_set_check_tolerance(self.h, float(tol)))
+
+    def last_recheck(self, cluster):
+        """Targets of `cluster` the last checked launch recomputed."""
+        n = ctypes.c_int(0)
+        self._chk(self._lib.abyss_ens_last_recheck(self.h, int(cluster), ctypes.byref(n)))
+        return n.value
 
     def upload(self, cluster, m, x, v, r2):
         x = _f64(x, (-1, 3))

@@ -339,6 +339,15 @@ int abyss_ens_set_block_range(abyss_ens_t ens, int cluster, int first, int count
 int abyss_ens_fetch(abyss_ens_t ens, int cluster, double *acc, double *adot, int *nnb,
                     int *list);
 void *abyss_ens_stream(abyss_ens_t ens);
+/* Precision of every cluster: ABYSS_PREC_SPLIT (default) or
+ * ABYSS_PREC_CHECKED (the flag rule of abyss_ctx_set_check_tolerance, over
+ * every target a launch evaluates).  Under ABYSS_PREC_CHECKED a launch
+ * waits once on the host after the batch (for the flagged counts), then
+ * queues each cluster's recomputation; last_recheck reports a cluster's
+ * recomputed targets of the last launch. */
+int abyss_ens_set_precision(abyss_ens_t ens, int mode);
+int abyss_ens_set_check_tolerance(abyss_ens_t ens, double tol);
+int abyss_ens_last_recheck(abyss_ens_t ens, int cluster, int *count);
 /* Device bytes held by the ensemble (all clusters + descriptors). */
 unsigned long long abyss_ens_device_bytes(abyss_ens_t ens);
 const char *abyss_ens_error_detail(abyss_ens_t ens);

@@ -89,6 +89,68 @@ def test_ensemble_matches_alone_bitwise(clusters):
             say(f"  cluster {c}: n={len(clusters[c][2])} nt={len(t)} identical alone/mixed")
 
 
+def test_ensemble_checked_precision(clusters):
+    """Checked precision in a batch: lists and counts those of split
+    precision, every target within 1e-6 relative of the oracle (a and jdot)
+    on rows whose regular set the FP64 test agrees with, recomputed targets
+    FP64-accurate, a cluster alone equal to the mix bit for bit, and split
+    precision restored bit for bit after switching back."""
+    say("case: ensemble checked precision")
+    with regf.Ensemble(device=0, cap=128, max_clusters=len(clusters)) as ens:
+        for c, (x, v, m, r2) in enumerate(clusters):
+            ens.upload(c, m, x, v, r2)
+        targets = [random_targets(len(cl[2]), 51 + c) for c, cl in enumerate(clusters)]
+        targets[10] = np.arange(len(clusters[10][2]), dtype=np.int32)   # one full cluster
+        for c, t in enumerate(targets):
+            ens.set_targets(c, t)
+        ens.launch()
+        ens.sync()
+        split = [ens.fetch(c) for c in range(len(clusters))]
+        ens.set_precision(regf.PREC_CHECKED)
+        ens.launch()
+        ens.sync()
+        chk = [ens.fetch(c) for c in range(len(clusters))]
+        nre = [ens.last_recheck(c) for c in range(len(clusters))]
+        # alone vs mixed for the full cluster and a small one
+        for c in (10, 3):
+            for k in range(len(clusters)):
+                ens.set_targets(k, targets[k] if k == c else [])
+            ens.launch()
+            ens.sync()
+            alone = ens.fetch(c)
+            assert ens.last_recheck(c) == nre[c]
+            for a, b in zip(alone, chk[c]):
+                assert np.array_equal(a, b), f"cluster {c} differs alone vs mixed (checked)"
+        ens.set_precision(regf.PREC_SPLIT)
+        for c, t in enumerate(targets):
+            ens.set_targets(c, t)
+        ens.launch()
+        ens.sync()
+        again = [ens.fetch(c) for c in range(len(clusters))]
+    say(f"  recomputed per cluster: {nre}")
+    assert sum(nre) > 0
+    for c, (cl, t) in enumerate(zip(clusters, targets)):
+        for a, b in zip(again[c], split[c]):
+            assert np.array_equal(a, b), f"cluster {c}: split precision not restored"
+        assert np.array_equal(chk[c][2], split[c][2]) and np.array_equal(chk[c][3], split[c][3])
+        changed = np.any(chk[c][0] != split[c][0], axis=1) | np.any(chk[c][1] != split[c][1],
+                                                                     axis=1)
+        assert changed.sum() <= nre[c]
+        x, v, m, r2 = cl
+        ra, rj, rn, rl = oracle.regf(x, v, m, r2, t, 128)
+        w = 128
+        valid = np.arange(w)[None, :] < np.minimum(np.abs(chk[c][2]), w)[:, None]
+        rows = (chk[c][2] == rn) & np.all((chk[c][3] == rl) | ~valid, axis=1)
+        err = force_errors(chk[c][0][rows], chk[c][1][rows], ra[rows], rj[rows])
+        assert err["a_rel"]["max"] <= 1e-6 and err["jdot_rel"]["max"] <= 1e-6, (c, err)
+        if (changed & rows).any():
+            rerr = force_errors(chk[c][0][changed & rows], chk[c][1][changed & rows],
+                                ra[changed & rows], rj[changed & rows])
+            assert rerr["a_rel"]["max"] <= 1e-11 and rerr["jdot_rel"]["max"] <= 1e-11, (c, rerr)
+        say(f"  cluster {c}: n={len(m)} nt={len(t)} recomputed={nre[c]} band_rows="
+            f"{(~rows).sum()} jdot_rel_max={err['jdot_rel']['max']:.2e}")
+
+
 def test_ensemble_graph_replays_with_changing_mix(clusters):
     say("case: ensemble graph replays")
     rng = np.random.default_rng(3)
