@@ -830,6 +830,20 @@ int abyss_srv_set_batch_budget(abyss_srv_t s, double pairs)
     return ABYSS_OK;
 }
 
+int abyss_srv_set_precision(abyss_srv_t s, int mode)
+{
+    if (!s) return ABYSS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> elk(s->ens_mu);   // between two batches
+    return abyss_ens_set_precision(s->ens, mode);
+}
+
+int abyss_srv_set_check_tolerance(abyss_srv_t s, double tol)
+{
+    if (!s) return ABYSS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> elk(s->ens_mu);
+    return abyss_ens_set_check_tolerance(s->ens, tol);
+}
+
 unsigned long long abyss_srv_cluster_footprint(abyss_srv_t s, int n)
 {
     if (!s || n <= 0) return 0;

@@ -63,7 +63,7 @@ EXPORTS = [
     "abyss_ctx_derivs23", "abyss_ctx_update_radius", "abyss_srv_set_admission",
     "abyss_srv_queue_stats", "abyss_ctx_set_check_tolerance", "abyss_ctx_last_recheck",
     "abyss_ctx_fetch_spread", "abyss_ens_set_precision", "abyss_ens_set_check_tolerance",
-    "abyss_ens_last_recheck",
+    "abyss_ens_last_recheck", "abyss_srv_set_precision", "abyss_srv_set_check_tolerance",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -122,6 +122,8 @@ def load():
         "abyss_ens_set_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
         "abyss_ens_set_check_tolerance": (ctypes.c_int, [_vp, ctypes.c_double]),
         "abyss_ens_last_recheck": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+        "abyss_srv_set_precision": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "abyss_srv_set_check_tolerance": (ctypes.c_int, [_vp, ctypes.c_double]),
         "abyss_ens_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.POINTER(_vp)]),
         "abyss_ens_destroy": (ctypes.c_int, [_vp]),
@@ -638,6 +640,17 @@ class Server:
             raise AbyssError(rc)
         self.h = h
         self.cap = cap
+
+    def set_precision(self, mode):
+        """PREC_SPLIT (default) or PREC_CHECKED, from the next batch on."""
+        rc = self._lib.abyss_srv_set_precision(self.h, int(mode))
+        if rc != OK:
+            raise AbyssError(rc)
+
+    def set_check_tolerance(self, tol):
+        rc = self._lib.abyss_srv_set_check_tolerance(self.h, float(tol))
+        if rc != OK:
+            raise AbyssError(rc)
 
     def close(self):
         if self.h:

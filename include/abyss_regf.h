@@ -368,6 +368,13 @@ int abyss_srv_create(int device, int cap, int max_clusters, unsigned long long h
 int abyss_srv_destroy(abyss_srv_t srv);
 /* Work budget of one batch in pair-interactions (default 3e10, ~25 ms). */
 int abyss_srv_set_batch_budget(abyss_srv_t srv, double pairs);
+/* Precision of the served clusters (abyss_ens_set_precision; takes effect
+ * from the next batch).  Under ABYSS_PREC_CHECKED a request whose blocks
+ * span several batches gets each batch's flagged targets recomputed with a
+ * plan for that batch's count: the same values alone and in the mix to
+ * ~1e-13 relative (unflagged targets bit for bit). */
+int abyss_srv_set_precision(abyss_srv_t srv, int mode);
+int abyss_srv_set_check_tolerance(abyss_srv_t srv, double tol);
 /* Admits a cluster (ABYSS_ERR_ADMISSION if it does not fit the budget,
  * ABYSS_ERR_BUSY if no slot is free); returns its id in *cluster. */
 int abyss_srv_add_cluster(abyss_srv_t srv, int n, const double *m, const double *x,

@@ -242,6 +242,47 @@ def test_server_concurrent_requests_match_alone(clusters, budget):
                 assert np.array_equal(alone[3][k, :nk], got[3][k, :nk]), f"request {q} list"
 
 
+def test_server_checked_precision_matches_alone(clusters):
+    """Served requests under checked precision, with a small batch budget so
+    requests span several batches: counts and lists equal to the cluster
+    alone (checked ensemble), a and jdot equal bit for bit where neither
+    recomputed, within 1e-12 relative elsewhere (both FP64-term results)."""
+    say("case: server checked precision vs alone")
+    sub = clusters[:8]
+    with regf.Server(device=0, cap=128, max_clusters=8, hbm_budget=8 << 30) as srv:
+        srv.set_precision(regf.PREC_CHECKED)
+        srv.set_batch_budget(2e7)
+        ids = [srv.add_cluster(m, x, v, r2) for (x, v, m, r2) in sub]
+        reqs = [(ids[c % 8], random_targets(len(sub[c % 8][2]), 900 + c)) for c in range(16)]
+        tickets = [srv.submit(*r) for r in reqs]
+        results = [srv.wait(tk, nt) for tk, nt in tickets]
+        assert srv.stats()["batches"] > len(set(ids))
+    differ = 0
+    with regf.Ensemble(device=0, cap=128, max_clusters=8) as ens:
+        ens.set_precision(regf.PREC_CHECKED)
+        for c, (x, v, m, r2) in enumerate(sub):
+            ens.upload(c, m, x, v, r2)
+        for q, (cid, t) in enumerate(reqs):
+            c = ids.index(cid)
+            for k in range(8):
+                ens.set_targets(k, t if k == c else [])
+            ens.launch()
+            ens.sync()
+            alone = ens.fetch(c)
+            got = results[q]
+            assert np.array_equal(alone[2], got[2]), f"request {q} counts"
+            for k in range(len(t)):
+                nk = alone[2][k]
+                assert np.array_equal(alone[3][k, :nk], got[3][k, :nk]), f"request {q} list"
+            for a, b in zip(alone[:2], got[:2]):
+                same = np.all(a == b, axis=1)
+                differ += int((~same).sum())
+                rel = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(a, axis=1),
+                                                                1e-300)
+                assert rel.max() <= 1e-12, f"request {q}: {rel.max()}"
+    say(f"  targets differing in the last bits (recomputed under another plan): {differ}")
+
+
 def test_server_admission_by_budget(clusters):
     say("case: server admission")
     x, v, m, r2 = clusters[6]             # n = 20000
