@@ -497,12 +497,14 @@ def test_king_262144_full_properties(ctx):
     check(js, sample, sub, label="king262144/sample")
 
 
-def test_king_262144_checked_every_target(ctx):
-    """Checked precision on the full King workload: every target within 1e-6
-    relative (a and jdot) of the FP64 kernel, over the rows whose regular
-    set the FP64 test agrees with (the rest differ by a band pair)."""
-    say("case: king_w6_binaries N=262144 checked, every target")
-    js = workloads.king_binaries(262144)
+@pytest.mark.parametrize("seed", [1, 2])
+def test_king_262144_checked_every_target(ctx, seed):
+    """Checked precision on the full King workload (two realisations): every
+    target within 1e-6 relative (a and jdot) of the FP64 kernel, over the
+    rows whose regular set the FP64 test agrees with (the rest differ by a
+    band pair)."""
+    say(f"case: king_w6_binaries N=262144 seed {seed} checked, every target")
+    js = workloads.king_binaries(262144, seed=seed)
     ctx.upload(js.m, js.x, js.v, js.r2)
     t = np.arange(js.n, dtype=np.int32)
     ctx.set_precision(regf.PREC_CHECKED)
