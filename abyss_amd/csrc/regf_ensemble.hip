@@ -520,11 +520,89 @@ int abyss_ens_launch(abyss_ens_t e)
         ENSCHK(e, hipMemcpyAsync(e->h_ckcount, e->d_ckcount, act.size() * sizeof(int),
                                  hipMemcpyDeviceToHost, e->stream));
         ENSCHK(e, hipStreamSynchronize(e->stream));
+        // The recomputation of every cluster's flagged targets in one
+        // persistent launch.  A cluster's items follow the rule above for
+        // its own flagged count and N, so its results do not depend on the
+        // rest of the batch.  The batch has finished: its descriptor, item,
+        // header, prefix and pool buffers can be refilled.
+        struct Rec {
+            int a, m, jl, nb, splits, jspan;
+        };
+        std::vector<Rec> rs;
+        size_t nitems2 = 0, pool2 = 0;
         for (size_t a = 0; a < act.size(); ++a) {
             abyss_ctx_t c = e->cl[act[a].cluster];
-            rc = abyss::ctx_recompute_flagged(c, e->h_ckcount[a]);
-            if (rc != ABYSS_OK) return from_ctx(e, c, rc);
+            const int m = e->h_ckcount[a];
+            c->n_rechecked = m;
             c->spread_valid = true;
+            if (m <= 0) continue;
+            if (m > act[a].nt_slice)
+                return ens_detail(e, ABYSS_ERR_HIP, "check: %d flagged of %d", m,
+                                  act[a].nt_slice);
+            const int jl = m >= 256 ? 4 : (m >= 16 ? 16 : 64);
+            const int tpb = 4 * (64 / jl);
+            const int tile = abyss::kTile;
+            const int nround = ((c->n + tile - 1) / tile) * tile;
+            int jspan = (int)std::ceil(kItemPairs / std::min(tpb, m) / tile) * tile;
+            if (jl >= 16) jspan = std::min(jspan, kItemTilesSmall * tile);
+            jspan = std::max(tile, std::min(jspan, nround));
+            int S = (c->n + jspan - 1) / jspan;
+            if (S > abyss::kMaxSplitsBatch) {
+                S = abyss::kMaxSplitsBatch;
+                jspan = (((c->n + S - 1) / S + tile - 1) / tile) * tile;
+                S = (c->n + jspan - 1) / jspan;
+            }
+            const int nb = (m + tpb - 1) / tpb;
+            rs.push_back({(int)a, m, jl, nb, S, jspan});
+            nitems2 += (size_t)nb * S;
+            pool2 += (size_t)S * m * (6 * sizeof(double) + sizeof(int)) + 3 * 256;
+        }
+        if (!rs.empty()) {
+            rc = ensure_items(e, nitems2);
+            if (rc != ABYSS_OK) return rc;
+            rc = ensure_pool(e, pool2);
+            if (rc != ABYSS_OK) return rc;
+            bump = e->d_pool;
+            size_t it2 = 0;
+            int total = 0;
+            for (size_t r = 0; r < rs.size(); ++r) {
+                const Rec &x = rs[r];
+                abyss_ctx_t c = e->cl[act[x.a].cluster];
+                ClusterDesc &d = e->h_cd[r];
+                d = ClusterDesc{};
+                d.js = c->js;
+                d.tgt = c->chk_fi;
+                d.nt = x.m;
+                d.jl = x.jl;
+                d.splits = x.splits;
+                d.jspan = x.jspan;
+                d.nt_stride = x.m;
+                const size_t mm = (size_t)x.splits * x.m;
+                d.part.acc = reinterpret_cast<double *>(take(mm * 3 * sizeof(double)));
+                d.part.adot = reinterpret_cast<double *>(take(mm * 3 * sizeof(double)));
+                d.part.cnt = reinterpret_cast<int *>(take(mm * sizeof(int)));
+                d.acc = c->d_acc;
+                d.adot = c->d_adot;
+                d.fq = c->chk_fq;
+                e->h_ck_off[r] = total;
+                total += x.m;
+                for (int b = 0; b < x.nb; ++b)
+                    for (int sp = 0; sp < x.splits; ++sp)
+                        e->h_items[it2++] = WorkItem{(int)r, b, sp, 0};
+            }
+            e->h_ck_off[rs.size()] = total;
+            e->h_hdr->nitems = (int)nitems2;
+            ENSCHK(e, hipMemcpyAsync(e->d_cd, e->h_cd, rs.size() * sizeof(ClusterDesc),
+                                     hipMemcpyHostToDevice, e->stream));
+            ENSCHK(e, hipMemcpyAsync(e->d_items, e->h_items, nitems2 * sizeof(WorkItem),
+                                     hipMemcpyHostToDevice, e->stream));
+            ENSCHK(e, hipMemcpyAsync(e->d_ck_off, e->h_ck_off, (rs.size() + 1) * sizeof(int),
+                                     hipMemcpyHostToDevice, e->stream));
+            ENSCHK(e, hipMemcpyAsync(e->d_hdr, e->h_hdr, sizeof(BatchHeader),
+                                     hipMemcpyHostToDevice, e->stream));
+            ENSCHK(e, abyss::launch_batch_recheck(e->d_cd, e->d_items, e->d_hdr, e->d_counter,
+                                                  e->d_ck_off, (int)rs.size(), total, e->grid,
+                                                  e->fin_grid, e->stream));
         }
     }
     return ABYSS_OK;

@@ -121,6 +121,14 @@ hipError_t launch_batch(const ClusterDesc *cd, const WorkItem *items, const Batc
 // *count (which must start at 0); the rule of launch_check_flag.
 hipError_t launch_batch_check_flag(const ClusterDesc *cd, const int *ck_off, int ncd, int total,
                                    double k2, hipStream_t st);
+// Checked precision of a batch, the recomputation: counter reset, the
+// persistent recompute over the items of the ncd descriptors (tgt = flagged
+// j-indices, nt = their count, part = split partials with stride nt), then
+// the merge of every flagged target (off[k]: prefix of the descriptors'
+// counts, off[ncd] = total) over the cluster's results at fq.
+hipError_t launch_batch_recheck(const ClusterDesc *cd, const WorkItem *items,
+                                const BatchHeader *hdr, int *counter, const int *off, int ncd,
+                                int total, int grid, int fin_grid, hipStream_t st);
 // p[0..n) := 0
 hipError_t launch_zero_ints(int *p, int n, hipStream_t st);
 // Persistent finalize over the targets of split slices: fin_off[k] (k <=

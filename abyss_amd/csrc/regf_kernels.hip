@@ -915,6 +915,70 @@ void batch_kernel(const ClusterDesc *__restrict__ cd, const WorkItem *__restrict
     }
 }
 
+// Checked precision of a batch: the recomputation of every cluster's
+// flagged targets in one persistent launch.  A descriptor's tgt holds the
+// flagged j-indices and nt their count; results always go to the split
+// partials d.part (stride d.nt), merged by batch_recheck_finalize_kernel.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+void batch_recheck_kernel(const ClusterDesc *__restrict__ cd, const WorkItem *__restrict__ items,
+                          const BatchHeader *__restrict__ hdr, int *__restrict__ counter)
+{
+    __shared__ double2 s_pos[2][kTile * 3];
+    __shared__ double s_m[2][kTile];
+    __shared__ int s_item;
+    const int nitems = hdr->nitems;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = s_item;
+        __syncthreads();
+        if (it >= nitems) break;
+        const WorkItem w = items[it];
+        const ClusterDesc &d = cd[w.cluster];
+        switch (d.jl) {
+            case 4:
+                regf_body_jl<4, 3>(d.js, d.tgt, d.nt, d.jspan, d.part, 0, 0, w.tblock, w.split, 0,
+                                   d.nt, s_pos, s_m);
+                break;
+            case 16:
+                regf_body_jl<16, 3>(d.js, d.tgt, d.nt, d.jspan, d.part, 0, 0, w.tblock, w.split,
+                                    0, d.nt, s_pos, s_m);
+                break;
+            default:
+                regf_body_jl<64, 3>(d.js, d.tgt, d.nt, d.jspan, d.part, 0, 0, w.tblock, w.split,
+                                    0, d.nt, s_pos, s_m);
+                break;
+        }
+    }
+}
+
+// One wave per flagged target, persistent over their flat index (descriptor
+// k by binary search over off, off[ncd] = total): the S partials summed in
+// split order and written over the cluster's results at fq.
+__global__ __launch_bounds__(256) void batch_recheck_finalize_kernel(
+    const ClusterDesc *__restrict__ cd, const int *__restrict__ off, int ncd, int total)
+{
+    const int lane = threadIdx.x & 63;
+    for (int base = blockIdx.x * 4; base < total; base += gridDim.x * 4) {
+        const int g = base + (threadIdx.x >> 6);
+        if (g >= total) continue;   // the whole wave; no barrier below
+        int lo = 0, hi = ncd - 1;   // largest k with off[k] <= g
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= g) lo = mid; else hi = mid - 1;
+        }
+        const ClusterDesc &d = cd[lo];
+        const int k = g - off[lo];
+        double a[6];
+        sum_splits(d.part, d.splits, d.nt, k, true, a);
+        const size_t q = (size_t)d.fq[k];
+        if (lane < 3)
+            d.acc[q * 3 + lane] = a[lane];
+        else if (lane < 6)
+            d.adot[q * 3 + lane - 3] = a[lane];
+    }
+}
+
 // Persistent over the flat index of (split slice, target); the loop bound is
 // the same for every wave of a workgroup, so the body's barrier is uniform.
 __global__ __launch_bounds__(256) void batch_finalize_kernel(const ClusterDesc *__restrict__ cd,
@@ -1154,6 +1218,21 @@ hipError_t launch_batch_check_flag(const ClusterDesc *cd, const int *ck_off, int
     if (total <= 0) return hipSuccess;
     hipLaunchKernelGGL(batch_check_flag_kernel, dim3((total + 255) / 256), dim3(256), 0, st, cd,
                        ck_off, ncd, total, k2);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_recheck(const ClusterDesc *cd, const WorkItem *items,
+                                const BatchHeader *hdr, int *counter, const int *off, int ncd,
+                                int total, int grid, int fin_grid, hipStream_t st)
+{
+    hipError_t e = launch_batch_reset(counter, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(batch_recheck_kernel, dim3(grid), dim3(kBlock), 0, st, cd, items, hdr,
+                       counter);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(batch_recheck_finalize_kernel, dim3(fin_grid), dim3(256), 0, st, cd, off,
+                       ncd, total);
     return hipGetLastError();
 }
 
