@@ -133,7 +133,4 @@ int ctx_launch_force(abyss_ctx_t c);
 int keep_launch_lists(abyss_ctx_t c);
 // Checked precision: the spread / flag buffers for nt targets and S splits.
 int ctx_ensure_check(abyss_ctx_t c, int nt, int S);
-// Checked precision: recompute the m flagged targets held in chk_fq / chk_fi
-// (sets n_rechecked; issues work on the context stream, no host wait).
-int ctx_recompute_flagged(abyss_ctx_t c, int m);
 }  // namespace abyss

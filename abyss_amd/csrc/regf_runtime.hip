@@ -712,8 +712,6 @@ namespace abyss {
 
 int ctx_ensure_check(abyss_ctx_t c, int nt, int S) { return ensure_check(c, nt, S); }
 
-int ctx_recompute_flagged(abyss_ctx_t c, int m) { return recompute_flagged(c, m); }
-
 int ctx_launch_force(abyss_ctx_t c)
 {
     if (!c) return ABYSS_ERR_INVALID_ARG;
