@@ -585,15 +585,19 @@ def bench_serve(args, rate=2000.0, window_s=1.0):
     out = {}
     # nominal: every request queues (unbounded); nominal_admission: requests
     # that would take the outstanding work past ADMIT_PAIRS (~0.25 s of the
-    # chip's work) are turned away; sustained: a rate the chip keeps up with
+    # chip's work) are turned away; sustained: a rate the chip keeps up with;
+    # sustained_checked: the same under checked precision
     legs = (("nominal", rate, window_s, 0.0), ("nominal_admission", rate, window_s, ADMIT_PAIRS),
-            ("sustained", args.serve_rate, 4.0, 0.0))
+            ("sustained", args.serve_rate, 4.0, 0.0),
+            ("sustained_checked", args.serve_rate, 4.0, 0.0))
     for name, r, w, admit in legs:
+        srv.set_precision(regf.PREC_CHECKED if name.endswith("checked") else regf.PREC_SPLIT)
         srv.set_admission(admit)
         srv.reset_stats()
         lat, inter, wall, rejected = serve_stream(srv, ids, sizes, r, w)
         st = srv.stats()
         out[name] = {"rate_req_s": r, "window_s": w, "admission_pairs": admit or None,
+                     "precision": "checked" if name.endswith("checked") else "split",
                      "requests": int(len(lat)) + rejected, "admitted": int(len(lat)),
                      "rejected": rejected, "max_queued": st["max_pending"],
                      "peak_outstanding_pairs": st["peak_outstanding_pairs"],
