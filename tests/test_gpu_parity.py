@@ -246,6 +246,18 @@ def test_checked_precision_small_block_with_recomputation(ctx, size):
     assert err["a_rel"]["max"] <= CHECK_TOL and err["jdot_rel"]["max"] <= CHECK_TOL
     rerr = force_errors(chk[0][changed], chk[1][changed], ra[changed], rj[changed])
     assert rerr["a_rel"]["max"] <= 1e-11 and rerr["jdot_rel"]["max"] <= 1e-11
+    # the legacy entry points (their own context, list stride 1000) give the
+    # same results for the same block
+    regf.OpenDevice(0)
+    try:
+        regf.set_precision(regf.PREC_CHECKED)
+        regf.SendToDevice(js.m, js.x, js.v, js.r2)
+        acc, adot, nnb, _ = regf.CalculateAccelerationOnDevice(t)
+    finally:
+        regf.set_precision(regf.PREC_SPLIT)
+        regf.CloseDevice()
+    assert np.array_equal(acc, chk[0]) and np.array_equal(adot, chk[1])
+    assert np.array_equal(nnb, chk[2])
 
 
 def test_checked_precision_tolerance_and_legacy(ctx):
