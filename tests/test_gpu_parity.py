@@ -364,6 +364,62 @@ def test_overflow(ctx):
         small.close()
 
 
+@pytest.mark.parametrize("tuning", [(0, 0, 0, 16), (0, 37, 0, 4), (0, 3, 0, 64)])
+def test_checked_precision_forced_plans(ctx, tuning):
+    """Forced lane groups and split counts: the flag and the recomputation
+    follow the plan of the split pass, and every result stays within 1e-6
+    of the oracle."""
+    say(f"case: checked precision with forced tuning {tuning}")
+    js = workloads.plummer_kroupa(65536)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    t = np.random.default_rng(71).permutation(js.n)[:2048].astype(np.int32)
+    ctx.set_tuning(*tuning)
+    ctx.set_precision(regf.PREC_CHECKED)
+    try:
+        chk = ctx.query(t)
+        nre = ctx.last_recheck()
+        plan = ctx.plan()
+    finally:
+        ctx.set_precision(regf.PREC_SPLIT)
+        ctx.set_tuning()
+    assert plan["lanes_per_target"] == tuning[3]
+    if tuning[1]:
+        assert plan["j_splits"] == tuning[1]
+    ra, rj, rn, rl = oracle.regf(js.x, js.v, js.m, js.r2, t, js.cap)
+    rows = _same_set_rows(chk[2], chk[3], rn, rl)
+    err = force_errors(chk[0][rows], chk[1][rows], ra[rows], rj[rows])
+    say(f"[checked forced {tuning}] recomputed={nre} a_rel={err['a_rel']['max']:.2e} "
+        f"jdot_rel={err['jdot_rel']['max']:.2e}")
+    assert err["a_rel"]["max"] <= CHECK_TOL and err["jdot_rel"]["max"] <= CHECK_TOL
+
+
+def test_checked_precision_on_predicted_positions(ctx):
+    """Checked precision on a j-set predicted on the device."""
+    say("case: checked precision after device prediction, plummer_kroupa N=65536")
+    js = workloads.plummer_kroupa(65536)
+    rng = np.random.default_rng(13)
+    a = rng.normal(size=(js.n, 3)) * 0.1
+    ad = rng.normal(size=(js.n, 3)) * 0.1
+    t0 = rng.uniform(0.0, 0.01, js.n)
+    ctx.upload_state(js.m, js.x, js.v, a, ad, t0, js.r2)
+    ctx.predict(0.01)
+    t = rng.permutation(js.n)[:4096].astype(np.int32)
+    ctx.set_precision(regf.PREC_CHECKED)
+    try:
+        chk = ctx.query(t)
+        nre = ctx.last_recheck()
+    finally:
+        ctx.set_precision(regf.PREC_SPLIT)
+    xp, vp = oracle.predict2(js.x, js.v, a, ad, 0.01 - t0)
+    ra, rj, rn, rl = oracle.regf(xp, vp, js.m, js.r2, t, js.cap)
+    rows = _same_set_rows(chk[2], chk[3], rn, rl)
+    err = force_errors(chk[0][rows], chk[1][rows], ra[rows], rj[rows])
+    say(f"[checked predicted] recomputed={nre} band_rows={(~rows).sum()} "
+        f"a_rel={err['a_rel']['max']:.2e} jdot_rel={err['jdot_rel']['max']:.2e}")
+    assert nre > 0 and (~rows).sum() <= 8
+    assert err["a_rel"]["max"] <= CHECK_TOL and err["jdot_rel"]["max"] <= CHECK_TOL
+
+
 def test_checked_precision_overflowed_lists():
     """Overflowed lists (cap 16, ~48 neighbours each): the recomputation
     still sums exactly the regular set (its test does not read the lists),
