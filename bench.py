@@ -743,12 +743,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (f64 i-j differences and per-particle sums)" if args.precision == "split"
-                 else "f64",
+        "dtype": "f32" if args.precision == "split" else "f64",
         "data": "synthetic",
         "config": {"workload": f"{js.name}_fullN", "n": js.n, "active": int(len(targets)),
                    "neighbours_target": js.meta["k_nb"], "cap": js.cap,
-                   "precision": args.precision, "plan": plan},
+                   "precision": args.precision,
+                   "arithmetic": "f64 i-j differences, f32 r^2 / rsqrt / terms, f32 runs of "
+                                 "64 pairs folded into f64 sums" if args.precision == "split"
+                                 else "f64 throughout",
+                   "plan": plan},
         "pct_fp32_peak": 100.0 * achieved / FP32_PEAK_TFLOPS,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
