@@ -565,6 +565,29 @@ def test_king_262144_full_properties(ctx):
     check(js, sample, sub, label="king262144/sample")
 
 
+def test_king_262144_exact_scalings(ctx):
+    """Size-independent properties of the full King step: doubling every mass
+    doubles a and jdot exactly, doubling every velocity doubles jdot exactly
+    and leaves a unchanged, and neither changes a list.  A power-of-two
+    factor commutes with every rounding of the split arithmetic (FP64
+    differences, FP32 terms and runs, FP64 folds and split sums), so the
+    equalities are bit for bit; a dropped or misplaced mass or velocity
+    factor in the kernel breaks them."""
+    say("case: king_w6_binaries N=262144 exact mass / velocity scalings")
+    js = workloads.king_binaries(262144)
+    t = np.arange(js.n, dtype=np.int32)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    base = ctx.query(t)
+    ctx.upload(2.0 * js.m, js.x, js.v, js.r2)
+    m2 = ctx.query(t)
+    ctx.upload(js.m, js.x, 2.0 * js.v, js.r2)
+    v2 = ctx.query(t)
+    for r in (m2, v2):
+        assert np.array_equal(r[2], base[2]) and np.array_equal(r[3], base[3])
+    assert np.array_equal(m2[0], 2.0 * base[0]) and np.array_equal(m2[1], 2.0 * base[1])
+    assert np.array_equal(v2[0], base[0]) and np.array_equal(v2[1], 2.0 * base[1])
+
+
 @pytest.mark.parametrize("seed", [1, 2])
 def test_king_262144_checked_every_target(ctx, seed):
     """Checked precision on the full King workload (two realisations): every
