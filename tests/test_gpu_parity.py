@@ -588,6 +588,35 @@ def test_king_262144_exact_scalings(ctx):
     assert np.array_equal(v2[0], base[0]) and np.array_equal(v2[1], 2.0 * base[1])
 
 
+def test_permuted_jset_relabels_lists():
+    """Full-N query of a permuted j-set (Plummer + Kroupa N = 65536): every
+    list is the original one relabelled (and ascending in the new labels),
+    counts are equal, and a / jdot agree to FP32 rounding (the summation
+    order changes with the permutation)."""
+    say("case: permuted j-set, plummer_kroupa N=65536 full")
+    js = workloads.plummer_kroupa(65536)
+    perm = np.random.default_rng(23).permutation(js.n)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(js.n)
+    with regf.Context(device=0, cap=js.cap) as c:
+        c.upload(js.m, js.x, js.v, js.r2)
+        a0, j0, n0, l0 = c.query(np.arange(js.n, dtype=np.int32))
+        c.upload(js.m[perm], js.x[perm], js.v[perm], js.r2[perm])
+        a1, j1, n1, l1 = c.query(np.arange(js.n, dtype=np.int32))
+    # new particle k is old particle perm[k]
+    assert np.array_equal(n1, n0[perm])
+    for k in range(0, js.n, 97):
+        cnt = n1[k]
+        got = l1[k, :cnt]
+        assert np.all(np.diff(got) > 0)
+        assert np.array_equal(np.sort(perm[got]), l0[perm[k], :cnt])
+    err = force_errors(a1, j1, a0[perm], j0[perm])
+    say(f"[permuted] a_rel={err['a_rel']} jdot_rel={err['jdot_rel']}")
+    # two FP32-term evaluations, each within ~5e-7 of FP64 for a
+    assert err["a_rel"]["p99"] <= 1e-6 and err["a_rel"]["max"] <= 2e-6
+    assert err["jdot_rel"]["p99"] <= 1e-6
+
+
 @pytest.mark.parametrize("seed", [1, 2])
 def test_king_262144_checked_every_target(ctx, seed):
     """Checked precision on the full King workload (two realisations): every
