@@ -1,7 +1,11 @@
-"""Build the HIP library in-tree: abyss_amd/libabyss_regf.so (gfx950 only).
+"""Build the HIP libraries in-tree (gfx950 only):
+  abyss_amd/libabyss_regf.so      legacy entry points with double (CUDA_REAL = double)
+  abyss_amd/libabyss_regf_f32.so  the same library with float legacy entry points
+                                  (regf_legacy.hip built with -DABYSS_LEGACY_FLOAT)
 
 Every HIP source is compiled with packed-FP32 instruction selection turned
-off (-Xclang -target-feature -Xclang -packed-fp32-ops).
+off (-Xclang -target-feature -Xclang -packed-fp32-ops).  Both libraries bind
+their own symbols first (-Bsymbolic), so the two can be loaded side by side.
 """
 import os
 import subprocess
@@ -11,8 +15,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libabyss_regf.so")
+LIB_F32 = os.path.join(HERE, "libabyss_regf_f32.so")
 SOURCES = ["regf_kernels.hip", "regf_runtime.hip", "regf_ensemble.hip", "regf_autotune.hip",
            "regf_state.hip", "regf_irregular.hip", "regf_derivs.hip"]
+# (object name, extra flags) of the legacy entry points, one per library
+LEGACY = [("regf_legacy.o", []), ("regf_legacy_f32.o", ["-DABYSS_LEGACY_FLOAT"])]
 ARCH = os.environ.get("ABYSS_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
@@ -33,23 +40,28 @@ def build(verbose: bool = False, extra=()) -> str:
     objs = []
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
+    jobs = [(os.path.join(CSRC, s), os.path.join(bdir, s.replace(".hip", ".o")), [])
+            for s in SOURCES]
+    jobs += [(os.path.join(CSRC, "regf_legacy.hip"), os.path.join(bdir, o), f)
+             for o, f in LEGACY]
     procs = []
-    for s in SOURCES:
-        o = os.path.join(bdir, s.replace(".hip", ".o"))
-        cmd = [_hipcc(), *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, s), "-o", o]
+    for src, o, flags in jobs:
+        cmd = [_hipcc(), *HIP_FLAGS, *extra, *flags, "-c", src, "-o", o]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((subprocess.Popen(cmd), cmd))
-        objs.append(o)
     for p, cmd in procs:
         if p.wait() != 0:
             raise RuntimeError("build failed: " + " ".join(cmd))
-    tmp = LIB + ".tmp"
-    cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.check_call(cmd)
-    os.replace(tmp, LIB)
+    objs = [o for _, o, _ in jobs[:len(SOURCES)]]
+    for lib, (legacy_obj, _) in ((LIB, LEGACY[0]), (LIB_F32, LEGACY[1])):
+        tmp = lib + ".tmp"
+        cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-Wl,-Bsymbolic",
+               "-o", tmp, *objs, os.path.join(bdir, legacy_obj)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.check_call(cmd)
+        os.replace(tmp, lib)
     return LIB
 
 

@@ -11,7 +11,7 @@ struct abyss_ctx_s {
     bool own_stream = true;   // false: the stream belongs to an ensemble
     int cu_count = 0;
     int cap = ABYSS_NUM_NEIGHBOR_MAX;
-    int precision = ABYSS_PREC_SPLIT;
+    int precision = ABYSS_PREC_CHECKED;
     int force_ipl = 0, force_splits = 0, force_unroll = 0, force_jl = 0;
     bool batch_member = false;   // an ensemble's cluster: lane-group plans only
     unsigned long long scratch_limit = 2ull << 30;   // split-j scratch bound

@@ -67,7 +67,7 @@ struct abyss_ens_s {
 
     // checked precision: per-descriptor flagged counts and the prefix of the
     // targets each descriptor checks ([max_clusters (+1)])
-    bool checked = false;
+    bool checked = true;   // ABYSS_PREC_CHECKED unless set otherwise
     double check_tol = 1.0e-6;
     int *d_ckcount = nullptr, *h_ckcount = nullptr;
     int *d_ck_off = nullptr, *h_ck_off = nullptr;

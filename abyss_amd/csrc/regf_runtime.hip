@@ -598,6 +598,15 @@ int abyss_ctx_upload(abyss_ctx_t c, int n, const double *m, const double *x, con
                      const double *r2)
 {
     if (!c) return ABYSS_ERR_INVALID_ARG;
+    // A rejected upload leaves no j-set: queries fail with ABYSS_ERR_NO_JSET
+    // until an upload succeeds, instead of silently using the previous one.
+    const int old_n = c->n, old_nt = c->nt;
+    const bool old_plan = c->plan_valid;
+    c->n = 0;
+    c->nt = 0;
+    c->plan_valid = false;
+    c->state_n = 0;
+    c->jset_version += 1;
     if (n <= 0 || !m || !x || !v || !r2)
         return set_detail(c, ABYSS_ERR_INVALID_ARG, "upload: n=%d or null array", n);
     // Validate everything before touching the device.
@@ -616,13 +625,11 @@ int abyss_ctx_upload(abyss_ctx_t c, int n, const double *m, const double *x, con
     HIPCHK(c, hipSetDevice(c->device));
     // The previous upload / query may still read the staging buffer.
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->state_n = 0;   // a device-resident state no longer matches the j-set
     const size_t need = jset_bytes(n);
     if (need > c->j_bytes) {
         dfree(c->d_j);
         hfree(c->h_j);
         c->j_bytes = 0;
-        c->n = 0;
         const size_t cap_b = grow(need);
         HIPCHK(c, hipMalloc(&c->d_j, cap_b));
         HIPCHK(c, hipHostMalloc(&c->h_j, cap_b, hipHostMallocDefault));
@@ -646,12 +653,12 @@ int abyss_ctx_upload(abyss_ctx_t c, int n, const double *m, const double *x, con
     HIPCHK(c, hipMemcpyAsync(c->d_j, c->h_j, need, hipMemcpyHostToDevice, c->stream));
     bind_jstate(c, c->d_j, n);
     c->jset_version += 1;
-    if (c->n != n) {
-        // targets set for another j-set may be out of range for this one
-        c->plan_valid = false;
-        c->nt = 0;
-    }
     c->n = n;
+    if (old_n == n) {
+        // the same size right after a good j-set: its targets stay set
+        c->nt = old_nt;
+        c->plan_valid = old_plan;
+    }
     c->n_send += 1;
     return ABYSS_OK;
 }
@@ -889,162 +896,6 @@ int abyss_ctx_query(abyss_ctx_t c, int nt, const int *targets, double *acc, doub
     rc = abyss_ctx_launch(c);
     if (rc != ABYSS_OK) return rc;
     return abyss_ctx_fetch(c, acc, adot, nnb, list);
-}
-
-}  // extern "C"
-
-// ---------------------------------------------------------------------------
-// Legacy entry points: one process-wide default context.
-// ---------------------------------------------------------------------------
-namespace {
-abyss_ctx_t g_ctx = nullptr;
-bool g_open = false;
-int g_cap = ABYSS_NUM_NEIGHBOR_MAX;
-int g_prec = ABYSS_PREC_SPLIT;
-int g_last = ABYSS_OK;
-char g_detail[512] = {0};
-char g_init_detail[256] = {0};
-
-void legacy_status(int rc)
-{
-    g_last = rc;
-    if (rc == ABYSS_OK) {
-        g_detail[0] = 0;
-        return;
-    }
-    const char *d = g_ctx ? g_ctx->detail : g_init_detail;
-    snprintf(g_detail, sizeof(g_detail), "%s%s%s", abyss_error_string(rc), d[0] ? ": " : "", d);
-    fprintf(stderr, "abyss_regf: %s\n", g_detail);
-}
-
-int legacy_init(int irank)
-{
-    if (g_ctx) return ABYSS_OK;
-    const int ndev = abyss_device_count();
-    if (ndev <= 0) {
-        snprintf(g_init_detail, sizeof(g_init_detail), "no HIP device visible");
-        return ABYSS_ERR_NO_DEVICE;
-    }
-    const int dev = ((irank % ndev) + ndev) % ndev;
-    int rc = abyss_ctx_create(dev, g_cap, &g_ctx);
-    if (rc != ABYSS_OK) {
-        snprintf(g_init_detail, sizeof(g_init_detail), "context creation on device %d failed",
-                 dev);
-        g_ctx = nullptr;
-        return rc;
-    }
-    g_init_detail[0] = 0;
-    g_ctx->timed = true;
-    abyss_ctx_set_precision(g_ctx, g_prec);
-    return ABYSS_OK;
-}
-}  // namespace
-
-extern "C" {
-
-int abyss_last_error(void) { return g_last; }
-const char *abyss_last_error_detail(void) { return g_detail; }
-int abyss_num_neighbor_max(void) { return g_cap; }
-
-int abyss_set_num_neighbor_max(int cap)
-{
-    if (cap <= 0) return ABYSS_ERR_INVALID_ARG;
-    if (g_ctx) return ABYSS_ERR_INVALID_ARG;  // only while closed
-    g_cap = cap;
-    return ABYSS_OK;
-}
-
-int abyss_set_precision(int mode)
-{
-    if (mode < ABYSS_PREC_SPLIT || mode > ABYSS_PREC_CHECKED) return ABYSS_ERR_INVALID_ARG;
-    g_prec = mode;
-    if (g_ctx) abyss_ctx_set_precision(g_ctx, mode);
-    return ABYSS_OK;
-}
-
-void InitializeDevice(int *irank)
-{
-    const int r = irank ? *irank : 0;
-    int rc = legacy_init(r);
-    if (rc == ABYSS_OK) g_open = true;  // src/cuda/cuda_my_acceleration.cu:454
-    legacy_status(rc);
-}
-
-void OpenDevice(const int *irank)
-{
-    if (g_open) {
-        fprintf(stderr, "abyss_regf: it is already open\n");
-        legacy_status(ABYSS_OK);
-        return;
-    }
-    int rc = legacy_init(irank ? *irank : 0);
-    if (rc == ABYSS_OK) g_open = true;
-    legacy_status(rc);
-}
-
-void CloseDevice(void)
-{
-    if (!g_open) {
-        fprintf(stderr, "abyss_regf: it is already closed\n");
-        legacy_status(ABYSS_OK);
-        return;
-    }
-    if (g_ctx) abyss_ctx_destroy(g_ctx);
-    g_ctx = nullptr;
-    g_open = false;
-    legacy_status(ABYSS_OK);
-}
-
-void ProfileDevice(int *irank)
-{
-    if (!g_ctx) {
-        legacy_status(ABYSS_ERR_NOT_OPEN);
-        return;
-    }
-    abyss_ctx_t c = g_ctx;
-    if (c->n_grav) {
-        const double gflops = c->t_grav_ms > 0 ? 60e-9 * c->interactions / (c->t_grav_ms * 1e-3)
-                                               : 0.0;
-        fprintf(stderr,
-                "[R.%d-D.%d GPU Reg.F ] Nsend %lld  Ngrav %lld  <Ni> %lld  grav(s) %f  "
-                "interactions %.6e  Perf.(Gflops) %f\n",
-                irank ? *irank : 0, c->device, c->n_send, c->n_grav,
-                c->n_send ? c->sum_ni / std::max(1LL, c->n_send) : 0LL, c->t_grav_ms * 1e-3,
-                c->interactions, gflops);
-    }
-    c->n_send = c->n_grav = c->sum_ni = 0;
-    c->interactions = 0.0;
-    c->t_grav_ms = 0.0;
-    legacy_status(ABYSS_OK);
-}
-
-void SendToDevice(int *NNB, double m[], double x[][3], double v[][3], double r2[], double mdot[])
-{
-    (void)mdot;  // ignored, as in the reference
-    if (!g_open || !g_ctx) {
-        legacy_status(ABYSS_ERR_NOT_OPEN);
-        return;
-    }
-    if (!NNB) {
-        legacy_status(set_detail(g_ctx, ABYSS_ERR_INVALID_ARG, "NNB is null"));
-        return;
-    }
-    legacy_status(abyss_ctx_upload(g_ctx, *NNB, m, &x[0][0], &v[0][0], r2));
-}
-
-void CalculateAccelerationOnDevice(int *NumTarget, int *h_target_list, double acc[][3],
-                                   double adot[][3], int NumNeighbor[], int *NeighborList)
-{
-    if (!g_open || !g_ctx) {
-        legacy_status(ABYSS_ERR_NOT_OPEN);
-        return;
-    }
-    if (!NumTarget) {
-        legacy_status(set_detail(g_ctx, ABYSS_ERR_INVALID_ARG, "NumTarget is null"));
-        return;
-    }
-    legacy_status(abyss_ctx_query(g_ctx, *NumTarget, h_target_list, &acc[0][0], &adot[0][0],
-                                  NumNeighbor, NeighborList));
 }
 
 }  // extern "C"

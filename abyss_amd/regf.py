@@ -36,6 +36,12 @@ PREC_FP64 = 1
 PREC_MIXED = 2
 PREC_REFINED = 3
 PREC_CHECKED = 4
+# the library's default precision (contexts, ensembles, servers and the
+# legacy entry points): every target's a and adot within 1e-6 of FP64
+PREC_DEFAULT = PREC_CHECKED
+# abyss_set_legacy_errors modes
+LEGACY_EXIT = 0
+LEGACY_RETURN = 1
 # ABYSS_CHECK_ERR_SCALE: the checked precision recomputes a target when
 # CHECK_ERR_SCALE * spread > tol * |a| (or |adot|) for either spread
 CHECK_ERR_SCALE = 1.0e-6
@@ -64,6 +70,7 @@ EXPORTS = [
     "abyss_srv_queue_stats", "abyss_ctx_set_check_tolerance", "abyss_ctx_last_recheck",
     "abyss_ctx_fetch_spread", "abyss_ens_set_precision", "abyss_ens_set_check_tolerance",
     "abyss_ens_last_recheck", "abyss_srv_set_precision", "abyss_srv_set_check_tolerance",
+    "abyss_set_legacy_errors",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -100,6 +107,7 @@ def load():
         "abyss_num_neighbor_max": (ctypes.c_int, []),
         "abyss_set_num_neighbor_max": (ctypes.c_int, [ctypes.c_int]),
         "abyss_set_precision": (ctypes.c_int, [ctypes.c_int]),
+        "abyss_set_legacy_errors": (ctypes.c_int, [ctypes.c_int]),
         "abyss_device_count": (ctypes.c_int, []),
         "abyss_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
         "abyss_ctx_destroy": (ctypes.c_int, [_vp]),
@@ -179,6 +187,9 @@ def load():
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args
+    # The C default ends the process on a failing legacy call, as the
+    # reference does; this mirror raises AbyssError instead.
+    lib.abyss_set_legacy_errors(LEGACY_RETURN)
     _lib = lib
     return lib
 
@@ -266,6 +277,15 @@ def set_precision(mode):
         raise AbyssError(rc)
 
 
+def set_legacy_errors(mode):
+    """LEGACY_EXIT: a failing legacy call ends the process (the C default);
+    LEGACY_RETURN: it records the status (this module's default, which
+    turns it into AbyssError)."""
+    rc = load().abyss_set_legacy_errors(int(mode))
+    if rc != OK:
+        raise AbyssError(rc)
+
+
 def CalculateAccelerationOnDevice(targets, out=None):
     """Returns (acc[nt,3], adot[nt,3], NumNeighbor[nt], NeighborList[nt, NumNeighborMax]).
     `out`: optional preallocated (acc, adot, nnb, lst) of at least those
@@ -317,7 +337,9 @@ def autotune_clear(device=0):
 class Context:
     """One device-resident j-set (a cluster) with its own stream and buffers."""
 
-    def __init__(self, device=0, cap=256, precision=PREC_SPLIT):
+    def __init__(self, device=0, cap=256, precision=None):
+        """precision: an ABYSS_PREC_* mode, or None for the library default
+        (PREC_DEFAULT)."""
         self._lib = load()
         h = _vp()
         rc = self._lib.abyss_ctx_create(device, cap, ctypes.byref(h))
@@ -327,7 +349,7 @@ class Context:
         self.cap = cap
         self.n = 0
         self.nt = 0
-        if precision != PREC_SPLIT:
+        if precision is not None:
             self.set_precision(precision)
 
     def _chk(self, rc, allow_overflow=False):
@@ -556,7 +578,7 @@ class Ensemble:
             pass
 
     def set_precision(self, mode):
-        """PREC_SPLIT (default) or PREC_CHECKED for every cluster."""
+        """PREC_CHECKED (default) or PREC_SPLIT for every cluster."""
         self._chk(self._lib.abyss_ens_set_precision(self.h, int(mode)))
 
     def set_check_tolerance(self, tol):
@@ -642,7 +664,7 @@ class Server:
         self.cap = cap
 
     def set_precision(self, mode):
-        """PREC_SPLIT (default) or PREC_CHECKED, from the next batch on."""
+        """PREC_CHECKED (default) or PREC_SPLIT, from the next batch on."""
         rc = self._lib.abyss_srv_set_precision(self.h, int(mode))
         if rc != OK:
             raise AbyssError(rc)

@@ -26,10 +26,12 @@
  *     status is ABYSS_ERR_NEIGHBOR_OVERFLOW (the reference asserts instead,
  *     src_mpi/cuda/cuda_kernels.cu:363,432).
  *
- * Precision (default ABYSS_PREC_SPLIT): positions and velocities are FP64 on
- * the device and the i-j differences are taken in FP64; r^2, rsqrt and the
- * force/jerk terms are FP32, summed in FP32 over short runs of j and folded
- * into FP64 per-particle sums.  ABYSS_PREC_MIXED keeps r^2, the neighbour
+ * Precision (default ABYSS_PREC_CHECKED, every target's a and adot within
+ * 1e-6 relative of FP64).  ABYSS_PREC_SPLIT: positions and velocities are
+ * FP64 on the device and the i-j differences are taken in FP64; r^2, rsqrt
+ * and the force/jerk terms are FP32, summed in FP32 over short runs of j and
+ * folded into FP64 per-particle sums (a target whose jerk sum cancels can
+ * miss 1e-6).  ABYSS_PREC_MIXED keeps r^2, the neighbour
  * test and the terms in FP64 as well (sums in FP64); only rsqrt is the FP32
  * hardware estimate (~1e-7 relative).  ABYSS_PREC_REFINED adds one FP64
  * Newton step to that estimate (~1e-14).  ABYSS_PREC_FP64 evaluates
@@ -88,6 +90,20 @@ extern "C" {
 /* identical list in src/cuda/cuda_functions.h:3-10).                    */
 /* ===================================================================== */
 
+/* Scalar type of the legacy entry points, the reference's CUDA_REAL.
+ * libabyss_regf.so exports them with double (src/defs.h:101; src_mpi with
+ * `#define CUDA_FLOAT` removed, src_mpi/def.h:69).  libabyss_regf_f32.so
+ * exports the same six symbols with float for an src_mpi build that keeps
+ * CUDA_FLOAT (src_mpi/def.h:69-74); a caller of that library compiles with
+ * -DABYSS_LEGACY_FLOAT.  The float library widens the caller's inputs to
+ * FP64 and rounds the FP64 results to float; everything in between is the
+ * FP64-input path. */
+#ifdef ABYSS_LEGACY_FLOAT
+typedef float abyss_legacy_real;
+#else
+typedef double abyss_legacy_real;
+#endif
+
 /* Selects device (*irank mod device count), creates the stream and marks
  * the library open.  Replaces _InitializeDevice
  * (src/cuda/cuda_my_acceleration.cu:415-455, which also sets is_open). */
@@ -107,10 +123,12 @@ void ProfileDevice(int *irank);
 
 /* Replaces the whole j-set: N = *NNB particles, masses m[N], positions
  * x[N][3], velocities v[N][3], r2[N] = r_nb^2 per particle; mdot is ignored.
- * Inputs are copied before return.  Replaces _ReceiveFromHost
+ * Inputs are copied before return.  A rejected upload (for example a
+ * non-finite position) leaves no j-set: later queries fail with
+ * ABYSS_ERR_NO_JSET until an upload succeeds.  Replaces _ReceiveFromHost
  * (src_mpi/cuda/cuda_my_acceleration.cu:603-778; src/...:313-411). */
-void SendToDevice(int *NNB, double m[], double x[][3], double v[][3],
-                  double r2[], double mdot[]);
+void SendToDevice(int *NNB, abyss_legacy_real m[], abyss_legacy_real x[][3],
+                  abyss_legacy_real v[][3], abyss_legacy_real r2[], abyss_legacy_real mdot[]);
 
 /* Regular acc/adot and neighbour lists of *NumTarget targets
  * h_target_list[k] in [0, N), 0 < *NumTarget <= N.  Outputs for k <
@@ -118,7 +136,7 @@ void SendToDevice(int *NNB, double m[], double x[][3], double v[][3],
  * NeighborList[k*NumNeighborMax + 0..count).  Replaces GetAcceleration
  * (src/cuda/cuda_my_acceleration.cu:66-302; src_mpi/...:95-344). */
 void CalculateAccelerationOnDevice(int *NumTarget, int *h_target_list,
-                                   double acc[][3], double adot[][3],
+                                   abyss_legacy_real acc[][3], abyss_legacy_real adot[][3],
                                    int NumNeighbor[], int *NeighborList);
 
 /* ===================================================================== */
@@ -134,14 +152,27 @@ const char *abyss_last_error_detail(void);
 /* Legacy list stride; setter only allowed while the library is closed. */
 int abyss_num_neighbor_max(void);
 int abyss_set_num_neighbor_max(int cap);
-/* Legacy API precision: any ABYSS_PREC_* mode (default ABYSS_PREC_SPLIT). */
+/* Legacy API precision: any ABYSS_PREC_* mode (default ABYSS_PREC_CHECKED,
+ * which keeps every target's a and adot within 1e-6 relative of FP64). */
 int abyss_set_precision(int mode);
+/* What a failing legacy call does.  ABYSS_LEGACY_EXIT (the default) prints
+ * the error to stderr and ends the process with exit(1), as the reference
+ * does on neighbour overflow (src/cuda/cuda_my_acceleration.cu:221-226,
+ * src_mpi/cuda/cuda_my_acceleration.cu:328) and through its asserts
+ * (:103-104); its callers never check a status.  ABYSS_LEGACY_RETURN
+ * prints, records the status for abyss_last_error() and returns; an
+ * overflowed list then comes back truncated with NumNeighbor[k] =
+ * -(true count). */
+#define ABYSS_LEGACY_EXIT   0
+#define ABYSS_LEGACY_RETURN 1
+int abyss_set_legacy_errors(int mode);
 /* Number of HIP devices visible (0 without a GPU; never fails). */
 int abyss_device_count(void);
 
 typedef struct abyss_ctx_s *abyss_ctx_t;
 
-/* Creates a context on `device` with neighbour-list stride `cap` (>0). */
+/* Creates a context on `device` with neighbour-list stride `cap` (>0),
+ * precision ABYSS_PREC_CHECKED. */
 int abyss_ctx_create(int device, int cap, abyss_ctx_t *out);
 int abyss_ctx_destroy(abyss_ctx_t ctx);
 int abyss_ctx_set_precision(abyss_ctx_t ctx, int mode);
@@ -339,8 +370,8 @@ int abyss_ens_set_block_range(abyss_ens_t ens, int cluster, int first, int count
 int abyss_ens_fetch(abyss_ens_t ens, int cluster, double *acc, double *adot, int *nnb,
                     int *list);
 void *abyss_ens_stream(abyss_ens_t ens);
-/* Precision of every cluster: ABYSS_PREC_SPLIT (default) or
- * ABYSS_PREC_CHECKED (the flag rule of abyss_ctx_set_check_tolerance, over
+/* Precision of every cluster: ABYSS_PREC_CHECKED (default) or
+ * ABYSS_PREC_SPLIT (ABYSS_PREC_CHECKED: the flag rule of abyss_ctx_set_check_tolerance, over
  * every target a launch evaluates).  Under ABYSS_PREC_CHECKED a launch
  * waits once on the host after the batch (for the flagged counts), then
  * queues each cluster's recomputation; last_recheck reports a cluster's

@@ -1,8 +1,7 @@
 """Builds the test harnesses (tests/harness/ac_hermite: shared-step Hermite
 over the legacy ABI; tests/harness/ac_full: two-time-scale Ahmad-Cohen over
 the device-resident entry points) against the HIP library and the oracle
-library, both in-tree, and the VALU issue-cost microbenchmark
-(tests/harness/valu_issue)."""
+library, both in-tree."""
 import os
 import subprocess
 
@@ -24,14 +23,6 @@ def build(verbose=False):
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd)
         os.replace(out + ".tmp", out)
-    # VALU issue-cost microbenchmark (gfx950)
-    out = os.path.join(HERE, "valu_issue")
-    cmd = ["/opt/rocm/bin/hipcc", "-O3", "--offload-arch=gfx950", "-Xclang", "-target-feature",
-           "-Xclang", "-packed-fp32-ops", os.path.join(HERE, "valu_issue.hip"), "-o", out + ".tmp"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.check_call(cmd, stderr=subprocess.DEVNULL)
-    os.replace(out + ".tmp", out)
     return OUT
 
 

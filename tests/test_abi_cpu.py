@@ -25,11 +25,17 @@ def test_header_lists_match_python_binding():
     assert sorted(declared_functions()) == sorted(regf.EXPORTS)
 
 
-def test_library_exports_every_declared_symbol():
-    lib = regf.load()
+LIB_F32 = regf.LIB_PATH.replace("libabyss_regf.so", "libabyss_regf_f32.so")
+
+
+@pytest.mark.parametrize("path", [regf.LIB_PATH, LIB_F32])
+def test_library_exports_every_declared_symbol(path):
+    """Both builds (FP64 and FP32 legacy scalar) export the whole header."""
+    assert os.path.exists(path), path
+    lib = regf.load() if path == regf.LIB_PATH else ctypes.CDLL(path)
     for name in declared_functions():
         assert hasattr(lib, name), name
-    out = subprocess.run(["nm", "-D", "--defined-only", regf.LIB_PATH], capture_output=True,
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True,
                          text=True, check=True).stdout
     exported = set(line.split()[-1] for line in out.splitlines() if line.strip())
     for name in declared_functions():
@@ -101,6 +107,7 @@ int main(void) {
     double acc[1][3], adot[1][3];
     int list[ABYSS_NUM_NEIGHBOR_MAX];
     if (abyss_device_count() > 0) return 0;
+    if (abyss_set_legacy_errors(ABYSS_LEGACY_RETURN) != ABYSS_OK) return 4;
     OpenDevice(&rank);
     if (abyss_last_error() != ABYSS_ERR_NO_DEVICE) return 1;
     SendToDevice(&n, m, x, v, r2, NULL);
@@ -117,3 +124,43 @@ int main(void) {
                     "-labyss_regf", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+
+
+@pytest.mark.parametrize("lib_name,flag", [("abyss_regf", ""), ("abyss_regf_f32",
+                                                                "-DABYSS_LEGACY_FLOAT")])
+def test_c_caller_stops_on_error_by_default(tmp_path, lib_name, flag):
+    """The reference's callers never check a status; by default a failing
+    legacy call prints its error and ends the process with exit(1), as the
+    reference does (exit on overflow, asserts on a closed device)."""
+    src = tmp_path / "caller.c"
+    src.write_text(r'''
+#include <stdio.h>
+#include "abyss_regf.h"
+int main(void) {
+    int nt = 1, idx[1] = {0}, nnb[1];
+    abyss_legacy_real acc[1][3], adot[1][3];
+    int list[ABYSS_NUM_NEIGHBOR_MAX];
+    printf("before\\n");
+    fflush(stdout);
+    CalculateAccelerationOnDevice(&nt, idx, acc, adot, nnb, list);   /* not open */
+    printf("after\\n");
+    return 0;
+}
+''')
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(regf.LIB_PATH)
+    cmd = ["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-L", libdir, f"-l{lib_name}",
+           f"-Wl,-rpath,{libdir}", "-o", str(exe)]
+    if flag:
+        cmd.insert(1, flag)
+    subprocess.run(cmd, check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 1, (r.returncode, r.stdout, r.stderr)
+    assert "before" in r.stdout and "after" not in r.stdout
+    assert "device not open" in r.stderr
+
+
+def test_legacy_error_mode_setter_validates():
+    lib = regf.load()
+    assert lib.abyss_set_legacy_errors(7) == regf.ERR_INVALID_ARG
+    assert lib.abyss_set_legacy_errors(regf.LEGACY_RETURN) == regf.OK
