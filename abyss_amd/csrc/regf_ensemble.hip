@@ -51,6 +51,10 @@ struct abyss_ens_s {
     std::vector<abyss_ctx_t> cl;     // nullptr: free slot
     std::vector<char> active;        // has targets for the next launch
     std::vector<int> blk_first, blk_count;   // target-block range (count < 0: all)
+    // checked precision: target blocks of the current targets evaluated
+    // since the cluster's flagged count was last reset; a pass ends (and
+    // its flagged targets are recomputed) once every block is done
+    std::vector<int> chk_done;
 
     ClusterDesc *d_cd = nullptr, *h_cd = nullptr;
     WorkItem *d_items = nullptr, *h_items = nullptr;
@@ -183,6 +187,7 @@ int abyss_ens_create(int device, int cap, int max_clusters, abyss_ens_t *out)
     e->active.assign(max_clusters, 0);
     e->blk_first.assign(max_clusters, 0);
     e->blk_count.assign(max_clusters, -1);
+    e->chk_done.assign(max_clusters, 0);
     int rc = ABYSS_OK;
     do {
         if (hipSetDevice(device) != hipSuccess) { rc = ABYSS_ERR_NO_DEVICE; break; }
@@ -266,6 +271,7 @@ int abyss_ens_set_precision(abyss_ens_t e, int mode)
         ENSCHK(e, hipStreamSynchronize(e->stream));
         drop_graph(e);   // the captured batch kernel depends on the mode
         e->checked = checked;
+        e->chk_done.assign(e->max_clusters, 0);   // passes start afresh
     }
     return ABYSS_OK;
 }
@@ -312,6 +318,7 @@ int abyss_ens_upload(abyss_ens_t e, int cluster, int n, const double *m, const d
         c->batch_member = true;
     }
     e->active[cluster] = 0;
+    e->chk_done[cluster] = 0;
     return from_ctx(e, c, abyss_ctx_upload(c, n, m, x, v, r2));
 }
 
@@ -339,6 +346,7 @@ int abyss_ens_set_targets(abyss_ens_t e, int cluster, int nt, const int *targets
     }
     int rc = abyss_ctx_set_targets(c, nt, targets);
     e->active[cluster] = rc == ABYSS_OK ? 1 : 0;
+    e->chk_done[cluster] = 0;
     e->blk_first[cluster] = 0;
     e->blk_count[cluster] = -1;
     return from_ctx(e, c, rc);
@@ -446,7 +454,10 @@ int abyss_ens_launch(abyss_ens_t e)
         d.spread = e->checked ? c->chk_spread : nullptr;
         d.fq = c->chk_fq;
         d.fi = c->chk_fi;
-        d.count = e->d_ckcount + a;
+        // the cluster's own flagged count: a request whose target blocks
+        // span several batches accumulates its flagged targets over them
+        d.count = e->d_ckcount + en.cluster;
+        d.chk_reset = e->chk_done[en.cluster] == 0 ? 1 : 0;
         d.chk_q0 = en.first * tpb;
         d.chk_nq = en.nt_slice;
         e->h_ck_off[a] = nck;
@@ -503,7 +514,7 @@ int abyss_ens_launch(abyss_ens_t e)
     if (e->checked) {
         ENSCHK(e, hipMemcpyAsync(e->d_ck_off, e->h_ck_off, (act.size() + 1) * sizeof(int),
                                  hipMemcpyHostToDevice, e->stream));
-        ENSCHK(e, abyss::launch_zero_ints(e->d_ckcount, (int)act.size(), e->stream));
+        ENSCHK(e, abyss::launch_batch_count_reset(e->d_cd, (int)act.size(), e->stream));
     }
     if (!e->exec) {
         rc = capture_graph(e);
@@ -517,13 +528,15 @@ int abyss_ens_launch(abyss_ens_t e)
         const double ratio = ABYSS_CHECK_ERR_SCALE / e->check_tol;
         ENSCHK(e, abyss::launch_batch_check_flag(e->d_cd, e->d_ck_off, (int)act.size(), nck,
                                                  ratio * ratio, e->stream));
-        ENSCHK(e, hipMemcpyAsync(e->h_ckcount, e->d_ckcount, act.size() * sizeof(int),
+        ENSCHK(e, hipMemcpyAsync(e->h_ckcount, e->d_ckcount, e->max_clusters * sizeof(int),
                                  hipMemcpyDeviceToHost, e->stream));
         ENSCHK(e, hipStreamSynchronize(e->stream));
-        // The recomputation of every cluster's flagged targets in one
-        // persistent launch.  A cluster's items follow the rule above for
-        // its own flagged count and N, so its results do not depend on the
-        // rest of the batch.  The batch has finished: its descriptor, item,
+        // The recomputation of the flagged targets of every cluster whose
+        // target blocks this batch completes, in one persistent launch.  A
+        // cluster's items follow the rule above for its total flagged count
+        // (over all the batches of its blocks) and N, so its results do not
+        // depend on the rest of the batch nor on how its blocks were split
+        // over batches.  The batch has finished: its descriptor, item,
         // header, prefix and pool buffers can be refilled.
         struct Rec {
             int a, m, jl, nb, splits, jspan;
@@ -532,13 +545,19 @@ int abyss_ens_launch(abyss_ens_t e)
         size_t nitems2 = 0, pool2 = 0;
         for (size_t a = 0; a < act.size(); ++a) {
             abyss_ctx_t c = e->cl[act[a].cluster];
-            const int m = e->h_ckcount[a];
-            c->n_rechecked = m;
             c->spread_valid = true;
+            int &done = e->chk_done[act[a].cluster];
+            done += act[a].nblocks;
+            if (done < c->plan.grid_x) {
+                c->n_rechecked = 0;   // more blocks of this pass to come
+                continue;
+            }
+            done = 0;                 // the pass is complete
+            const int m = e->h_ckcount[act[a].cluster];
+            c->n_rechecked = m;
             if (m <= 0) continue;
-            if (m > act[a].nt_slice)
-                return ens_detail(e, ABYSS_ERR_HIP, "check: %d flagged of %d", m,
-                                  act[a].nt_slice);
+            if (m > c->nt)
+                return ens_detail(e, ABYSS_ERR_HIP, "check: %d flagged of %d", m, c->nt);
             const int jl = m >= 256 ? 4 : (m >= 16 ? 16 : 64);
             const int tpb = 4 * (64 / jl);
             const int tile = abyss::kTile;

@@ -89,6 +89,7 @@ struct ClusterDesc {
     float *spread;
     int *fq, *fi, *count;
     int chk_q0, chk_nq;
+    int chk_reset;   // zero *count before this batch's flags (a new pass)
 };
 
 // (cluster, target block, j-split): one workgroup's worth of work.
@@ -115,6 +116,10 @@ hipError_t launch_batch_reset(int *counters, hipStream_t st);
 int batch_blocks_per_cu();
 hipError_t launch_batch(const ClusterDesc *cd, const WorkItem *items, const BatchHeader *hdr,
                         int *counter, int cap, int grid, bool est, hipStream_t st);
+// Checked precision of a batch: zeroes *cd[a].count of every descriptor
+// with chk_reset set (the first batch of a pass over the cluster's target
+// blocks); the counts of later batches of the pass keep accumulating.
+hipError_t launch_batch_count_reset(const ClusterDesc *cd, int ncd, hipStream_t st);
 // Checked precision of a batch: over the chk_nq targets of each of the ncd
 // descriptors (ck_off[k]: prefix of their counts, ck_off[ncd] = total),
 // appends the flagged ones to the descriptor's fq / fi and counts them in

@@ -235,6 +235,13 @@ __device__ __forceinline__ void regf_body(const JState &js, const int *__restric
     const int ntile = (j1 - j0 + kTile - 1) / kTile;
     TileLoader<MT> ld;
     static_assert(kTile == kBlock, "one j per thread per tile load");
+    // A wave none of whose lanes holds a target (the tail of a small block)
+    // still loads its share of every tile and meets every barrier, but skips
+    // the pair loop.
+    bool any_target = false;
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) any_target |= T.idx[k] >= 0;
+    const bool wave_busy = __any(any_target);
 
     if (ntile > 0) {
         ld.load(js, j0 + tid, j1);
@@ -251,7 +258,9 @@ __device__ __forceinline__ void regf_body(const JState &js, const int *__restric
         bool mine = false;
 #pragma unroll
         for (int k = 0; k < IPL; ++k) mine |= (unsigned)(T.idx[k] - jt) < (unsigned)count;
-        if (__any(mine)) {
+        if (!wave_busy) {
+            // no target in this wave: nothing to evaluate
+        } else if (__any(mine)) {
             run_tile<IPL, U, F64, false, true>(T, s_pos[b], s_m[b], count, jt, cap);
         } else if (count == kTile) {
             run_tile<IPL, U, F64, true, false>(T, s_pos[b], s_m[b], count, jt, cap);
@@ -567,6 +576,9 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
     };
 
     const int ntile = (j1 - j0 + kTile - 1) / kTile;
+    // a wave without targets loads its share of every tile and meets every
+    // barrier, but evaluates nothing
+    const bool wave_busy = __any(q < nt);
     TileLoader<MT> ld;
     if (ntile > 0) {
         ld.load(js, j0 + tid, j1);
@@ -578,7 +590,9 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
         if (t + 1 < ntile) ld.load(js, j0 + (t + 1) * kTile + tid, j1);
         const int jt = j0 + t * kTile;
         const int count = min(kTile, j1 - jt);
-        if (__any((unsigned)(L.idx - jt) < (unsigned)count))
+        if (!wave_busy) {
+            // no target in this wave (the tail of a small block)
+        } else if (__any((unsigned)(L.idx - jt) < (unsigned)count))
             jl_tile<JL, TM, false, true>(L, s_pos[b], s_m[b], count, jt);
         else if (count == kTile) {
             if constexpr (TM == 0)
@@ -1038,6 +1052,14 @@ __global__ __launch_bounds__(256) void batch_check_flag_kernel(const ClusterDesc
     }
 }
 
+// Zeroes the flagged count of every descriptor that starts a pass over its
+// cluster's target blocks; later batches of the pass keep appending to it.
+__global__ void batch_count_reset_kernel(const ClusterDesc *__restrict__ cd, int ncd)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < ncd && cd[a].chk_reset) *cd[a].count = 0;
+}
+
 __global__ void zero_ints_kernel(int *__restrict__ p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1209,6 +1231,14 @@ hipError_t launch_batch(const ClusterDesc *cd, const WorkItem *items, const Batc
     else
         hipLaunchKernelGGL(batch_kernel<false>, dim3(grid), dim3(kBlock), 0, st, cd, items, hdr,
                            counter, cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_count_reset(const ClusterDesc *cd, int ncd, hipStream_t st)
+{
+    if (ncd <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_count_reset_kernel, dim3((ncd + 255) / 256), dim3(256), 0, st, cd,
+                       ncd);
     return hipGetLastError();
 }
 

@@ -4,6 +4,10 @@ plummer_equal(n=1024)   equal-mass Plummer, seed 1, ~48 neighbours
 plummer_kroupa(n=65536) Plummer with Kroupa IMF, ~100 neighbours, cap 256
 king_binaries(n=262144) King W0=6, Kroupa IMF, 5% of particles in hard
                         binaries (a ~ log-U[1e-5, 1e-3]), ~100 neighbours
+ensemble_clusters()     256 Plummer clusters, N log-uniform in [1024, 65536],
+                        active fraction uniform in [0.01, 1] (batched ensemble)
+served_clusters()       64 clusters, N log-uniform in [4096, 1048576], Plummer
+                        or King W0 uniform in [3, 9] (served ensemble)
 """
 from __future__ import annotations
 
@@ -73,3 +77,43 @@ def active_blocks(n, count, seed=1, lo=1, hi=None):
     sizes = ic.log_uniform_block_sizes(count, lo, hi, seed=seed)
     rng = np.random.default_rng(seed + 101)
     return [rng.permutation(n)[:s].astype(np.int32) for s in sizes]
+
+
+def ensemble_clusters(count=256, lo=1024, hi=65536, seed=1, k_nb=64):
+    """`count` Plummer clusters, N log-uniform in [lo, hi], distinct seeds,
+    active fraction uniform in [0.01, 1]; Kroupa masses; r_nb for ~k_nb
+    neighbours.  Returns [(x, v, m, r2, targets)]."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(int)
+    fr = rng.uniform(0.01, 1.0, count)
+    out = []
+    for k in range(count):
+        n = int(ns[k])
+        r = np.random.default_rng(seed * 1000 + k)
+        x, v, m = ic.plummer(n, seed=seed * 1000 + k, masses=ic.kroupa_masses(n, r))
+        r2 = ic.neighbour_radius2(x, k_nb)
+        t = r.permutation(n)[:max(1, int(round(fr[k] * n)))].astype(np.int32)
+        out.append((x, v, m, r2, t))
+    return out
+
+
+def served_clusters(count=64, lo=4096, hi=1048576, seed=5, k_nb=32, sizes=None):
+    """`count` clusters, N log-uniform in [lo, hi] (or the given `sizes`),
+    Plummer (even k) or King with W0 uniform in [3, 9] (odd k), Kroupa
+    masses, r_nb for ~k_nb neighbours.  Returns [(x, v, m, r2)]."""
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(int)
+    if sizes is not None:
+        ns = np.asarray(sizes, dtype=int)
+    out = []
+    for k in range(len(ns)):
+        n = int(ns[k])
+        r = np.random.default_rng(seed * 1000 + k)
+        masses = ic.kroupa_masses(n, r)
+        if k % 2:
+            x, v, m = ic.king(n, W0=float(r.uniform(3.0, 9.0)), seed=seed * 1000 + k,
+                              masses=masses)
+        else:
+            x, v, m = ic.plummer(n, seed=seed * 1000 + k, masses=masses)
+        out.append((x, v, m, ic.neighbour_radius2(x, k_nb)))
+    return out

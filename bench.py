@@ -206,55 +206,60 @@ def checked_leg(ctx, targets, inter, reps=5):
     return info, res
 
 
-def ensemble_workload(count=256, lo=1024, hi=65536, seed=1, k_nb=64):
-    """256 Plummer clusters, N log-uniform in [lo, hi], distinct seeds, active
-    fraction uniform in [0.01, 1]; Kroupa masses; r_nb for ~k_nb neighbours."""
-    import numpy as np
-    from abyss_amd import ic
-    rng = np.random.default_rng(seed)
-    ns = np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(int)
-    fr = rng.uniform(0.01, 1.0, count)
-    out = []
-    for k in range(count):
-        n = int(ns[k])
-        r = np.random.default_rng(seed * 1000 + k)
-        x, v, m = ic.plummer(n, seed=seed * 1000 + k, masses=ic.kroupa_masses(n, r))
-        r2 = ic.neighbour_radius2(x, k_nb)
-        t = r.permutation(n)[:max(1, int(round(fr[k] * n)))].astype(np.int32)
-        out.append((x, v, m, r2, t))
-    return out
-
-
 def bench_ensemble(args):
-    """Batched ensemble vs per-cluster launches (device-resident and legacy)."""
+    """Batched ensemble vs per-cluster launches (device-resident and legacy).
+
+    Every leg at the library's default precision (checked: every target
+    within 1e-6 of FP64) unless named split.  The reference's own kernel
+    shape, one target per thread and GridDimY = 64 contiguous j-ranges
+    (src_mpi/def.h:77-79, src_mpi/cuda/cuda_kernels.cu:277-279), exists
+    here only as split precision (one target per lane), so it is compared
+    with the batch in split precision."""
     import numpy as np
     from abyss_amd import regf
     t0 = time.perf_counter()
-    cl = ensemble_workload()
+    from abyss_amd import workloads
+    cl = workloads.ensemble_clusters()
     log(f"ensemble workload: {len(cl)} clusters, sum N = {sum(len(c[2]) for c in cl)}, "
         f"built in {time.perf_counter() - t0:.1f} s")
     inter = float(sum(len(c[4]) * (len(c[2]) - 1) for c in cl))
     cap = 256
     ev = HipEvents()
+
+    def timed_steps(launch, stream, steps):
+        """HIP-event time per step over `steps` launches (the checked
+        precision waits on the host inside each launch; that wait is part
+        of the step)."""
+        e0, e1 = ev.create(), ev.create()
+        ev.record(e0, stream)
+        for _ in range(steps):
+            launch()
+        ev.record(e1, stream)
+        return ev.elapsed_ms(e0, e1) / steps
+
     # (1) batched: one persistent graph launch per step
     ens = regf.Ensemble(device=0, cap=cap, max_clusters=len(cl))
     for k, (x, v, m, r2, t) in enumerate(cl):
         ens.upload(k, m, x, v, r2)
         ens.set_targets(k, t)
-    for _ in range(args.warmup):
-        ens.launch()
-    ens.sync()
-    e0, e1 = ev.create(), ev.create()
-    ev.record(e0, ens.stream)
-    for _ in range(args.steps):
-        ens.launch()
-    ev.record(e1, ens.stream)
-    ms_batched = ev.elapsed_ms(e0, e1) / args.steps
-    ens.sync()
+    ms_batched = {}
+    for prec, name in ((regf.PREC_CHECKED, "checked"), (regf.PREC_SPLIT, "split")):
+        ens.set_precision(prec)
+        for _ in range(args.warmup):
+            ens.launch()
+        ens.sync()
+        ms_batched[name] = timed_steps(ens.launch, ens.stream, args.steps)
+        ens.sync()
+        if name == "checked":
+            recomputed = sum(ens.last_recheck(k) for k in range(len(cl)))
+        log(f"batched ({name}): {ms_batched[name]:.3f} ms/step")
     # results into preallocated host buffers, as the reference caller keeps
     # its AccRegReceive / ACListReceive arrays (both paths below use them)
+    ens.set_precision(regf.PREC_CHECKED)
     bufs = [(np.zeros((len(t), 3)), np.zeros((len(t), 3)), np.zeros(len(t), np.int32),
              np.zeros((len(t), cap), np.int32)) for (_, _, _, _, t) in cl]
+    ens.launch()
+    ens.sync()
     for k in range(len(cl)):           # warm-up: sizes each cluster's pinned staging
         ens.fetch(k, out=bufs[k])
     w0 = time.perf_counter()
@@ -262,31 +267,53 @@ def bench_ensemble(args):
     ens.sync()
     res_b = [ens.fetch(k, out=bufs[k]) for k in range(len(cl))]
     ms_batched_fetch = (time.perf_counter() - w0) * 1e3
-    log(f"batched: {ms_batched:.3f} ms/step on device, {ms_batched_fetch:.1f} ms incl. fetch")
+    log(f"batched (checked) incl. fetch: {ms_batched_fetch:.1f} ms")
     res_b = [tuple(a.copy() for a in r) for r in res_b]   # the buffers are reused below
-    # (1c) the same batch under checked precision (each launch waits once on
-    # the host for the flagged counts): wall clock per launch + sync
-    ens.set_precision(regf.PREC_CHECKED)
-    ens.launch()
-    ens.sync()
-    times = []
-    for _ in range(max(1, args.steps)):
-        w0 = time.perf_counter()
-        ens.launch()
-        ens.sync()
-        times.append((time.perf_counter() - w0) * 1e3)
-    ms_checked = float(np.median(times))
-    recomputed = sum(ens.last_recheck(k) for k in range(len(cl)))
-    log(f"batched, checked precision: {ms_checked:.3f} ms/step, {recomputed} targets recomputed")
     ens.close()
-    # (2) per-cluster device-resident launches, one after another
+    # (2) per-cluster device-resident launches with the library's own plans
+    # (lanes per target, j-splits), one after another
     ctxs = []
     for (x, v, m, r2, t) in cl:
         c = regf.Context(device=0, cap=cap)
         c.upload(m, x, v, r2)
         c.set_targets(t)
         ctxs.append(c)
-    for c in ctxs:
+    ms_planned = {}
+    for prec, name in ((regf.PREC_CHECKED, "checked"), (regf.PREC_SPLIT, "split")):
+        for c in ctxs:
+            c.set_precision(prec)
+            c.launch()
+            c.sync()
+        times = []
+        for _ in range(max(1, args.steps // 2)):
+            w0 = time.perf_counter()
+            for c in ctxs:
+                c.launch()
+                c.sync()
+            times.append((time.perf_counter() - w0) * 1e3)
+        ms_planned[name] = float(np.median(times))
+        log(f"per-cluster planned launches ({name}): {ms_planned[name]:.3f} ms/step")
+        if name == "checked":
+            # equal accuracy: both meet 1e-6 per target; the batch splits j
+            # by work-item size and the planned launch by its own rule, so
+            # values differ at rounding level, counts and lists not at all
+            lists_equal, max_rel = 0, 0.0
+            for c, rb in zip(ctxs, res_b):
+                ra = c.fetch()
+                w = np.arange(cap)[None, :] < np.minimum(np.abs(ra[2]), cap)[:, None]
+                lists_equal += int(np.array_equal(ra[2], rb[2]) and
+                                   bool(np.all((ra[3] == rb[3]) | ~w)))
+                for xx, yy in zip(ra[:2], rb[:2]):
+                    n = np.linalg.norm(xx, axis=1)
+                    d = np.linalg.norm(xx - yy, axis=1)
+                    max_rel = max(max_rel, float(np.max(d / np.where(n > 0, n, 1.0))))
+    # (2b) the reference kernel's shape: one target per lane, 64 contiguous
+    # j-ranges per cluster (GridDimY = 64; fewer where N / 64 is under one
+    # 256-record tile), split precision
+    for c, (_, _, _, _, t) in zip(ctxs, cl):
+        c.set_precision(regf.PREC_SPLIT)
+        c.set_tuning(1, 64, 2, 1)
+        c.set_targets(t)
         c.launch()
         c.sync()
     times = []
@@ -296,33 +323,8 @@ def bench_ensemble(args):
             c.launch()
             c.sync()
         times.append((time.perf_counter() - w0) * 1e3)
-    ms_plain = float(np.median(times))
-    log(f"per-cluster device-resident launches: {ms_plain:.3f} ms/step")
-    # equal accuracy: the batch splits each cluster's j-range by work-item
-    # size, the per-cluster plan by its own rule, so the FP32 partial sums
-    # differ in rounding; counts and lists (plan-independent) must match
-    lists_equal, max_rel = 0, 0.0
-    for c, rb in zip(ctxs, res_b):
-        ra = c.fetch()
-        w = np.arange(cap)[None, :] < np.minimum(np.abs(ra[2]), cap)[:, None]
-        lists_equal += int(np.array_equal(ra[2], rb[2]) and bool(np.all((ra[3] == rb[3]) | ~w)))
-        for x, y in zip(ra[:2], rb[:2]):
-            n = np.linalg.norm(x, axis=1)
-            d = np.linalg.norm(x - y, axis=1)
-            max_rel = max(max_rel, float(np.max(d / np.where(n > 0, n, 1.0))))
-    # (2b) the same launches with the plain kernel shape: one target per
-    # lane, no j-splits (the reference kernel's decomposition), same precision
-    for c, (_, _, _, _, t) in zip(ctxs, cl):
-        c.set_tuning(1, 1, 2, 1)
-        c.set_targets(t)
-        c.launch()
-        c.sync()
-    w0 = time.perf_counter()
-    for c in ctxs:
-        c.launch()
-        c.sync()
-    ms_unplanned = (time.perf_counter() - w0) * 1e3
-    log(f"per-cluster launches, one target per lane, no j-splits: {ms_unplanned:.1f} ms/step")
+    ms_refshape = float(np.median(times))
+    log(f"per-cluster launches in the reference's shape (split): {ms_refshape:.1f} ms/step")
     for c in ctxs:
         c.close()
     # (3) per-cluster legacy ABI calls (upload + query + results on the host)
@@ -338,45 +340,27 @@ def bench_ensemble(args):
     ms_legacy = (time.perf_counter() - w0) * 1e3
     regf.CloseDevice()
     regf.set_num_neighbor_max(1000)
-    log(f"per-cluster legacy calls: {ms_legacy:.1f} ms/step")
+    log(f"per-cluster legacy calls (checked): {ms_legacy:.1f} ms/step")
     return {
         "workload": "ensemble_256_plummer_n1024-65536_active0.01-1",
         "clusters": len(cl), "interactions_per_step": inter, "cap": cap,
-        "batched_ms_per_step": ms_batched,
-        "interactions_per_s": inter / (ms_batched * 1e-3),
-        "per_cluster_device_launches_ms": ms_plain,
-        "per_cluster_one_target_per_lane_unsplit_ms": ms_unplanned,
+        "batched_ms_per_step": ms_batched["checked"],
+        "batched_split_ms_per_step": ms_batched["split"],
+        "interactions_per_s": inter / (ms_batched["checked"] * 1e-3),
+        "interactions_per_s_split": inter / (ms_batched["split"] * 1e-3),
+        "checked_precision_recomputed_targets": recomputed,
+        "per_cluster_planned_launches_ms": ms_planned["checked"],
+        "per_cluster_planned_launches_split_ms": ms_planned["split"],
+        "per_cluster_reference_shape_split_ms": ms_refshape,
         "per_cluster_legacy_calls_ms": ms_legacy,
         "batched_with_fetch_ms": ms_batched_fetch,
-        "speedup_vs_device_launches": ms_plain / ms_batched,
-        "speedup_vs_one_target_per_lane_unsplit": ms_unplanned / ms_batched,
+        "speedup_vs_planned_launches": ms_planned["checked"] / ms_batched["checked"],
+        "speedup_vs_planned_launches_split": ms_planned["split"] / ms_batched["split"],
+        "speedup_vs_reference_shape_split": ms_refshape / ms_batched["split"],
         "speedup_vs_legacy_calls": ms_legacy / ms_batched_fetch,
-        "checked_precision_ms_per_step": ms_checked,
-        "checked_precision_recomputed_targets": recomputed,
         "clusters_with_lists_equal_to_per_cluster_launches": lists_equal,
         "max_rel_diff_vs_per_cluster_launches": max_rel,
     }
-
-
-def served_workload(count=64, lo=4096, hi=1048576, seed=5, k_nb=32):
-    """64 clusters, N log-uniform in [lo, hi], Plummer or King (W0 uniform in
-    [3, 9]), Kroupa masses, r_nb for ~k_nb neighbours."""
-    import numpy as np
-    from abyss_amd import ic
-    rng = np.random.default_rng(seed)
-    ns = np.exp(rng.uniform(np.log(lo), np.log(hi), count)).astype(int)
-    out = []
-    for k in range(count):
-        n = int(ns[k])
-        r = np.random.default_rng(seed * 1000 + k)
-        masses = ic.kroupa_masses(n, r)
-        if k % 2:
-            x, v, m = ic.king(n, W0=float(r.uniform(3.0, 9.0)), seed=seed * 1000 + k,
-                              masses=masses)
-        else:
-            x, v, m = ic.plummer(n, seed=seed * 1000 + k, masses=masses)
-        out.append((x, v, m, ic.neighbour_radius2(x, k_nb)))
-    return out
 
 
 ADMIT_PAIRS = 3.0e11
@@ -571,7 +555,8 @@ def bench_serve(args, rate=2000.0, window_s=1.0):
     import numpy as np
     from abyss_amd import regf
     t0 = time.perf_counter()
-    cl = served_workload()
+    from abyss_amd import workloads
+    cl = workloads.served_clusters()
     sizes = [len(c[2]) for c in cl]
     log(f"served workload: {len(cl)} clusters, sum N = {sum(sizes)}, built in "
         f"{time.perf_counter() - t0:.1f} s")

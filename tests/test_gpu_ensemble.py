@@ -1,12 +1,16 @@
 """GPU tests of the batched ensemble and the served ensemble.
 
 * every cluster's result in a batch is bitwise identical to the same cluster
-  run alone (its decomposition depends only on its own N and target count),
-  and meets the oracle bars of test_gpu_parity;
+  run alone (its decomposition depends only on its own N and target count;
+  under checked precision a cluster's flagged targets are recomputed once
+  all its target blocks are done, with a plan for their total count), and
+  meets the oracle bars of test_gpu_parity;
 * the captured graph replays correctly with eager work between replays and a
   different mix each time;
 * the server completes concurrently submitted requests with the same bits,
-  refuses admission beyond its HBM budget, and reports latency percentiles.
+  refuses admission beyond its HBM budget, and reports latency percentiles;
+* the batched-ensemble workload (256 clusters) and a served mix with
+  clusters up to 1048576 particles at their sizes.
 """
 import sys
 import threading
@@ -103,6 +107,7 @@ def test_ensemble_checked_precision(clusters):
         targets[10] = np.arange(len(clusters[10][2]), dtype=np.int32)   # one full cluster
         for c, t in enumerate(targets):
             ens.set_targets(c, t)
+        ens.set_precision(regf.PREC_SPLIT)
         ens.launch()
         ens.sync()
         split = [ens.fetch(c) for c in range(len(clusters))]
@@ -178,26 +183,41 @@ def test_ensemble_graph_replays_with_changing_mix(clusters):
             say(f"  replay {rep}: clusters {chosen} ok")
 
 
-def test_ensemble_block_ranges_give_same_bits(clusters):
-    say("case: ensemble block ranges")
+@pytest.mark.parametrize("prec", [regf.PREC_CHECKED, regf.PREC_SPLIT])
+def test_ensemble_block_ranges_give_same_bits(clusters, prec):
+    """A cluster's target blocks launched over several batches (in any
+    order) give the bits of one launch; under checked precision the flagged
+    targets of all batches are recomputed together once the last block is
+    done."""
+    say(f"case: ensemble block ranges, precision {prec}")
     x, v, m, r2 = clusters[10]
     t = np.random.default_rng(11).permutation(len(m))[:20000].astype(np.int32)
     with regf.Ensemble(device=0, cap=128, max_clusters=2) as ens:
+        ens.set_precision(prec)
         ens.upload(0, m, x, v, r2)
         ens.set_targets(0, t)
         ens.launch()
         ens.sync()
         whole = ens.fetch(0)
-        ens.set_targets(0, t)
-        nb, tpb = ens.blocks(0)
-        say(f"  {nb} blocks of {tpb} targets")
-        for first, count in ((0, 3), (3, nb // 2), (3 + nb // 2, -1)):
-            ens.set_block_range(0, first, count)
-            ens.launch()
-            ens.sync()
-        sliced = ens.fetch(0)
-    for a, b in zip(whole, sliced):
-        assert np.array_equal(a, b)
+        nre = ens.last_recheck(0)
+        for ranges in (((0, 3), (3, None), (-1, -1)), ((-1, -1), (0, 3), (3, None))):
+            ens.set_targets(0, t)
+            nb, tpb = ens.blocks(0)
+            say(f"  {nb} blocks of {tpb} targets")
+            for first, count in ranges:
+                if first < 0:
+                    first, count = 3 + nb // 2, -1
+                elif count is None:
+                    count = nb // 2
+                ens.set_block_range(0, first, count)
+                ens.launch()
+                ens.sync()
+            sliced = ens.fetch(0)
+            assert ens.last_recheck(0) == nre
+            for a, b in zip(whole, sliced):
+                assert np.array_equal(a, b)
+    if prec == regf.PREC_CHECKED:
+        assert nre > 0
 
 
 @pytest.mark.parametrize("budget", [3e10, 2e7])
@@ -244,9 +264,9 @@ def test_server_concurrent_requests_match_alone(clusters, budget):
 
 def test_server_checked_precision_matches_alone(clusters):
     """Served requests under checked precision, with a small batch budget so
-    requests span several batches: counts and lists equal to the cluster
-    alone (checked ensemble), a and jdot equal bit for bit where neither
-    recomputed, within 1e-12 relative elsewhere (both FP64-term results)."""
+    requests span several batches: every result bit for bit that of the
+    cluster alone (checked ensemble), since the flagged targets of a request
+    are recomputed together after its last block."""
     say("case: server checked precision vs alone")
     sub = clusters[:8]
     with regf.Server(device=0, cap=128, max_clusters=8, hbm_budget=8 << 30) as srv:
@@ -275,12 +295,9 @@ def test_server_checked_precision_matches_alone(clusters):
                 nk = alone[2][k]
                 assert np.array_equal(alone[3][k, :nk], got[3][k, :nk]), f"request {q} list"
             for a, b in zip(alone[:2], got[:2]):
-                same = np.all(a == b, axis=1)
-                differ += int((~same).sum())
-                rel = np.linalg.norm(a - b, axis=1) / np.maximum(np.linalg.norm(a, axis=1),
-                                                                1e-300)
-                assert rel.max() <= 1e-12, f"request {q}: {rel.max()}"
-    say(f"  targets differing in the last bits (recomputed under another plan): {differ}")
+                differ += int((~np.all(a == b, axis=1)).sum())
+    say(f"  targets differing from the cluster alone: {differ}")
+    assert differ == 0
 
 
 def test_server_admission_by_budget(clusters):
@@ -332,3 +349,136 @@ def test_server_request_admission(clusters):
         assert st["rejected"] == 1 and st["completed"] == 7 and st["max_pending"] >= 1
         srv.reset_stats()
         assert srv.stats()["rejected"] == 0
+
+
+def _sample_vs_oracle(cl, t, res, label, nsample=8, seed=0):
+    """a and jdot of `nsample` targets within 1e-6 relative of the oracle
+    (the default precision's contract); lists exact outside the band."""
+    x, v, m, r2 = cl[:4]
+    acc, adot, nnb, lst = res[:4]
+    cap = lst.shape[1]
+    pick = np.random.default_rng(seed).permutation(len(t))[:nsample]
+    ts = t[pick]
+    ra, rj, rn, rl = oracle.regf(x, v, m, r2, ts, cap)
+    lists = compare_lists(nnb[pick], lst[pick], rn, rl, x, r2, ts)
+    w = cap
+    valid = np.arange(w)[None, :] < np.minimum(np.abs(nnb[pick]), w)[:, None]
+    rows = (nnb[pick] == rn) & np.all((lst[pick] == rl) | ~valid, axis=1)
+    err = force_errors(acc[pick][rows], adot[pick][rows], ra[rows], rj[rows])
+    assert lists["hard_mismatches"] == 0 and lists["not_ascending"] == 0, (label, lists)
+    assert err["a_rel"]["max"] <= 1e-6 and err["jdot_rel"]["max"] <= 1e-6, (label, err)
+    return err
+
+
+def test_config4_ensemble_256_clusters():
+    """The batched-ensemble workload at its size: 256 Plummer clusters, N
+    log-uniform in [1024, 65536], active fraction in [0.01, 1], one batched
+    launch (default precision).  Every cluster's counts and lists equal
+    those of its own planned launch, a and jdot of both agree to the
+    contract, 16 sampled targets per cluster meet 1e-6 against the oracle,
+    and eight clusters run alone give the bits of the mix."""
+    from abyss_amd import workloads
+    cl = workloads.ensemble_clusters()
+    say(f"case: config 4, {len(cl)} clusters, sum N {sum(len(c[2]) for c in cl)}")
+    cap = 256
+    with regf.Ensemble(device=0, cap=cap, max_clusters=len(cl)) as ens:
+        for k, (x, v, m, r2, t) in enumerate(cl):
+            ens.upload(k, m, x, v, r2)
+            ens.set_targets(k, t)
+        ens.launch()
+        ens.sync()
+        mixed = [ens.fetch(k) for k in range(len(cl))]
+        nre = sum(ens.last_recheck(k) for k in range(len(cl)))
+        order = np.argsort([len(c[2]) for c in cl])
+        alone_set = sorted(set([int(order[0]), int(order[-1])] +
+                               np.random.default_rng(4).permutation(len(cl))[:6].tolist()))
+        for k in alone_set:
+            for j in range(len(cl)):
+                ens.set_targets(j, cl[j][4] if j == k else [])
+            ens.launch()
+            ens.sync()
+            alone = ens.fetch(k)
+            for a, b in zip(alone, mixed[k]):
+                assert np.array_equal(a, b), f"cluster {k} differs alone vs mixed"
+    say(f"  recomputed {nre}; alone == mixed for clusters {alone_set}")
+    worst = 0.0
+    with regf.Context(device=0, cap=cap) as c:
+        for k, (x, v, m, r2, t) in enumerate(cl):
+            c.upload(m, x, v, r2)
+            own = c.query(t)
+            assert np.array_equal(own[2], mixed[k][2]), f"cluster {k} counts"
+            w = np.arange(cap)[None, :] < np.minimum(np.abs(own[2]), cap)[:, None]
+            assert np.all((own[3] == mixed[k][3]) | ~w), f"cluster {k} lists"
+            d = force_errors(mixed[k][0], mixed[k][1], own[0], own[1])
+            worst = max(worst, d["a_rel"]["max"], d["jdot_rel"]["max"])
+            _sample_vs_oracle((x, v, m, r2), t, mixed[k], f"config4/{k}", nsample=16, seed=k)
+    say(f"  batch vs planned launches: max rel diff {worst:.2e}")
+    assert worst <= 2e-6
+
+
+def test_config5_served_mix_with_large_clusters():
+    """A served mix at the served workload's sizes: 16 clusters, N
+    log-uniform in [4096, 1048576] plus a King cluster of 524288 and a
+    Plummer cluster of 1048576 particles; 48 regular-step requests (active
+    blocks log-uniform in [1, min(N, 32768)]) submitted by four threads at
+    Poisson times, under the default precision.  Every request's results
+    equal the cluster run alone bit for bit and meet the contract against
+    the oracle on sampled targets."""
+    import time
+    from abyss_amd import workloads
+    sizes = list(np.exp(np.random.default_rng(5).uniform(np.log(4096), np.log(1048576),
+                                                         16)).astype(int))
+    sizes[1] = 524288        # odd index: a King cluster
+    sizes[2] = 1048576       # even index: a Plummer cluster
+    cl = workloads.served_clusters(sizes=sizes, seed=5)
+    say(f"case: config 5 served mix, sizes {sizes}")
+    cap = 128
+    rng = np.random.default_rng(17)
+    reqs = []
+    for q in range(48):
+        c = int(rng.integers(len(cl)))
+        if q < 4:
+            c = 1 + (q % 2)   # the two large clusters are always in the mix
+        n = len(cl[c][2])
+        size = int(min(n, 32768, max(1, np.exp(rng.uniform(0.0, np.log(min(n, 32768) + 1))))))
+        reqs.append((c, rng.choice(n, size, replace=False).astype(np.int32)))
+    results = [None] * len(reqs)
+    with regf.Server(device=0, cap=cap, max_clusters=len(cl), hbm_budget=64 << 30) as srv:
+        ids = [srv.add_cluster(m, x, v, r2) for (x, v, m, r2) in cl]
+
+        def client(part, seed):
+            r = np.random.default_rng(seed)
+            tickets = []
+            for q in part:
+                time.sleep(r.exponential(1.0 / 100.0))
+                tk, nt = srv.submit(ids[reqs[q][0]], reqs[q][1])
+                tickets.append((q, tk, nt))
+            for q, tk, nt in tickets:
+                results[q] = srv.wait(tk, nt)
+
+        th = [threading.Thread(target=client, args=(list(range(k, len(reqs), 4)), 40 + k))
+              for k in range(4)]
+        for t_ in th:
+            t_.start()
+        for t_ in th:
+            t_.join()
+        st = srv.stats()
+        say(f"  stats {st}")
+        assert st["completed"] == len(reqs)
+    with regf.Ensemble(device=0, cap=cap, max_clusters=len(cl)) as ens:
+        for c, (x, v, m, r2) in enumerate(cl):
+            ens.upload(c, m, x, v, r2)
+        for q, (c, t) in enumerate(reqs):
+            for k in range(len(cl)):
+                ens.set_targets(k, t if k == c else [])
+            ens.launch()
+            ens.sync()
+            alone = ens.fetch(c)
+            got = results[q]
+            for a, b in zip(alone[:3], got[:3]):
+                assert np.array_equal(a, b), f"request {q} (cluster {c}) differs from alone"
+            for k in range(len(t)):
+                nk = min(abs(int(alone[2][k])), cap)
+                assert np.array_equal(alone[3][k, :nk], got[3][k, :nk]), f"request {q} list"
+            _sample_vs_oracle(cl[c], t, got, f"config5/req{q}", nsample=8, seed=q)
+    say("  every request equals its cluster alone and meets the contract")
