@@ -17,6 +17,7 @@ HEADER = os.path.join(ROOT, "include", "abyss_regf.h")
 def declared_functions():
     src = open(HEADER).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    src = re.sub(r"^\s*#.*$", "", src, flags=re.M)   # preprocessor lines
     names = re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", src)
     return sorted(set(n for n in names if n not in ("if", "while", "return")))
 

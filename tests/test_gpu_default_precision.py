@@ -169,7 +169,9 @@ regf.SendToDevice(m32.astype(np.float64), x32.astype(np.float64), v32.astype(np.
                   r32.astype(np.float64))
 a64, j64, n64, l64 = regf.CalculateAccelerationOnDevice(t)
 regf.CloseDevice()
-assert np.array_equal(nnb, n64) and np.array_equal(lst, l64)
+assert np.array_equal(nnb, n64)
+filled = np.arange(cap)[None, :] < np.minimum(nnb, cap)[:, None]   # populated list entries
+assert np.array_equal(lst[filled], l64[filled])
 assert np.array_equal(acc, a64.astype(np.float32)) and np.array_equal(adot, j64.astype(np.float32))
 print("f32 ABI ok", nt, flush=True)
 '''
