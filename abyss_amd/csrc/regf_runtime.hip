@@ -18,7 +18,10 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <limits.h>
+
 #include <algorithm>
+#include <mutex>
 
 using abyss::JState;
 using abyss::Plan;
@@ -449,6 +452,8 @@ const char *abyss_error_string(int code)
         case ABYSS_ERR_NEIGHBOR_OVERFLOW: return "neighbour list overflow";
         case ABYSS_ERR_ADMISSION: return "admission refused (memory budget)";
         case ABYSS_ERR_BUSY: return "busy: no free slot, or request turned away by admission";
+        case ABYSS_ERR_SCRATCH_LIMIT:
+            return "no-allocation query: sizes exceed the static scratch";
         default: return "unknown error";
     }
 }
@@ -896,6 +901,213 @@ int abyss_ctx_query(abyss_ctx_t c, int nt, const int *targets, double *acc, doub
     rc = abyss_ctx_launch(c);
     if (rc != ABYSS_OK) return rc;
     return abyss_ctx_fetch(c, acc, adot, nnb, list);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// No-allocation query.  Every device buffer the query needs beyond the
+// caller's inputs and outputs (FP32 masses, a checked copy of the target
+// list, split partials, the checked-precision spread, flags and
+// recomputation partials) lives in `scratch`, one static device array per
+// GPU.  Its device address is looked up once per device with
+// hipGetSymbolAddress; in host code the array's name is only a host-side
+// stand-in, so kernels are always given the looked-up address.  Concurrent
+// calls on one device share the array and race.
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __align__(256) char scratch[ABYSS_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kNoAllocDevices = 64;
+std::mutex g_na_mutex;
+char *g_na_ptr[kNoAllocDevices] = {};   // device address of `scratch` per device
+int g_na_cu[kNoAllocDevices] = {};
+
+// With `device` current: the per-device address of `scratch` and the CU count.
+int noalloc_init(int device, char **ptr, int *cu)
+{
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (!g_na_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return ABYSS_ERR_HIP;
+        int n_cu = 0;
+        if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device) !=
+            hipSuccess)
+            return ABYSS_ERR_HIP;
+        g_na_cu[device] = n_cu;
+        g_na_ptr[device] = static_cast<char *>(p);
+    }
+    *ptr = g_na_ptr[device];
+    *cu = g_na_cu[device];
+    return ABYSS_OK;
+}
+
+// FP32 masses (the rounding of the context's upload) and the checked target
+// copy: an index outside [0, n) becomes 0 in the copy and is counted.
+__global__ void noalloc_prep_kernel(const double *__restrict__ m, float *__restrict__ m32, int n,
+                                    const int *__restrict__ tgt, int *__restrict__ tgt_s, int nt,
+                                    int *__restrict__ bad)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) m32[i] = (float)m[i];
+    if (i < nt) {
+        const int t = tgt[i];
+        const bool ok = t >= 0 && t < n;
+        tgt_s[i] = ok ? t : 0;
+        if (!ok) atomicAdd(bad, 1);
+    }
+}
+
+// Results of targets outside [0, n): NaN forces and INT_MIN as the count.
+__global__ void noalloc_mark_kernel(const int *__restrict__ tgt, int nt, int n,
+                                    const int *__restrict__ bad, double *__restrict__ acc,
+                                    double *__restrict__ adot, int *__restrict__ nnb)
+{
+    if (*bad == 0) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nt) return;
+    const int t = tgt[q];
+    if (t >= 0 && t < n) return;
+    const double nan = __builtin_nan("");
+    for (int c = 0; c < 3; ++c) {
+        acc[3 * (size_t)q + c] = nan;
+        adot[3 * (size_t)q + c] = nan;
+    }
+    nnb[q] = INT_MIN;
+}
+
+// Bump allocator over the static array (256-byte aligned pieces).
+struct Carve {
+    char *base;
+    size_t used = 0;
+    char *take(size_t bytes)
+    {
+        char *r = base + used;
+        used += (bytes + 255) & ~size_t(255);
+        return r;
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int abyss_query_noalloc(int device, int n, const double *jset, int nt, const int *targets,
+                        int cap, int precision, double *acc, double *adot, int *nnb, int *list,
+                        void *stream)
+{
+    if (n <= 0 || nt <= 0 || nt > n || cap <= 0 || !jset || !targets || !acc || !adot || !nnb ||
+        !list)
+        return ABYSS_ERR_INVALID_ARG;
+    if (precision != ABYSS_PREC_SPLIT && precision != ABYSS_PREC_CHECKED)
+        return ABYSS_ERR_INVALID_ARG;
+    if (device < 0 || device >= kNoAllocDevices || device >= abyss_device_count())
+        return ABYSS_ERR_INVALID_ARG;
+    int prev = -1;
+    if (hipGetDevice(&prev) != hipSuccess) return ABYSS_ERR_HIP;
+    if (prev != device && hipSetDevice(device) != hipSuccess) return ABYSS_ERR_HIP;
+    auto done = [&](int rc) {
+        if (prev != device) (void)hipSetDevice(prev);
+        return rc;
+    };
+    char *base = nullptr;
+    int cu = 0;
+    int rc = noalloc_init(device, &base, &cu);
+    if (rc != ABYSS_OK) return done(rc);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+
+    // the context's plan for these sizes under a split-scratch limit of
+    // ABYSS_NOALLOC_SPLIT_BYTES (a context given the same limit plans alike)
+    abyss_ctx_s tmp;
+    tmp.device = device;
+    tmp.cu_count = cu;
+    tmp.cap = cap;
+    tmp.precision = precision;
+    tmp.scratch_limit = ABYSS_NOALLOC_SPLIT_BYTES;
+    const Plan p = plan_for(&tmp, nt, n, precision, cap, true);
+    const bool checked = precision == ABYSS_PREC_CHECKED;
+    const size_t S = (size_t)p.splits, t = (size_t)nt;
+
+    Carve cv{base};
+    int *counters = reinterpret_cast<int *>(cv.take(256));   // [0] flagged, [1] bad targets
+    int *tgt_s = reinterpret_cast<int *>(cv.take(t * sizeof(int)));
+    float *m32 = reinterpret_cast<float *>(cv.take((size_t)n * sizeof(float)));
+    QueryOut part{};
+    if (S > 1) {
+        part.acc = reinterpret_cast<double *>(cv.take(S * t * 3 * sizeof(double)));
+        part.adot = reinterpret_cast<double *>(cv.take(S * t * 3 * sizeof(double)));
+        part.cnt = reinterpret_cast<int *>(cv.take(S * t * sizeof(int)));
+        part.list = reinterpret_cast<int *>(cv.take(S * t * (size_t)cap * sizeof(int)));
+    }
+    float *spread = nullptr;
+    int *fq = nullptr, *fi = nullptr;
+    if (checked) {
+        spread = reinterpret_cast<float *>(cv.take(2 * t * sizeof(float)));
+        if (S > 1) part.spread = reinterpret_cast<float *>(cv.take(2 * S * t * sizeof(float)));
+        fq = reinterpret_cast<int *>(cv.take(t * sizeof(int)));
+        fi = reinterpret_cast<int *>(cv.take(t * sizeof(int)));
+    }
+    if (cv.used > ABYSS_NOALLOC_SCRATCH_BYTES) return done(ABYSS_ERR_SCRATCH_LIMIT);
+    // the recomputation of checked precision: planned for every target (the
+    // flagged count stays on the device), its splits bounded by what is left
+    Plan rp{};
+    QueryOut rpart{};
+    if (checked) {
+        const size_t row = 2 * 3 * sizeof(double) + sizeof(int);
+        const size_t left = ABYSS_NOALLOC_SCRATCH_BYTES - cv.used;
+        if (left < t * row + 3 * 256) return done(ABYSS_ERR_SCRATCH_LIMIT);
+        tmp.scratch_limit = left - 3 * 256;
+        rp = plan_for(&tmp, nt, n, ABYSS_PREC_REFINED, 0, false);
+        const size_t rows = (size_t)rp.splits * t;
+        rpart.acc = reinterpret_cast<double *>(cv.take(rows * 3 * sizeof(double)));
+        rpart.adot = reinterpret_cast<double *>(cv.take(rows * 3 * sizeof(double)));
+        rpart.cnt = reinterpret_cast<int *>(cv.take(rows * sizeof(int)));
+        if (cv.used > ABYSS_NOALLOC_SCRATCH_BYTES) return done(ABYSS_ERR_SCRATCH_LIMIT);
+    }
+
+    JState js{};
+    js.x = jset + 0 * (size_t)n;
+    js.y = jset + 1 * (size_t)n;
+    js.z = jset + 2 * (size_t)n;
+    js.vx = jset + 3 * (size_t)n;
+    js.vy = jset + 4 * (size_t)n;
+    js.vz = jset + 5 * (size_t)n;
+    js.m = jset + 6 * (size_t)n;
+    js.rnb2 = jset + 7 * (size_t)n;
+    js.m32 = m32;
+    js.n = n;
+
+#define NACHK(call)                                  \
+    do {                                             \
+        if ((call) != hipSuccess) return done(ABYSS_ERR_HIP); \
+    } while (0)
+    NACHK(abyss::launch_batch_reset(counters, st));   // both counters := 0
+    const int w = std::max(n, nt);
+    hipLaunchKernelGGL(noalloc_prep_kernel, dim3((w + 255) / 256), dim3(256), 0, st, js.m, m32, n,
+                       targets, tgt_s, nt, counters + 1);
+    NACHK(hipGetLastError());
+    QueryOut fin{acc, adot, nnb, list};
+    fin.spread = spread;
+    if (S == 1) {
+        NACHK(abyss::launch_regf(js, tgt_s, nt, p, fin, cap, precision, st));
+    } else {
+        NACHK(abyss::launch_regf(js, tgt_s, nt, p, part, cap, precision, st));
+        NACHK(abyss::launch_finalize(part, p.splits, nt, cap, acc, adot, nnb, list,
+                                     abyss::FinalMirror{}, spread, st));
+    }
+    if (checked) {
+        const double ratio = ABYSS_CHECK_ERR_SCALE / 1.0e-6;
+        NACHK(abyss::launch_check_flag(acc, adot, spread, tgt_s, nt, ratio * ratio, counters, fq,
+                                       fi, st));
+        NACHK(abyss::launch_recheck_dev(js, fi, counters, nt, rp, rpart, fq, acc, adot,
+                                        abyss::FinalMirror{}, nullptr, st));
+    }
+    hipLaunchKernelGGL(noalloc_mark_kernel, dim3((nt + 255) / 256), dim3(256), 0, st, targets,
+                       nt, n, counters + 1, acc, adot, nnb);
+    NACHK(hipGetLastError());
+#undef NACHK
+    return done(ABYSS_OK);
 }
 
 }  // extern "C"

@@ -30,6 +30,10 @@ ERR_OUT_OF_MEMORY = 8
 ERR_NEIGHBOR_OVERFLOW = 9
 ERR_ADMISSION = 10
 ERR_BUSY = 11
+ERR_SCRATCH_LIMIT = 12
+# static device scratch of abyss_query_noalloc, and its j-split share
+NOALLOC_SCRATCH_BYTES = 128 << 20
+NOALLOC_SPLIT_BYTES = 64 << 20
 
 PREC_SPLIT = 0
 PREC_FP64 = 1
@@ -70,7 +74,7 @@ EXPORTS = [
     "abyss_srv_queue_stats", "abyss_ctx_set_check_tolerance", "abyss_ctx_last_recheck",
     "abyss_ctx_fetch_spread", "abyss_ens_set_precision", "abyss_ens_set_check_tolerance",
     "abyss_ens_last_recheck", "abyss_srv_set_precision", "abyss_srv_set_check_tolerance",
-    "abyss_set_legacy_errors",
+    "abyss_set_legacy_errors", "abyss_query_noalloc",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -125,6 +129,9 @@ def load():
         "abyss_ctx_plan": (ctypes.c_int, [_vp, _ip, _ip, _ip, _ip, _ip, _ip]),
         "abyss_ctx_error_detail": (ctypes.c_char_p, [_vp]),
         "abyss_ctx_set_check_tolerance": (ctypes.c_int, [_vp, ctypes.c_double]),
+        "abyss_query_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp,
+                                               ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp,
+                                               _vp]),
         "abyss_ctx_last_recheck": (ctypes.c_int, [_vp, _ip]),
         "abyss_ctx_fetch_spread": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
         "abyss_ens_set_precision": (ctypes.c_int, [_vp, ctypes.c_int]),

@@ -8,6 +8,7 @@
  * Part 2 adds explicit contexts (one per cluster / per thread), a
  * device-resident launch path, tuning controls and error reporting.
  * Part 3 is the batched-ensemble and served-ensemble interface.
+ * Part 4 is one query that allocates nothing, for callers that cannot.
  *
  * Semantics shared by every query (reference: src_mpi/cuda/cuda_kernels.cu
  * :269-450 and the callers cited below):
@@ -77,6 +78,7 @@ extern "C" {
 #define ABYSS_ERR_NEIGHBOR_OVERFLOW 9  /* results valid; some NumNeighbor < 0 */
 #define ABYSS_ERR_ADMISSION        10  /* served ensemble: HBM budget exhausted */
 #define ABYSS_ERR_BUSY             11  /* served ensemble: no free slot, or request turned away */
+#define ABYSS_ERR_SCRATCH_LIMIT    12  /* no-allocation query: sizes exceed its static scratch */
 
 #define ABYSS_PREC_SPLIT   0
 #define ABYSS_PREC_FP64    1
@@ -436,6 +438,44 @@ int abyss_srv_queue_stats(abyss_srv_t srv, long long *rejected, long long *max_p
                           double *peak_outstanding_pairs);
 /* Estimated device bytes a cluster of n particles needs (admission rule). */
 unsigned long long abyss_srv_cluster_footprint(abyss_srv_t srv, int n);
+
+/* ===================================================================== */
+/* Part 4: no-allocation query.  No counterpart in the reference: it is  */
+/* for callers that cannot allocate device memory (code under HIP graph  */
+/* capture, tight serving loops, harnesses that forbid hipMalloc).       */
+/* ===================================================================== */
+
+/* Bytes of the library's static device scratch (one array per GPU the
+ * library runs on, held until the process ends), and the part of it the
+ * plan may give to j-split partials (a context given the same limit with
+ * abyss_ctx_set_scratch_limit plans the same launch). */
+#define ABYSS_NOALLOC_SCRATCH_BYTES (128ull << 20)
+#define ABYSS_NOALLOC_SPLIT_BYTES (64ull << 20)
+
+/* One regular-force + neighbour query with every buffer in device memory
+ * owned by the caller, enqueued on `stream` (a hipStream_t; null = the null
+ * stream) and not waited for; it allocates nothing.  Same results as
+ * abyss_ctx_set_targets + abyss_ctx_launch on a context holding the same
+ * j-set, with the same precision and scratch limit ABYSS_NOALLOC_SPLIT_BYTES
+ * (checked precision: targets it recomputes agree to FP64 rounding, since
+ * the recomputation here is planned for every target and reads the flagged
+ * count on the device).
+ *   jset: device SoA of n particles, 8 arrays of n doubles
+ *         [x | y | z | vx | vy | vz | m | r_nb^2];
+ *   targets: device int[nt], indices into the j-set (0 < nt <= n);
+ *   acc, adot: device double[nt][3]; nnb: device int[nt];
+ *   list: device int[nt][cap] (rows of cap entries, ascending j);
+ *   precision: ABYSS_PREC_SPLIT or ABYSS_PREC_CHECKED (tolerance 1e-6).
+ * A target outside [0, n) gets NaN forces and nnb = INT_MIN (the device
+ * cannot report it without a host wait).  Returns ABYSS_ERR_SCRATCH_LIMIT
+ * when n and nt need more than ABYSS_NOALLOC_SCRATCH_BYTES of scratch
+ * (about 4 n + 20 nt bytes plus the recomputation's partials), before
+ * anything is enqueued.  Every call on a device uses that device's one
+ * scratch array: calls that overlap on one device (two streams, two host
+ * threads) race and must be ordered by the caller. */
+int abyss_query_noalloc(int device, int n, const double *jset, int nt, const int *targets,
+                        int cap, int precision, double *acc, double *adot, int *nnb, int *list,
+                        void *stream);
 
 #ifdef __cplusplus
 }
