@@ -1,4 +1,4 @@
-"""Build the HIP libraries in-tree (gfx950 only):
+"""Build the HIP libraries in-tree (gfx950 only; host-only sources with g++):
   abyss_amd/libabyss_regf.so      legacy entry points with double (CUDA_REAL = double)
   abyss_amd/libabyss_regf_f32.so  the same library with float legacy entry points
                                   (regf_legacy.hip built with -DABYSS_LEGACY_FLOAT)
@@ -20,6 +20,9 @@ SOURCES = ["regf_kernels.hip", "regf_runtime.hip", "regf_ensemble.hip", "regf_au
            "regf_state.hip", "regf_irregular.hip", "regf_derivs.hip"]
 # (object name, extra flags) of the legacy entry points, one per library
 LEGACY = [("regf_legacy.o", []), ("regf_legacy_f32.o", ["-DABYSS_LEGACY_FLOAT"])]
+# host-only C++ (no device code), compiled with the host compiler
+HOST_SOURCES = ["regf_pack.cpp"]
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-pthread", "-Wall"]
 ARCH = os.environ.get("ABYSS_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
@@ -50,14 +53,22 @@ def build(verbose: bool = False, extra=()) -> str:
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((subprocess.Popen(cmd), cmd))
+    host_objs = []
+    for src in HOST_SOURCES:
+        o = os.path.join(bdir, src.replace(".cpp", ".o"))
+        cmd = ["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", o]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((subprocess.Popen(cmd), cmd))
+        host_objs.append(o)
     for p, cmd in procs:
         if p.wait() != 0:
             raise RuntimeError("build failed: " + " ".join(cmd))
-    objs = [o for _, o, _ in jobs[:len(SOURCES)]]
+    objs = [o for _, o, _ in jobs[:len(SOURCES)]] + host_objs
     for lib, (legacy_obj, _) in ((LIB, LEGACY[0]), (LIB_F32, LEGACY[1])):
         tmp = lib + ".tmp"
         cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-Wl,-Bsymbolic",
-               "-o", tmp, *objs, os.path.join(bdir, legacy_obj)]
+               "-pthread", "-o", tmp, *objs, os.path.join(bdir, legacy_obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd)

@@ -112,6 +112,16 @@ struct abyss_ctx_s {
     long long n_send = 0, n_grav = 0, sum_ni = 0;
     double interactions = 0.0, t_grav_ms = 0.0;
 
+    // force-kernel timing (abyss_ctx_set_kernel_timing): an event pair
+    // around the force kernel of every launch, in a ring that is read when
+    // it fills and at sync
+    static constexpr int kTimingRing = 64;
+    bool ktiming = false;
+    hipEvent_t kev[2 * kTimingRing] = {};
+    int kev_used = 0;
+    double kforce_ms = 0.0;
+    long long klaunches = 0;
+
     char detail[512] = {0};
 };
 

@@ -707,6 +707,11 @@ struct abyss_srv_s {
     std::vector<double> latencies;
     double interactions = 0.0;
     long long batches = 0, completed = 0;
+    // GPU time of the batches (events around each on the ensemble stream)
+    // and the wall-clock start of the statistics
+    hipEvent_t ev_b0 = nullptr, ev_b1 = nullptr;
+    double busy_ms = 0.0;
+    Clock::time_point t_stats = Clock::now();
 
     // request admission: submitted, not yet completed work (pairs) and its
     // limit (0: unlimited); rejections and queue high-water marks
@@ -724,6 +729,7 @@ namespace {
 // whose blocks are all done goes to the completion thread.
 void serve_loop(abyss_srv_t s)
 {
+    (void)hipSetDevice(s->device);
     std::vector<ReqPtr> running;
     for (;;) {
         std::vector<ReqPtr> starting;
@@ -787,8 +793,15 @@ void serve_loop(abyss_srv_t s)
                     take[k] = nb;
                     abyss_ens_set_block_range(s->ens, r->cluster, r->next_block, nb);
                 }
+                const bool timed = s->ev_b0 && s->ev_b1 &&
+                                   hipEventRecord(s->ev_b0, s->ens->stream) == hipSuccess;
                 int rc = abyss_ens_launch(s->ens);
+                const bool timed_end =
+                    timed && hipEventRecord(s->ev_b1, s->ens->stream) == hipSuccess;
                 if (rc == ABYSS_OK) rc = abyss_ens_sync(s->ens);
+                float batch_ms = 0.0f;
+                if (!timed_end || hipEventElapsedTime(&batch_ms, s->ev_b0, s->ev_b1) != hipSuccess)
+                    batch_ms = 0.0f;
                 double inter = 0.0;
                 std::vector<ReqPtr> still;
                 for (size_t k = 0; k < running.size(); ++k) {
@@ -806,6 +819,7 @@ void serve_loop(abyss_srv_t s)
                 running.swap(still);
                 std::lock_guard<std::mutex> lk(s->mu);
                 s->batches += 1;
+                s->busy_ms += batch_ms;
                 (void)inter;
             }
         }
@@ -898,6 +912,13 @@ int abyss_srv_create(int device, int cap, int max_clusters, unsigned long long h
     s->footprint.assign(max_clusters, 0);
     s->nparticles.assign(max_clusters, 0);
     s->busy.assign(max_clusters, 0);
+    if (hipSetDevice(device) != hipSuccess || hipEventCreate(&s->ev_b0) != hipSuccess ||
+        hipEventCreate(&s->ev_b1) != hipSuccess) {
+        if (s->ev_b0) (void)hipEventDestroy(s->ev_b0);
+        abyss_ens_destroy(e);
+        delete s;
+        return ABYSS_ERR_HIP;
+    }
     s->worker = std::thread(serve_loop, s);
     s->completer = std::thread(complete_loop, s);
     *out = s;
@@ -915,6 +936,8 @@ int abyss_srv_destroy(abyss_srv_t s)
     if (s->worker.joinable()) s->worker.join();   // drains every submitted request
     if (s->completer.joinable()) s->completer.join();
     abyss_ens_destroy(s->ens);
+    if (s->ev_b0) (void)hipEventDestroy(s->ev_b0);
+    if (s->ev_b1) (void)hipEventDestroy(s->ev_b1);
     delete s;
     return ABYSS_OK;
 }
@@ -1089,6 +1112,17 @@ int abyss_srv_reset_stats(abyss_srv_t s)
     s->batches = s->completed = 0;
     s->rejected = s->max_pending = 0;
     s->peak_outstanding = s->outstanding;
+    s->busy_ms = 0.0;
+    s->t_stats = Clock::now();
+    return ABYSS_OK;
+}
+
+int abyss_srv_gpu_busy(abyss_srv_t s, double *busy_ms, double *wall_ms)
+{
+    if (!s || !busy_ms || !wall_ms) return ABYSS_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *busy_ms = s->busy_ms;
+    *wall_ms = std::chrono::duration<double, std::milli>(Clock::now() - s->t_stats).count();
     return ABYSS_OK;
 }
 

@@ -38,6 +38,17 @@ struct FinalMirror {
     int *list = nullptr;     // [nt][cap]
 };
 
+// Checked precision's flagging done by the finalize of a split launch
+// (counter null: none): targets that need recomputation are appended to fq
+// (position) / fi (j-index, from tgt) and counted in *counter, which must
+// start at 0.  k2 = (ABYSS_CHECK_ERR_SCALE / tol)^2.
+struct CheckFlag {
+    double k2 = 0.0;
+    int *counter = nullptr;
+    int *fq = nullptr, *fi = nullptr;
+    const int *tgt = nullptr;
+};
+
 struct Plan {
     int ipl = 1;     // targets per lane
     int splits = 1;  // j-splits (grid.y)
@@ -144,8 +155,11 @@ hipError_t launch_batch_finalize(const ClusterDesc *cd, const int *fin_cl, const
 // Launch the pairwise kernel for nt targets against js (all j), split into
 // plan.splits contiguous j-ranges.  prec: an ABYSS_PREC_* mode, or
 // kPrecRecheck (the recomputation of checked precision).
+// reset (checked precision, lanes-per-target plans): a counter the kernel
+// zeroes before any later launch on the stream reads it.
 hipError_t launch_regf(const JState &js, const int *tgt, int nt, const Plan &plan,
-                       const QueryOut &out, int cap, int prec, hipStream_t st);
+                       const QueryOut &out, int cap, int prec, hipStream_t st,
+                       int *reset = nullptr);
 
 // Internal mode of launch_regf: the neighbour test of split precision (FP32
 // r^2 of the rounded differences, so the regular set is the same) with FP64
@@ -158,7 +172,8 @@ constexpr int kPrecRecheck = 100;
 // `spread` non-null also the S spread partials.
 hipError_t launch_finalize(const QueryOut &part, int S, int nt, int cap,
                            double *acc, double *adot, int *nnb, int *list,
-                           const FinalMirror &mirror, float *spread, hipStream_t st);
+                           const FinalMirror &mirror, float *spread, hipStream_t st,
+                           const CheckFlag &flag = CheckFlag{});
 
 // Checked precision.  counters[0] must be 0.  Appends to (fq, fi) the
 // position and j-index of every target q whose a or jdot may be off by

@@ -331,7 +331,7 @@ struct TermMass<0> { using type = float; };
 // Per-lane state of the lanes-per-target kernel.
 struct JLLane {
     double xi, yi, zi, vxi, vyi, vzi;
-    double rnbd;     // r_nb^2 (modes 1, 2)
+    double rnbd;     // r_nb^2 (modes 1, 2); r_nb^2 (1 + 1e-6) (mode 3)
     float rnbf;      // r_nb^2 (mode 0)
     int idx, sub, cnt, cap;
     int *lst;
@@ -408,22 +408,33 @@ __device__ __forceinline__ void jl_tile(JLLane &L, const double2 *__restrict__ p
             const double dvy = p2.x - L.vyi;
             const double dvz = p2.y - L.vzi;
             const double r2 = fma(dz, dz, fma(dy, dy, dx * dx));
-            float rf;
-            bool near;
+            // rsqrt seed of the FP64 r^2 (modes 2 and 3 refine it below)
+            float rf = __builtin_amdgcn_rsqf((float)r2);
             if constexpr (TM == 3) {
-                // the same expression as mode 0, hence the same bits; it also
-                // seeds the rsqrt (within 3.1e-7 of the FP64 r^2, which the
-                // Newton step below then corrects to ~1e-13)
-                const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
-                const float r2f = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
-                rf = __builtin_amdgcn_rsqf(r2f);
-                near = !(r2f > L.rnbf);
+                // The neighbour test must be mode 0's: the FP32 r^2 of the
+                // rounded differences against the FP32 r_nb^2.  That r^2 is
+                // within 5 u (u = 2^-24) of the FP64 one and the FP32 r_nb^2
+                // within u of r_nb^2, so an FP64 r^2 above r_nb^2 (1 + 1e-6)
+                // (L.rnbd in this mode) is regular under mode 0 as well; only
+                // steps with a pair below that margin evaluate mode 0's
+                // expression, in this rare branch.
+                if (__any(!(r2 > L.rnbd))) {
+                    float fx, fy, fz;
+                    // volatile: the converts (and the test after them) stay
+                    // in the branch instead of being computed for every pair
+                    asm volatile("v_cvt_f32_f64 %0, %1" : "=v"(fx) : "v"(dx));
+                    asm volatile("v_cvt_f32_f64 %0, %1" : "=v"(fy) : "v"(dy));
+                    asm volatile("v_cvt_f32_f64 %0, %1" : "=v"(fz) : "v"(dz));
+                    const float r2f = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
+                    const bool near = !(r2f > L.rnbf);
+                    const unsigned long long nb = __ballot(near);
+                    if (nb) jl_near<FULL, SELF>(L, near, nb, r, count, jt, rf);
+                }
             } else {
-                rf = __builtin_amdgcn_rsqf((float)r2);
-                near = !(r2 > L.rnbd);
+                const bool near = !(r2 > L.rnbd);
+                const unsigned long long nb = __ballot(near);
+                if (nb) jl_near<FULL, SELF>(L, near, nb, r, count, jt, rf);
             }
-            const unsigned long long nb = __ballot(near);
-            if (nb) jl_near<FULL, SELF>(L, near, nb, r, count, jt, rf);
             double rinv = (double)rf;
             if constexpr (TM >= 2) {
                 const double h = fma(-(r2 * rinv), rinv, 1.0);
@@ -559,6 +570,8 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
         L.vxi = js.vx[L.idx]; L.vyi = js.vy[L.idx]; L.vzi = js.vz[L.idx];
         L.rnbd = js.rnb2[L.idx];
         L.rnbf = (float)L.rnbd;
+        // mode 3 compares the FP64 r^2 with this bound first (see jl_tile)
+        if constexpr (TM == 3) L.rnbd *= 1.0 + 1.0e-6;
         L.lst = out.list + ((size_t)s * nt_stride + (q - q_base)) * (size_t)cap;
     }
 #pragma unroll
@@ -654,11 +667,14 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
 
 // 5 waves per SIMD (<= 96 VGPRs); the FP64-term modes would otherwise take
 // 98 registers and drop to 4.
+// reset (checked precision, may be null): the flagged-target counter, zeroed
+// here so that no separate launch is needed before the flagging after it.
 template <int JL, int TM, bool EST>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
 void regf_kernel_jl(JState js, const int *__restrict__ tgt, int nt, int jspan, QueryOut out,
-                    int cap, int final_counts)
+                    int cap, int final_counts, int *__restrict__ reset)
 {
+    if (reset && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *reset = 0;
     __shared__ double2 s_pos[2][kTile * 3];
     __shared__ typename TermMass<TM>::type s_m[2][kTile];
     regf_body_jl<JL, TM, EST>(js, tgt, nt, jspan, out, cap, final_counts, blockIdx.x,
@@ -703,13 +719,25 @@ __device__ __forceinline__ void sum_splits(const QueryOut &part, int S, int nt_s
         for (int c = 0; c < 6; ++c) a[c] += __shfl_xor(a[c], d, 64);
 }
 
+// The checked precision's rule: a target is recomputed when either spread,
+// scaled by k2 = (ABYSS_CHECK_ERR_SCALE / tol)^2, exceeds the squared norm
+// of its sum.  One expression (explicit FMAs) for every kernel that applies
+// it, so they flag the same targets.
+__device__ __forceinline__ bool check_rule(float sa, float sj, double ax, double ay, double az,
+                                           double jx, double jy, double jz, double k2)
+{
+    return (double)sa * k2 > fma(az, az, fma(ay, ay, ax * ax)) ||
+           (double)sj * k2 > fma(jz, jz, fma(jy, jy, jx * jx));
+}
+
 __device__ __forceinline__ void finalize_body(const QueryOut &part, int S, int nt_stride, int cap,
                                               int q_rel, int q, bool active, int *off,
                                               double *__restrict__ acc,
                                               double *__restrict__ adot,
                                               int *__restrict__ nnb, int *__restrict__ list,
                                               const FinalMirror &mir = FinalMirror{},
-                                              float *__restrict__ spread = nullptr)
+                                              float *__restrict__ spread = nullptr,
+                                              const CheckFlag &flag = CheckFlag{})
 {
     const int lane = threadIdx.x & 63;
 
@@ -731,6 +759,13 @@ __device__ __forceinline__ void finalize_body(const QueryOut &part, int S, int n
         if (active && lane == 0) {
             spread[2 * (size_t)q + 0] = ea;
             spread[2 * (size_t)q + 1] = ej;
+            // checked precision: flag the target here instead of in a
+            // separate pass (the rule of check_flag_kernel)
+            if (flag.counter && check_rule(ea, ej, a[0], a[1], a[2], a[3], a[4], a[5], flag.k2)) {
+                const int k = atomicAdd(flag.counter, 1);
+                flag.fq[k] = q;
+                flag.fi[k] = flag.tgt[q];
+            }
         }
     }
 
@@ -784,22 +819,16 @@ __global__ __launch_bounds__(256) void finalize_kernel(QueryOut part, int S, int
                                                        double *__restrict__ adot,
                                                        int *__restrict__ nnb,
                                                        int *__restrict__ list, FinalMirror mir,
-                                                       float *__restrict__ spread)
+                                                       float *__restrict__ spread, CheckFlag flag)
 {
     extern __shared__ int s_off_all[];   // [4 waves][S + 1]
     const int w = threadIdx.x >> 6;
     const int q = blockIdx.x * 4 + w;
     finalize_body(part, S, nt, cap, q, q, q < nt, s_off_all + w * (S + 1), acc, adot, nnb, list,
-                  mir, spread);
+                  mir, spread, flag);
 }
 
 // ---- checked precision ---------------------------------------------------
-
-__device__ __forceinline__ double norm2(const double *__restrict__ v, size_t q)
-{
-    const double x = v[3 * q + 0], y = v[3 * q + 1], z = v[3 * q + 2];
-    return x * x + y * y + z * z;
-}
 
 // One thread per target.  A wave appends its flagged targets with one
 // atomic add (of its flag count) and their ranks within the wave.
@@ -815,8 +844,9 @@ __global__ __launch_bounds__(256) void check_flag_kernel(const double *__restric
     const int lane = threadIdx.x & 63;
     bool flag = false;
     if (q < nt) {
-        flag = (double)spread[2 * (size_t)q + 0] * k2 > norm2(acc, q) ||
-               (double)spread[2 * (size_t)q + 1] * k2 > norm2(adot, q);
+        const size_t o = 3 * (size_t)q;
+        flag = check_rule(spread[2 * (size_t)q + 0], spread[2 * (size_t)q + 1], acc[o],
+                          acc[o + 1], acc[o + 2], adot[o], adot[o + 1], adot[o + 2], k2);
     }
     const unsigned long long b = __ballot(flag);
     if (b == 0) return;   // the whole wave
@@ -1042,10 +1072,8 @@ __global__ __launch_bounds__(256) void batch_check_flag_kernel(const ClusterDesc
     }
     const ClusterDesc &d = cd[lo];
     const size_t q = (size_t)d.chk_q0 + (g - ck_off[lo]);
-    const double ax = d.acc[3 * q], ay = d.acc[3 * q + 1], az = d.acc[3 * q + 2];
-    const double jx = d.adot[3 * q], jy = d.adot[3 * q + 1], jz = d.adot[3 * q + 2];
-    if ((double)d.spread[2 * q + 0] * k2 > ax * ax + ay * ay + az * az ||
-        (double)d.spread[2 * q + 1] * k2 > jx * jx + jy * jy + jz * jz) {
+    if (check_rule(d.spread[2 * q + 0], d.spread[2 * q + 1], d.acc[3 * q], d.acc[3 * q + 1],
+                   d.acc[3 * q + 2], d.adot[3 * q], d.adot[3 * q + 1], d.adot[3 * q + 2], k2)) {
         const int k = atomicAdd(d.count, 1);
         d.fq[k] = (int)q;
         d.fi[k] = d.tgt[q];
@@ -1079,25 +1107,25 @@ hipError_t launch_t(const JState &js, const int *tgt, int nt, const Plan &p, con
 // tm: term mode; est: the spread estimate of checked precision (mode 0)
 template <int JL>
 hipError_t launch_jl(const JState &js, const int *tgt, int nt, const Plan &p, const QueryOut &out,
-                     int cap, int tm, bool est, hipStream_t st)
+                     int cap, int tm, bool est, int *reset, hipStream_t st)
 {
     dim3 grid(p.grid_x, p.splits);
     const int fc = p.splits == 1 ? 1 : 0;
     if (tm == 0 && est)
         hipLaunchKernelGGL((regf_kernel_jl<JL, 0, true>), grid, dim3(kBlock), 0, st, js, tgt, nt,
-                           p.jspan, out, cap, fc);
+                           p.jspan, out, cap, fc, reset);
     else if (tm == 0)
         hipLaunchKernelGGL((regf_kernel_jl<JL, 0, false>), grid, dim3(kBlock), 0, st, js, tgt,
-                           nt, p.jspan, out, cap, fc);
+                           nt, p.jspan, out, cap, fc, reset);
     else if (tm == 1)
         hipLaunchKernelGGL((regf_kernel_jl<JL, 1, false>), grid, dim3(kBlock), 0, st, js, tgt,
-                           nt, p.jspan, out, cap, fc);
+                           nt, p.jspan, out, cap, fc, reset);
     else if (tm == 2)
         hipLaunchKernelGGL((regf_kernel_jl<JL, 2, false>), grid, dim3(kBlock), 0, st, js, tgt,
-                           nt, p.jspan, out, cap, fc);
+                           nt, p.jspan, out, cap, fc, reset);
     else
         hipLaunchKernelGGL((regf_kernel_jl<JL, 3, false>), grid, dim3(kBlock), 0, st, js, tgt,
-                           nt, p.jspan, out, cap, fc);
+                           nt, p.jspan, out, cap, fc, reset);
     return hipGetLastError();
 }
 
@@ -1130,15 +1158,16 @@ int targets_per_block(const Plan &p)
 }
 
 hipError_t launch_regf(const JState &js, const int *tgt, int nt, const Plan &p,
-                       const QueryOut &out, int cap, int prec, hipStream_t st)
+                       const QueryOut &out, int cap, int prec, hipStream_t st, int *reset)
 {
     if (!variant_supported(prec, p.ipl, p.unroll, p.jl)) return hipErrorInvalidValue;
     const int tm = term_mode(prec);
     const bool est = prec == ABYSS_PREC_CHECKED;
     if (est && !out.spread) return hipErrorInvalidValue;
-    if (p.jl == 4) return launch_jl<4>(js, tgt, nt, p, out, cap, tm, est, st);
-    if (p.jl == 16) return launch_jl<16>(js, tgt, nt, p, out, cap, tm, est, st);
-    if (p.jl == 64) return launch_jl<64>(js, tgt, nt, p, out, cap, tm, est, st);
+    if (p.jl == 4) return launch_jl<4>(js, tgt, nt, p, out, cap, tm, est, reset, st);
+    if (p.jl == 16) return launch_jl<16>(js, tgt, nt, p, out, cap, tm, est, reset, st);
+    if (p.jl == 64) return launch_jl<64>(js, tgt, nt, p, out, cap, tm, est, reset, st);
+    if (reset) return hipErrorInvalidValue;   // the lanes-per-target kernels only
     if (prec == ABYSS_PREC_FP64) {
         if (p.ipl == 1) return launch_t<1, 2, true>(js, tgt, nt, p, out, cap, st);
         return launch_t<2, 2, true>(js, tgt, nt, p, out, cap, st);
@@ -1156,13 +1185,14 @@ hipError_t launch_regf(const JState &js, const int *tgt, int nt, const Plan &p,
 
 hipError_t launch_finalize(const QueryOut &part, int S, int nt, int cap, double *acc,
                            double *adot, int *nnb, int *list, const FinalMirror &mirror,
-                           float *spread, hipStream_t st)
+                           float *spread, hipStream_t st, const CheckFlag &flag)
 {
     if (spread && !part.spread) return hipErrorInvalidValue;
+    if (flag.counter && !spread) return hipErrorInvalidValue;
     dim3 grid((nt + 3) / 4);
     const size_t lds = 4 * (size_t)(S + 1) * sizeof(int);
     hipLaunchKernelGGL(finalize_kernel, grid, dim3(256), lds, st, part, S, nt, cap, acc, adot, nnb,
-                       list, mirror, spread);
+                       list, mirror, spread, flag);
     return hipGetLastError();
 }
 

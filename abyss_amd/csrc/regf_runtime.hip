@@ -12,13 +12,13 @@
 // move only the populated columns of the neighbour lists; small target lists
 // are read by the kernels from mapped host memory.
 #include "regf_context.h"
+#include "regf_pack.h"
 
+#include <limits.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
-
-#include <limits.h>
 
 #include <algorithm>
 #include <mutex>
@@ -360,20 +360,30 @@ int recompute_flagged(abyss_ctx_t c, int m)
     return ABYSS_OK;
 }
 
+// The checked precision's k2 = (ABYSS_CHECK_ERR_SCALE / tol)^2.
+double check_k2(abyss_ctx_t c)
+{
+    const double ratio = ABYSS_CHECK_ERR_SCALE / c->check_tol;
+    return ratio * ratio;
+}
+
 // After a checked launch's force and finalize: flags the targets whose a or
-// jdot may miss the tolerance and recomputes them with FP64 terms over the
+// jdot may miss the tolerance (unless the finalize of a split launch has
+// flagged them: `flagged`) and recomputes them with FP64 terms over the
 // same regular set.  Blocks of up to kMappedTargets targets plan the
 // recomputation for all of them and read the flagged count on the device
 // (no host wait; the count reaches h_chk[1]); larger blocks read the count
 // back first (a host wait) and plan for it.  `mirror`: the finalize wrote
 // the mapped host copy, which the recomputed targets must update too.
-int recheck(abyss_ctx_t c, bool mirror)
+int recheck(abyss_ctx_t c, bool mirror, bool flagged)
 {
-    const double ratio = ABYSS_CHECK_ERR_SCALE / c->check_tol;
-    HIPCHK(c, abyss::launch_batch_reset(c->chk_counters, c->stream));
-    HIPCHK(c, abyss::launch_check_flag(c->d_acc, c->d_adot, c->chk_spread, c->tgt, c->nt,
-                                       ratio * ratio, c->chk_counters, c->chk_fq, c->chk_fi,
-                                       c->stream));
+    if (!flagged) {
+        // one-split launches: a flag pass over the final results (the force
+        // kernel zeroed the counter)
+        HIPCHK(c, abyss::launch_check_flag(c->d_acc, c->d_adot, c->chk_spread, c->tgt, c->nt,
+                                           check_k2(c), c->chk_counters, c->chk_fq, c->chk_fi,
+                                           c->stream));
+    }
     if (c->nt <= kMappedTargets) {
         const Plan rp = plan_for(c, c->nt, c->n, ABYSS_PREC_REFINED, 0, false);
         QueryOut rpart;
@@ -390,6 +400,21 @@ int recheck(abyss_ctx_t c, bool mirror)
                              c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return recompute_flagged(c, c->h_chk[0]);
+}
+
+// Force-kernel timing: adds up the event pairs recorded so far (waits for
+// the last of them) and empties the ring.
+int collect_kernel_timing(abyss_ctx_t c)
+{
+    for (int k = 0; k < c->kev_used; ++k) {
+        float ms = 0.0f;
+        HIPCHK(c, hipEventSynchronize(c->kev[2 * k + 1]));
+        HIPCHK(c, hipEventElapsedTime(&ms, c->kev[2 * k], c->kev[2 * k + 1]));
+        c->kforce_ms += ms;
+        c->klaunches += 1;
+    }
+    c->kev_used = 0;
+    return ABYSS_OK;
 }
 
 int collect_timing(abyss_ctx_t c)
@@ -416,6 +441,11 @@ void release_all(abyss_ctx_t c)
     hfree(c->h_drv);
     hfree(c->h_state); hfree(c->h_upd); hfree(c->h_irr);
     hfree(c->h_tgt); hfree(c->h_out); hfree(c->h_list);
+    for (hipEvent_t &e : c->kev)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -614,19 +644,6 @@ int abyss_ctx_upload(abyss_ctx_t c, int n, const double *m, const double *x, con
     c->jset_version += 1;
     if (n <= 0 || !m || !x || !v || !r2)
         return set_detail(c, ABYSS_ERR_INVALID_ARG, "upload: n=%d or null array", n);
-    // Validate everything before touching the device.
-    for (int j = 0; j < n; ++j) {
-        for (int d = 0; d < 3; ++d) {
-            if (!isfinite(x[3 * j + d]) || !isfinite(v[3 * j + d]))
-                return set_detail(c, ABYSS_ERR_INVALID_INPUT,
-                                  "upload: non-finite position/velocity at j=%d", j);
-        }
-        if (!isfinite(m[j]) || m[j] < 0.0)
-            return set_detail(c, ABYSS_ERR_INVALID_INPUT, "upload: bad mass %g at j=%d", m[j], j);
-        if (isnan(r2[j]) || r2[j] < 0.0)
-            return set_detail(c, ABYSS_ERR_INVALID_INPUT, "upload: bad r_nb^2 %g at j=%d", r2[j],
-                              j);
-    }
     HIPCHK(c, hipSetDevice(c->device));
     // The previous upload / query may still read the staging buffer.
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -640,20 +657,22 @@ int abyss_ctx_upload(abyss_ctx_t c, int n, const double *m, const double *x, con
         HIPCHK(c, hipHostMalloc(&c->h_j, cap_b, hipHostMallocDefault));
         c->j_bytes = cap_b;
     }
-    // AoS -> SoA pack (src/cuda/cuda_my_acceleration.cu:399-405 packs the
-    // same arrays with stride N).
+    // Validation and AoS -> SoA pack in one pass over the caller's arrays,
+    // on a few host threads for large N, before anything is copied
+    // (src/cuda/cuda_my_acceleration.cu:399-405 packs the same arrays with
+    // stride N).
     double *h = reinterpret_cast<double *>(c->h_j);
-    float *h32 = reinterpret_cast<float *>(h + 8 * (size_t)n);
-    for (int j = 0; j < n; ++j) {
-        h[0 * (size_t)n + j] = x[3 * j + 0];
-        h[1 * (size_t)n + j] = x[3 * j + 1];
-        h[2 * (size_t)n + j] = x[3 * j + 2];
-        h[3 * (size_t)n + j] = v[3 * j + 0];
-        h[4 * (size_t)n + j] = v[3 * j + 1];
-        h[5 * (size_t)n + j] = v[3 * j + 2];
-        h[6 * (size_t)n + j] = m[j];
-        h[7 * (size_t)n + j] = r2[j];
-        h32[j] = (float)m[j];
+    const abyss::PackStatus st =
+        abyss::pack_jset(n, m, x, v, r2, h, reinterpret_cast<float *>(h + 8 * (size_t)n));
+    if (st.index >= 0) {
+        const int j = st.index;
+        if (st.kind == abyss::kPackBadState)
+            return set_detail(c, ABYSS_ERR_INVALID_INPUT,
+                              "upload: non-finite position/velocity at j=%d", j);
+        if (st.kind == abyss::kPackBadMass)
+            return set_detail(c, ABYSS_ERR_INVALID_INPUT, "upload: bad mass %g at j=%d", m[j],
+                              j);
+        return set_detail(c, ABYSS_ERR_INVALID_INPUT, "upload: bad r_nb^2 %g at j=%d", r2[j], j);
     }
     HIPCHK(c, hipMemcpyAsync(c->d_j, c->h_j, need, hipMemcpyHostToDevice, c->stream));
     bind_jstate(c, c->d_j, n);
@@ -755,25 +774,44 @@ int ctx_launch_force(abyss_ctx_t c)
     // (two event records and their readback cost a small query ~7 us)
     c->mirror_valid = false;
     if (c->timed) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    if (p.splits == 1) {
-        HIPCHK(c, abyss::launch_regf(c->js, c->tgt, c->nt, p, fin, c->cap, c->precision,
-                                     c->stream));
-    } else {
-        HIPCHK(c, abyss::launch_regf(c->js, c->tgt, c->nt, p, part, c->cap, c->precision,
-                                     c->stream));
+    if (c->ktiming) {
+        if (c->kev_used == abyss_ctx_s::kTimingRing) {
+            int rc = collect_kernel_timing(c);
+            if (rc != ABYSS_OK) return rc;
+        }
+        HIPCHK(c, hipEventRecord(c->kev[2 * c->kev_used], c->stream));
+    }
+    // checked precision: the force kernel zeroes the flagged-target counter
+    // and a split launch's finalize flags the targets (no launches of their
+    // own)
+    HIPCHK(c, abyss::launch_regf(c->js, c->tgt, c->nt, p, p.splits == 1 ? fin : part, c->cap,
+                                 c->precision, c->stream, checked ? c->chk_counters : nullptr));
+    if (c->ktiming) {
+        HIPCHK(c, hipEventRecord(c->kev[2 * c->kev_used + 1], c->stream));
+        c->kev_used += 1;
+    }
+    if (p.splits > 1) {
         // small results also go straight into the mapped host block, so a
         // following fetch needs no copy (a checked recomputation updates it
         // too)
         const bool mirror = c->nt <= kMappedTargets && !c->batch_member &&
                             (size_t)c->nt * c->cap * sizeof(int) <= kOneTripListBytes;
+        abyss::CheckFlag flag;
+        if (checked) {
+            flag.k2 = check_k2(c);
+            flag.counter = c->chk_counters;
+            flag.fq = c->chk_fq;
+            flag.fi = c->chk_fi;
+            flag.tgt = c->tgt;
+        }
         HIPCHK(c, abyss::launch_finalize(part, p.splits, c->nt, c->cap, c->d_acc, c->d_adot,
                                          c->d_nnb, c->d_list,
                                          mirror ? c->mirror : abyss::FinalMirror{},
-                                         fin.spread, c->stream));
+                                         fin.spread, c->stream, flag));
         c->mirror_valid = mirror;
     }
     if (checked) {
-        int rc = recheck(c, c->mirror_valid);
+        int rc = recheck(c, c->mirror_valid, p.splits > 1);
         if (rc != ABYSS_OK) return rc;
         c->spread_valid = true;
     }
@@ -804,7 +842,34 @@ int abyss_ctx_sync(abyss_ctx_t c)
     if (!c) return ABYSS_ERR_INVALID_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc = collect_kernel_timing(c);
+    if (rc != ABYSS_OK) return rc;
     return collect_timing(c);
+}
+
+int abyss_ctx_set_kernel_timing(abyss_ctx_t c, int enable)
+{
+    if (!c) return ABYSS_ERR_INVALID_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (enable && !c->kev[0])
+        for (hipEvent_t &e : c->kev) HIPCHK(c, hipEventCreate(&e));
+    int rc = collect_kernel_timing(c);   // pairs of earlier launches
+    if (rc != ABYSS_OK) return rc;
+    c->ktiming = enable != 0;
+    c->kforce_ms = 0.0;
+    c->klaunches = 0;
+    return ABYSS_OK;
+}
+
+int abyss_ctx_kernel_timing(abyss_ctx_t c, double *force_ms, long long *launches)
+{
+    if (!c || !force_ms || !launches) return ABYSS_ERR_INVALID_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc = collect_kernel_timing(c);
+    if (rc != ABYSS_OK) return rc;
+    *force_ms = c->kforce_ms;
+    *launches = c->klaunches;
+    return ABYSS_OK;
 }
 
 }  // extern "C"

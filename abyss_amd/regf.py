@@ -74,7 +74,8 @@ EXPORTS = [
     "abyss_srv_queue_stats", "abyss_ctx_set_check_tolerance", "abyss_ctx_last_recheck",
     "abyss_ctx_fetch_spread", "abyss_ens_set_precision", "abyss_ens_set_check_tolerance",
     "abyss_ens_last_recheck", "abyss_srv_set_precision", "abyss_srv_set_check_tolerance",
-    "abyss_set_legacy_errors", "abyss_query_noalloc",
+    "abyss_set_legacy_errors", "abyss_query_noalloc", "abyss_ctx_set_kernel_timing",
+    "abyss_ctx_kernel_timing", "abyss_srv_gpu_busy",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -128,6 +129,9 @@ def load():
         "abyss_ctx_stream": (_vp, [_vp]),
         "abyss_ctx_plan": (ctypes.c_int, [_vp, _ip, _ip, _ip, _ip, _ip, _ip]),
         "abyss_ctx_error_detail": (ctypes.c_char_p, [_vp]),
+        "abyss_ctx_set_kernel_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "abyss_srv_gpu_busy": (ctypes.c_int, [_vp, _dp, _dp]),
+        "abyss_ctx_kernel_timing": (ctypes.c_int, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
         "abyss_ctx_set_check_tolerance": (ctypes.c_int, [_vp, ctypes.c_double]),
         "abyss_query_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp,
                                                ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp,
@@ -408,6 +412,18 @@ class Context:
 
     def set_scratch_limit(self, nbytes):
         self._chk(self._lib.abyss_ctx_set_scratch_limit(self.h, int(nbytes)))
+
+    def kernel_timing(self, enable):
+        """Record HIP events around the force kernel of every launch (and
+        reset the totals)."""
+        self._chk(self._lib.abyss_ctx_set_kernel_timing(self.h, int(bool(enable))))
+
+    def read_kernel_timing(self):
+        """{"force_ms": summed force-kernel time, "launches": count} since
+        kernel_timing was last switched."""
+        ms, n = ctypes.c_double(), ctypes.c_longlong()
+        self._chk(self._lib.abyss_ctx_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return {"force_ms": ms.value, "launches": n.value}
 
     def upload(self, m, x, v, r2):
         x = _f64(x, (-1, 3))
@@ -769,9 +785,13 @@ class Server:
         rej, mp, peak = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_double()
         self._lib.abyss_srv_queue_stats(self.h, ctypes.byref(rej), ctypes.byref(mp),
                                         ctypes.byref(peak))
+        busy, wall = ctypes.c_double(), ctypes.c_double()
+        self._lib.abyss_srv_gpu_busy(self.h, ctypes.byref(busy), ctypes.byref(wall))
         return {"completed": c.value, "batches": b.value, "interactions": i.value,
                 "p50_ms": p50.value, "p99_ms": p99.value, "rejected": rej.value,
-                "max_pending": mp.value, "peak_outstanding_pairs": peak.value}
+                "max_pending": mp.value, "peak_outstanding_pairs": peak.value,
+                "gpu_busy_ms": busy.value, "wall_ms": wall.value,
+                "gpu_busy_frac": busy.value / wall.value if wall.value > 0 else 0.0}
 
     def reset_stats(self):
         self._lib.abyss_srv_reset_stats(self.h)

@@ -4,19 +4,28 @@
 Workload (default): King W0=6 cluster, N=262144, Kroupa IMF, 5% of the
 particles in hard binaries, r_nb for ~100 neighbours, list cap 256; one
 step = one full-N regular-force + neighbour-list evaluation (m = N targets)
-through the library's device-resident launch path (inputs already in HBM).
+through the library's device-resident launch path (inputs already in HBM),
+in the library's default precision (checked: every target's a and jdot
+within 1e-6 of FP64; --precision split / fp64 for the others).
 
 Prints ONE JSON line.  Progress goes to stderr.
-  value     = interactions / s = steps * m * (N - 1) / (HIP-event time)
-  roofline  = FP32 VALU roof: 60 flop per interaction (the reference's
-              convention) / kernel time vs 157.3 TFLOP/s
+  value     = interactions / s = steps * m * (N - 1) / (HIP-event time of the
+              whole step: force kernel, merge, flagging and recomputation)
+  pct_fp32_peak = the step's rate at 60 flop per interaction (the reference's
+              convention) against the 157.3 TFLOP/s FP32 vector peak
+  roofline  = the force kernel alone: the same flop over its own HIP-event
+              time (events around it on the context stream, every timed
+              launch); traffic = its HBM bytes per launch from two rocprofv3
+              counter passes in this run (FETCH_SIZE, WRITE_SIZE, each scaled
+              by its reading for a copy of known size)
   cpu_baseline = the oracle's FP64 C loop (OpenMP over targets) on a bounded
               sample of the same workload on this host's cores; its results on
               256 of those targets also check the GPU results
-  checked_precision = the same step under the checked precision (targets whose
-              a or jdot rounding may exceed 1e-6 recomputed with FP64 terms)
+  other_precisions = the same step in the other of split / checked precision
   accuracy  = per-particle errors over every target against the FP64 kernel
               (split and checked precision), plus the oracle sample
+  active_blocks = Plummer + Kroupa N = 65536: blocks of 1, 64, 4096, 65536
+              targets (launch, synchronous query, legacy upload + call)
 """
 import argparse
 import ctypes
@@ -29,6 +38,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3
+# the precision modes a run can name (the library's ABYSS_PREC_* codes)
+PREC_NAMES = ("checked", "split", "fp64")
+
+
+def prec_code(name):
+    from abyss_amd import regf
+    return {"checked": regf.PREC_CHECKED, "split": regf.PREC_SPLIT,
+            "fp64": regf.PREC_FP64}[name]
 FLOP_PER_INTERACTION = 60
 HBM_PEAK_BPS = 8.0e12
 
@@ -93,17 +110,88 @@ def copy_calibration(nbytes, reps=4):
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
 
 
-def profiled_traffic(name="regf_kernel_king262144_fullN_traffic.md"):
-    """HBM bytes per launch from the committed rocprofv3 counter summary
-    (FETCH_SIZE + WRITE_SIZE, calibrated against a copy of known size)."""
-    path = os.path.join(ROOT, "profiles", name)
-    try:
-        for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1])
-    except OSError:
-        pass
-    return None
+def _counter_rows(outdir):
+    """(kernel name, counter value) per dispatch from rocprofv3's CSV files
+    under outdir, in dispatch order."""
+    import csv
+    import glob
+    rows = []
+    for f in sorted(glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"),
+                              recursive=True)):
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                rows.append((int(r.get("Dispatch_Id") or 0), r.get("Kernel_Name", ""),
+                             float(r.get("Counter_Value") or 0.0)))
+    rows.sort()
+    return [(k, v) for _, k, v in rows]
+
+
+def live_traffic(precision, kernel="regf_kernel_jl", timeout=300):
+    """HBM bytes of one launch of the force kernel, measured in this run: two
+    rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch)
+    over `bench.py --traffic-probe`, which copies 256 MiB device to device
+    (the reference that scales each counter to bytes) and then runs two
+    full-N steps (the second is read).  Returns (info, None) or (None, why)."""
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    if "rocprofiler" in os.environ.get("LD_PRELOAD", ""):
+        return None, "bench.py itself runs under a profiler"
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(exe):
+        return None, "rocprofv3 not found"
+    calib = 256 << 20
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix=f"abyss_pmc_{counter.lower()}_")
+        cmd = [exe, "--pmc", counter, "-d", d, "-o", "run", "-f", "csv", "--",
+               sys.executable, os.path.abspath(__file__), "--traffic-probe",
+               "--precision", precision]
+        p = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                             start_new_session=True, cwd=ROOT)
+        try:
+            _, err = p.communicate(timeout=timeout)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.wait()
+            return None, f"rocprofv3 --pmc {counter} timed out"
+        rows = _counter_rows(d)
+        shutil.rmtree(d, ignore_errors=True)
+        copy = [v for k, v in rows if "copyBuffer" in k]
+        force = [v for k, v in rows if kernel in k]
+        if p.returncode != 0 and not (copy and force):
+            return None, f"rocprofv3 --pmc {counter} failed ({p.returncode}): " + \
+                err.decode(errors="replace")[-300:]
+        if not copy or not force:
+            return None, f"rocprofv3 --pmc {counter}: no copy / force-kernel rows"
+        factor = (calib / 1024.0) / (sum(copy) / len(copy))
+        out[counter] = {"kib": force[-1], "copy_kib": sum(copy) / len(copy), "factor": factor,
+                        "bytes": force[-1] * 1024.0 * factor}
+    total = out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]
+    return {"bytes": total, "fetch_bytes": out["FETCH_SIZE"]["bytes"],
+            "write_bytes": out["WRITE_SIZE"]["bytes"],
+            "fetch_factor": out["FETCH_SIZE"]["factor"],
+            "write_factor": out["WRITE_SIZE"]["factor"],
+            "method": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch) of the "
+                      "second full-N launch, each scaled by the same counter's reading for a "
+                      "256 MiB device-to-device copy"}, None
+
+
+def traffic_probe(args):
+    """Counter-run body of live_traffic: a 256 MiB copy, then two full-N
+    steps of the King N=262144 workload in the given precision."""
+    import numpy as np
+    from abyss_amd import regf, workloads
+    copy_calibration(256 << 20, reps=2)
+    js = workloads.king_binaries(args.n)
+    ctx = regf.Context(device=0, cap=js.cap, precision=prec_code(args.precision))
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    ctx.set_targets(np.arange(js.n, dtype=np.int32))
+    for _ in range(2):
+        ctx.launch()
+        ctx.sync()
+    ctx.close()
 
 
 def cpu_baseline(js, target_seconds=10.0, gpu_results=None, nsample=256):
@@ -180,29 +268,32 @@ def full_n_errors(res, ref):
             "jdot_over_1e-6": int((ej > 1e-6).sum()), "a_over_1e-6": int((ea > 1e-6).sum())}
 
 
-def checked_leg(ctx, targets, inter, reps=5):
-    """Checked precision on the same context and targets: wall time per
-    synchronous launch (median of reps) and the recomputed target count."""
+def precision_leg(ctx, targets, inter, name, reps=5):
+    """The same step on the same context and targets in another precision:
+    HIP-event time per launch over `reps` launches (checked precision waits
+    on the host inside each launch), restoring the context's mode after."""
     import numpy as np
     from abyss_amd import regf
-    ctx.set_precision(regf.PREC_CHECKED)
+    ev = HipEvents()
+    ctx.set_precision(prec_code(name))
     ctx.set_targets(targets)
     ctx.launch()
     ctx.sync()
-    ts = []
+    e0, e1 = ev.create(), ev.create()
+    ev.record(e0, ctx.stream)
     for _ in range(reps):
-        t0 = time.perf_counter()
         ctx.launch()
-        ctx.sync()
-        ts.append(time.perf_counter() - t0)
-    ms = float(np.median(ts)) * 1e3
+    ev.record(e1, ctx.stream)
+    ms = ev.elapsed_ms(e0, e1) / reps
+    ctx.sync()
     info = {"ms_per_step": ms, "value": inter / (ms * 1e-3),
             "pct_fp32_peak": 100.0 * FLOP_PER_INTERACTION * inter / (ms * 1e-3) / 1e12
                              / FP32_PEAK_TFLOPS,
-            "recomputed_targets": ctx.last_recheck(), "tolerance": 1e-6,
-            "timing": f"wall clock per launch + sync, median of {reps}"}
+            "timing": f"HIP events over {reps} launches"}
+    if name == "checked":
+        info["recomputed_targets"] = ctx.last_recheck()
+        info["tolerance"] = 1e-6
     res = ctx.fetch()
-    ctx.set_precision(regf.PREC_SPLIT)
     return info, res
 
 
@@ -366,11 +457,13 @@ def bench_ensemble(args):
 ADMIT_PAIRS = 3.0e11
 
 
-def serve_stream(srv, ids, sizes, rate, window_s, seed=9):
+def serve_stream(srv, ids, sizes, rate, window_s, seed=9, check_every=0):
     """Poisson request stream at `rate` req/s for `window_s` seconds; each
     request is one regular step of a random cluster with an active block
-    log-uniform in [1, N].  Returns (latencies_ms, interactions, wall_s,
-    rejected) over the requests the server admitted."""
+    log-uniform in [1, N].  With check_every = k, every k-th admitted
+    request's results are copied back and kept.  Returns (latencies_ms,
+    interactions, wall_s, rejected, kept) over the admitted requests, kept =
+    [(cluster index, targets, results)]."""
     import threading
     import numpy as np
     rng = np.random.default_rng(seed)
@@ -381,10 +474,10 @@ def serve_stream(srv, ids, sizes, rate, window_s, seed=9):
         c = int(rng.integers(len(ids)))
         n = sizes[c]
         mblk = int(min(n, max(1, np.exp(rng.uniform(0.0, np.log(n + 1))))))
-        reqs.append((ids[c], rng.choice(n, mblk, replace=False).astype(np.int32), n))
+        reqs.append((c, rng.choice(n, mblk, replace=False).astype(np.int32), n))
     tickets = [None] * len(reqs)
     lat = [None] * len(reqs)
-    done_sub = threading.Event()
+    kept = []
 
     def waiter():
         q = 0
@@ -393,26 +486,55 @@ def serve_stream(srv, ids, sizes, rate, window_s, seed=9):
                 time.sleep(0.0002)
             tk, nt = tickets[q]
             if tk is not None:   # None: turned away by admission
-                lat[q] = srv.wait(tk, nt, copy=False)
+                if check_every and q % check_every == 0:
+                    *res, lat[q] = srv.wait(tk, nt)
+                    kept.append((reqs[q][0], reqs[q][1], res))
+                else:
+                    lat[q] = srv.wait(tk, nt, copy=False)
             q += 1
 
     wt = threading.Thread(target=waiter)
     wt.start()
     t0 = time.perf_counter()
-    for q, (cid, t, n) in enumerate(reqs):
+    for q, (c, t, n) in enumerate(reqs):
         delay = t_arr[q] - (time.perf_counter() - t0)
         if delay > 0:
             time.sleep(delay)
-        tickets[q] = srv.submit(cid, t)
-    done_sub.set()
+        tickets[q] = srv.submit(ids[c], t)
     wt.join()
     wall = time.perf_counter() - t0
     ok = [q for q in range(len(reqs)) if tickets[q][0] is not None]
     inter = float(sum(len(reqs[q][1]) * (reqs[q][2] - 1) for q in ok))
-    return np.array([lat[q] for q in ok]), inter, wall, len(reqs) - len(ok)
+    return np.array([lat[q] for q in ok]), inter, wall, len(reqs) - len(ok), kept
 
 
-def bench_blocks(args):
+def check_served(cl, kept, nsample=4):
+    """a and jdot of `nsample` targets of each kept request against the
+    oracle (relative error, rows whose neighbour set equals the oracle's),
+    and neighbour counts / lists on the same targets."""
+    import numpy as np
+    import oracle
+    worst_a = worst_j = 0.0
+    list_mismatch = 0
+    for k, (c, t, (acc, adot, nnb, lst)) in enumerate(kept):
+        x, v, m, r2 = cl[c]
+        pick = np.random.default_rng(k).permutation(len(t))[:nsample]
+        ra, rj, rn, rl = oracle.regf(x, v, m, r2, t[pick], lst.shape[1])
+        w = lst.shape[1]
+        valid = np.arange(w)[None, :] < np.minimum(np.abs(nnb[pick]), w)[:, None]
+        rows = (nnb[pick] == rn) & np.all((lst[pick] == rl) | ~valid, axis=1)
+        list_mismatch += int((~rows).sum())
+        if rows.any():
+            ea = np.linalg.norm(acc[pick] - ra, axis=1) / np.linalg.norm(ra, axis=1)
+            ej = np.linalg.norm(adot[pick] - rj, axis=1) / np.linalg.norm(rj, axis=1)
+            worst_a = max(worst_a, float(ea[rows].max()))
+            worst_j = max(worst_j, float(ej[rows].max()))
+    return {"requests_checked": len(kept), "targets_per_request": nsample,
+            "a_rel_max": worst_a, "jdot_rel_max": worst_j,
+            "rows_differing_from_oracle": list_mismatch}
+
+
+def bench_blocks(args, sizes=(1, 4, 16, 64, 256, 1024, 4096, 16384, 65536)):
     """Active-block sweep on Plummer + Kroupa N = 65536 (cap 256, ~100
     neighbours): per block size m, the device-resident launch (HIP events on
     the context's stream, inputs in HBM) and the synchronous query (targets
@@ -424,14 +546,16 @@ def bench_blocks(args):
     from abyss_amd import regf, workloads
     js = workloads.plummer_kroupa(65536)
     n = js.n
-    ctx = regf.Context(device=0, cap=js.cap)
+    prec = prec_code(args.precision)
+    ctx = regf.Context(device=0, cap=js.cap, precision=prec)
     ctx.upload(js.m, js.x, js.v, js.r2)
     regf.set_num_neighbor_max(js.cap)
+    regf.set_precision(prec)
     regf.OpenDevice(0)
     ev = HipEvents()
     rng = np.random.default_rng(7)
     rows = []
-    for m in (1, 4, 16, 64, 256, 1024, 4096, 16384, 65536):
+    for m in sizes:
         t = np.sort(rng.permutation(n)[:m]).astype(np.int32)
         inter = float(m) * (n - 1)
         reps = int(min(2000, max(10, 0.2 / max(inter / 1.2e12, 1e-5))))
@@ -480,7 +604,9 @@ def bench_blocks(args):
     ctx.close()
     regf.CloseDevice()
     regf.set_num_neighbor_max(1000)
-    return {"workload": "plummer_kroupa_n65536_cap256", "rows": rows}
+    regf.set_precision(regf.PREC_DEFAULT)
+    return {"workload": "plummer_kroupa_n65536_cap256", "precision": args.precision,
+            "rows": rows}
 
 
 def bench_neighbour(args, reps=5):
@@ -552,6 +678,11 @@ def bench_neighbour(args, reps=5):
 
 
 def bench_serve(args, rate=2000.0, window_s=1.0):
+    """Served ensemble (64 clusters, N log-uniform in [4096, 1048576]) at the
+    library's default precision (checked): Poisson streams at the nominal
+    2000 req/s with admission, and at 100 and 300 req/s without; p50/p99
+    latency, interactions/s, the GPU-busy fraction, and an oracle check of a
+    sample of the served results."""
     import numpy as np
     from abyss_amd import regf
     t0 = time.perf_counter()
@@ -567,30 +698,33 @@ def bench_serve(args, rate=2000.0, window_s=1.0):
     for c in ids:
         tk, nt = srv.submit(c, np.arange(8, dtype=np.int32))
         srv.wait(tk, nt, copy=False)
-    out = {}
-    # nominal: every request queues (unbounded); nominal_admission: requests
-    # that would take the outstanding work past ADMIT_PAIRS (~0.25 s of the
-    # chip's work) are turned away; sustained: a rate the chip keeps up with;
-    # sustained_checked: the same under checked precision
-    legs = (("nominal", rate, window_s, 0.0), ("nominal_admission", rate, window_s, ADMIT_PAIRS),
-            ("sustained", args.serve_rate, 4.0, 0.0),
-            ("sustained_checked", args.serve_rate, 4.0, 0.0))
-    for name, r, w, admit in legs:
-        srv.set_precision(regf.PREC_CHECKED if name.endswith("checked") else regf.PREC_SPLIT)
+    out = {"precision": "checked"}
+    # nominal_admission: requests that would take the outstanding work past
+    # ADMIT_PAIRS (~0.25 s of the chip's work) are turned away; 100 and 300
+    # req/s: every request queues
+    legs = (("nominal_admission", rate, window_s, ADMIT_PAIRS, 0),
+            ("rate100", args.serve_rate, 4.0, 0.0, 10),
+            ("rate300", 3 * args.serve_rate, 3.0, 0.0, 0))
+    kept_all = []
+    for name, r, w, admit, check_every in legs:
         srv.set_admission(admit)
         srv.reset_stats()
-        lat, inter, wall, rejected = serve_stream(srv, ids, sizes, r, w)
+        lat, inter, wall, rejected, kept = serve_stream(srv, ids, sizes, r, w,
+                                                        check_every=check_every)
         st = srv.stats()
+        kept_all += kept
         out[name] = {"rate_req_s": r, "window_s": w, "admission_pairs": admit or None,
-                     "precision": "checked" if name.endswith("checked") else "split",
                      "requests": int(len(lat)) + rejected, "admitted": int(len(lat)),
                      "rejected": rejected, "max_queued": st["max_pending"],
                      "peak_outstanding_pairs": st["peak_outstanding_pairs"],
                      "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                      "interactions_per_s": inter / wall, "wall_s": wall, "batches": st["batches"],
-                     "mean_batch_size": st["completed"] / max(1, st["batches"])}
+                     "mean_batch_size": st["completed"] / max(1, st["batches"]),
+                     "gpu_busy_frac": st["gpu_busy_frac"]}
         log(f"serve {name}: {out[name]}")
     srv.close()
+    out["oracle_sample"] = check_served(cl, kept_all)
+    log(f"serve oracle sample: {out['oracle_sample']}")
     return out
 
 
@@ -600,7 +734,9 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=262144)
-    ap.add_argument("--precision", choices=["split", "fp64"], default="split")
+    ap.add_argument("--precision", choices=PREC_NAMES, default="checked",
+                    help="precision of the timed step (default: checked, the library "
+                         "default, every target within 1e-6 of FP64)")
     ap.add_argument("--ipl", type=int, default=0)
     ap.add_argument("--splits", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -608,7 +744,12 @@ def main():
                     help="skip the batched-ensemble and served-ensemble legs")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the built-in plan rules instead of tuning first")
-    ap.add_argument("--no-check", action="store_true", help="skip the sampled accuracy check")
+    ap.add_argument("--no-check", action="store_true", help="skip the accuracy legs")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 counter passes that measure HBM traffic")
+    ap.add_argument("--no-blocks", action="store_true",
+                    help="skip the compact active-block sweep (N = 65536)")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration)")
@@ -620,6 +761,9 @@ def main():
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.traffic_probe:
+        traffic_probe(args)
         return
     if args.workload == "ensemble":
         print(json.dumps({"metric": "batched ensemble regular-force pair-interactions/s",
@@ -643,6 +787,14 @@ def main():
               "(multi-GPU sharding is design-only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    # HBM traffic of the force kernel: counter passes in child processes,
+    # before this process opens the device
+    traffic = traffic_why = None
+    if not args.no_traffic and args.n == 262144:
+        t0 = time.perf_counter()
+        traffic, traffic_why = live_traffic(args.precision)
+        log(f"traffic {traffic or traffic_why} ({time.perf_counter() - t0:.1f} s)")
+
     from abyss_amd import regf, workloads
 
     t0 = time.perf_counter()
@@ -650,8 +802,7 @@ def main():
     log(f"workload {js.name}: built in {time.perf_counter() - t0:.1f} s")
     import numpy as np
     targets = np.arange(js.n, dtype=np.int32)
-    ctx = regf.Context(device=0, cap=js.cap,
-                       precision=regf.PREC_FP64 if args.precision == "fp64" else regf.PREC_SPLIT)
+    ctx = regf.Context(device=0, cap=js.cap, precision=prec_code(args.precision))
     ctx.set_tuning(args.ipl, args.splits)
     ctx.upload(js.m, js.x, js.v, js.r2)
     ctx.set_targets(targets)
@@ -664,6 +815,7 @@ def main():
     ev = HipEvents()
     e0, e1 = ev.create(), ev.create()
     stream = ctx.stream
+    ctx.kernel_timing(True)   # events around the force kernel of every launch
     w0 = time.perf_counter()
     ev.record(e0, stream)
     for _ in range(args.steps):
@@ -672,20 +824,30 @@ def main():
     ms = ev.elapsed_ms(e0, e1)
     ctx.sync()
     wall = time.perf_counter() - w0
+    kt = ctx.read_kernel_timing()
+    ctx.kernel_timing(False)
+    recomputed = ctx.last_recheck() if args.precision == "checked" else None
     ms_per_step = ms / args.steps
     inter = float(len(targets)) * (js.n - 1)
     value = inter / (ms_per_step * 1e-3)
-    achieved = FLOP_PER_INTERACTION * inter / (ms_per_step * 1e-3) / 1e12
+    flop = FLOP_PER_INTERACTION * inter
+    step_tflops = flop / (ms_per_step * 1e-3) / 1e12
+    kernel_ms = kt["force_ms"] / max(1, kt["launches"])
+    achieved = flop / (kernel_ms * 1e-3) / 1e12
     log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step (wall {wall:.3f} s), "
-        f"{value:.4e} interactions/s, {achieved:.2f} TFLOP/s")
+        f"{value:.4e} interactions/s, {step_tflops:.2f} TFLOP/s; force kernel "
+        f"{kernel_ms:.3f} ms ({achieved:.2f} TFLOP/s); recomputed {recomputed}")
 
-    acc_info = chk_info = None
+    acc_info = None
+    legs = {}
     gpu_results = {}
     if not args.no_check:
         gpu_results[args.precision] = ctx.fetch()
-        if args.precision == "split":
-            chk_info, gpu_results["checked"] = checked_leg(ctx, targets, inter)
-            log(f"checked precision {chk_info}")
+        # the other modes on the same targets, for their time and accuracy
+        for other in ("split", "checked"):
+            if other != args.precision:
+                legs[other], gpu_results[other] = precision_leg(ctx, targets, inter, other)
+                log(f"{other} precision {legs[other]}")
         ctx.set_precision(regf.PREC_FP64)
         f64 = ctx.query(targets)
         acc_info = {"full_n_vs_fp64_kernel": {k: full_n_errors(r, f64)
@@ -700,6 +862,9 @@ def main():
         if checks:
             acc_info["sample_vs_oracle"] = checks
             log(f"oracle sample {checks}")
+    blocks = None
+    if not args.no_blocks:
+        blocks = bench_blocks(args, sizes=(1, 64, 4096, 65536))
     ens_info = serve_info = None
     tuned = False
     if not args.no_ensemble:
@@ -717,6 +882,14 @@ def main():
         log(f"batched ensemble {ens_info}")
         serve_info = bench_serve(args)
 
+    arith = {"checked": "f64 i-j differences, f32 r^2 / rsqrt / terms, f32 runs of 64 pairs "
+                        "folded into f64 sums; targets whose rounding may exceed 1e-6 "
+                        "recomputed with f64 terms",
+             "split": "f64 i-j differences, f32 r^2 / rsqrt / terms, f32 runs of 64 pairs "
+                      "folded into f64 sums",
+             "fp64": "f64 throughout"}[args.precision]
+    kname = {"checked": "regf_kernel_jl<JL, 0, true>", "split": "regf_kernel_jl<JL, 0, false>",
+             "fp64": "regf_kernel<IPL, 2, true>"}[args.precision]
     line = {
         "metric": "regular-force pair-interactions/s",
         "value": value,
@@ -728,24 +901,27 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.precision == "split" else "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic",
         "config": {"workload": f"{js.name}_fullN", "n": js.n, "active": int(len(targets)),
                    "neighbours_target": js.meta["k_nb"], "cap": js.cap,
-                   "precision": args.precision,
-                   "arithmetic": "f64 i-j differences, f32 r^2 / rsqrt / terms, f32 runs of "
-                                 "64 pairs folded into f64 sums" if args.precision == "split"
-                                 else "f64 throughout",
-                   "plan": plan},
-        "pct_fp32_peak": 100.0 * achieved / FP32_PEAK_TFLOPS,
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                   "precision": args.precision, "arithmetic": arith, "plan": plan,
+                   "recomputed_targets": recomputed},
+        "pct_fp32_peak": 100.0 * step_tflops / FP32_PEAK_TFLOPS,
+        "roofline": {"bound": "valu", "kernel": kname,
+                     "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": profiled_traffic() if args.n == 262144 else None,
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "kernel_ms": kernel_ms,
+                     "kernel_timing": f"HIP events around the force kernel on the context "
+                                      f"stream, {kt['launches']} timed launches",
                      "flop_per_interaction": FLOP_PER_INTERACTION,
                      "interactions_per_launch": inter},
+        "traffic_detail": traffic if traffic else {"unmeasured": traffic_why},
         "cpu_baseline": cpu,
         "accuracy": acc_info,
-        "checked_precision": chk_info,
+        "other_precisions": legs or None,
+        "active_blocks": blocks,
         "batched_ensemble": ens_info,
         "served_ensemble": serve_info,
     }

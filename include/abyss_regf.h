@@ -210,6 +210,15 @@ int abyss_ctx_plan(abyss_ctx_t ctx, int *i_per_lane, int *j_splits,
                    int *grid_x, int *tile, int *j_unroll, int *lanes_per_target);
 /* Detail string of the context's last error. */
 const char *abyss_ctx_error_detail(abyss_ctx_t ctx);
+/* Force-kernel timing: with it on, every launch records a HIP event pair
+ * on the context stream around its pairwise force kernel (not the merge of
+ * the j-splits, nor checked precision's flagging and recomputation);
+ * kernel_timing returns the summed milliseconds and the number of launches
+ * since it was last switched (it waits for the last recorded launch).
+ * Counterpart of the GPU time ProfileDevice reports
+ * (src_mpi/cuda/cuda_my_acceleration.cu:1051-1062), per kernel. */
+int abyss_ctx_set_kernel_timing(abyss_ctx_t ctx, int enable);
+int abyss_ctx_kernel_timing(abyss_ctx_t ctx, double *force_ms, long long *launches);
 
 /* Checked precision (ABYSS_PREC_CHECKED).  A launch sums, for each target,
  * the squared norms of its FP32 run partials of a and of jdot (Sa, Sj; the
@@ -436,6 +445,11 @@ int abyss_srv_set_admission(abyss_srv_t srv, double max_outstanding_pairs);
  * reset. */
 int abyss_srv_queue_stats(abyss_srv_t srv, long long *rejected, long long *max_pending,
                           double *peak_outstanding_pairs);
+/* GPU time of the batches run since creation or the last reset (HIP events
+ * around each batch on the ensemble stream; a checked batch's host wait for
+ * its flagged count falls inside) and the wall-clock time over the same
+ * span, in ms: their ratio is the fraction of time the GPU was busy. */
+int abyss_srv_gpu_busy(abyss_srv_t srv, double *busy_ms, double *wall_ms);
 /* Estimated device bytes a cluster of n particles needs (admission rule). */
 unsigned long long abyss_srv_cluster_footprint(abyss_srv_t srv, int n);
 
