@@ -132,18 +132,20 @@ PackStatus pack_range(int n, int j0, int j1, const double *m, const double *x, c
 
 }  // namespace
 
-PackStatus pack_jset(int n, const double *m, const double *x, const double *v, const double *r2,
-                     double *soa, float *m32)
+PackStatus pack_jset(int n, int j0, int j1, const double *m, const double *x, const double *v,
+                     const double *r2, double *soa, float *m32)
 {
-    if (n < kPackParallelMin) return pack_range(n, 0, n, m, x, v, r2, soa, m32);
+    const int len = j1 - j0;
+    if (len < kPackParallelMin) return pack_range(n, j0, j1, m, x, v, r2, soa, m32);
     const int w = pool().workers();
-    const int parts = std::min(w + 1, (n + kPackParallelMin / 2 - 1) / (kPackParallelMin / 2));
+    const int parts = std::min(w + 1, (len + kPackParallelMin / 2 - 1) / (kPackParallelMin / 2));
     std::vector<PackStatus> st(parts);
     const auto job = [&](int p) {
-        const int j0 = (int)((long long)n * p / parts), j1 = (int)((long long)n * (p + 1) / parts);
-        st[p] = pack_range(n, j0, j1, m, x, v, r2, soa, m32);
+        const int a = j0 + (int)((long long)len * p / parts);
+        const int b = j0 + (int)((long long)len * (p + 1) / parts);
+        st[p] = pack_range(n, a, b, m, x, v, r2, soa, m32);
     };
-    if (parts <= 1 || !pool().run(parts, job)) return pack_range(n, 0, n, m, x, v, r2, soa, m32);
+    if (parts <= 1 || !pool().run(parts, job)) return pack_range(n, j0, j1, m, x, v, r2, soa, m32);
     for (const PackStatus &s : st)   // parts in index order: the first is the lowest
         if (s.index >= 0) return s;
     return {};

@@ -17,10 +17,11 @@ struct PackStatus {
     int kind = 0;
 };
 
-// Validates n particles (x, v: [n][3]; m, r2: [n]) and packs them into the
-// device layout: soa = 8 arrays of n doubles [x|y|z|vx|vy|vz|m|r2], m32 = the
-// masses rounded to FP32.  After a failure soa / m32 hold partial contents.
-PackStatus pack_jset(int n, const double *m, const double *x, const double *v, const double *r2,
-                     double *soa, float *m32);
+// Validates particles [j0, j1) of n (x, v: [n][3]; m, r2: [n]) and packs
+// them into the device layout: soa = 8 arrays of n doubles
+// [x|y|z|vx|vy|vz|m|r2], m32 = the masses rounded to FP32.  After a failure
+// soa / m32 hold partial contents.
+PackStatus pack_jset(int n, int j0, int j1, const double *m, const double *x, const double *v,
+                     const double *r2, double *soa, float *m32);
 
 }  // namespace abyss

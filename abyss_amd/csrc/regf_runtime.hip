@@ -660,10 +660,13 @@ int abyss_ctx_upload(abyss_ctx_t c, int n, const double *m, const double *x, con
     // Validation and AoS -> SoA pack in one pass over the caller's arrays,
     // on a few host threads for large N, before anything is copied
     // (src/cuda/cuda_my_acceleration.cu:399-405 packs the same arrays with
-    // stride N).
+    // stride N).  Packing and copying in chunks, so that the copy of one
+    // chunk overlaps the pack of the next, was slower: the strided chunk
+    // copies run as blit kernels (192 vs 168 us for a 65536-particle
+    // upload + query).
     double *h = reinterpret_cast<double *>(c->h_j);
     const abyss::PackStatus st =
-        abyss::pack_jset(n, m, x, v, r2, h, reinterpret_cast<float *>(h + 8 * (size_t)n));
+        abyss::pack_jset(n, 0, n, m, x, v, r2, h, reinterpret_cast<float *>(h + 8 * (size_t)n));
     if (st.index >= 0) {
         const int j = st.index;
         if (st.kind == abyss::kPackBadState)

@@ -9,6 +9,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <random>
 #include <thread>
 #include <vector>
@@ -65,8 +66,8 @@ static void check_pack(int n, unsigned seed)
     JSet s = make(n, seed);
     std::vector<double> soa(8 * (size_t)n, -1.0);
     std::vector<float> m32(n, -1.0f);
-    abyss::PackStatus st =
-        abyss::pack_jset(n, s.m.data(), s.x.data(), s.v.data(), s.r2.data(), soa.data(), m32.data());
+    abyss::PackStatus st = abyss::pack_jset(n, 0, n, s.m.data(), s.x.data(), s.v.data(),
+                                            s.r2.data(), soa.data(), m32.data());
     CHECK(st.index == -1);
     for (int j = 0; j < n; ++j) {
         for (int d = 0; d < 3; ++d) {
@@ -86,8 +87,8 @@ static void check_pack(int n, unsigned seed)
         if (k == 2) s.m[bad[2]] = on ? -2.0 : 1.0;
     };
     auto pack = [&] {
-        return abyss::pack_jset(n, s.m.data(), s.x.data(), s.v.data(), s.r2.data(), soa.data(),
-                                m32.data());
+        return abyss::pack_jset(n, 0, n, s.m.data(), s.x.data(), s.v.data(), s.r2.data(),
+                                soa.data(), m32.data());
     };
     for (int k = 0; k < 3; ++k) {
         spoil(k, true);
@@ -100,6 +101,15 @@ static void check_pack(int n, unsigned seed)
     CHECK(st.index == bad[2] && st.kind == kinds[2]);   // n/3 + 1 is the lowest
     for (int k = 0; k < 3; ++k) spoil(k, false);
     CHECK(pack().index == -1);
+    // in consecutive ranges (as an upload pipelines them): the same contents
+    std::vector<double> soa2(8 * (size_t)n, -1.0);
+    std::vector<float> m322(n, -1.0f);
+    for (int j0 = 0; j0 < n; j0 += 20000) {
+        const int j1 = std::min(n, j0 + 20000);
+        CHECK(abyss::pack_jset(n, j0, j1, s.m.data(), s.x.data(), s.v.data(), s.r2.data(),
+                               soa2.data(), m322.data()).index == -1);
+    }
+    CHECK(soa2 == soa && m322 == m32);
 }
 
 static void check_pack_concurrent()
