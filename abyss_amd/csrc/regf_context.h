@@ -143,4 +143,9 @@ int ctx_launch_force(abyss_ctx_t c);
 int keep_launch_lists(abyss_ctx_t c);
 // Checked precision: the spread / flag buffers for nt targets and S splits.
 int ctx_ensure_check(abyss_ctx_t c, int nt, int S);
+// abyss_ctx_set_targets with (sync) or without the wait for the context's
+// stream; _idle: the caller guarantees that no launch using this context
+// is in flight (an ensemble member whose last batch has finished).
+int ctx_set_targets_sync(abyss_ctx_t c, int nt, const int *targets, bool sync);
+int ctx_set_targets_idle(abyss_ctx_t c, int nt, const int *targets);
 }  // namespace abyss

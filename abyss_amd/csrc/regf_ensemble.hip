@@ -16,9 +16,11 @@
 // least one target block of each running request -- smallest remaining work
 // first -- plus more blocks until a pair budget (~25 ms of work) is used.  So
 // a small request that arrives while a huge one runs waits at most about one
-// slice instead of the whole huge step.  Finished requests go to a completion
-// thread that copies results to the host on its own stream while the next
-// batch runs.  Splitting a request's target blocks over batches does not
+// slice instead of the whole huge step.  The worker enqueues the next batch
+// while the current one runs (two alternating pinned staging sets), so the
+// GPU does not wait for the host between batches.  Finished requests go to
+// a completion thread that copies results to the host on its own stream
+// while later batches run.  Splitting a request's target blocks over batches does not
 // change its bits.  Admission compares a cluster's estimated device
 // footprint with the remaining HBM budget.
 #include "regf_context.h"
@@ -56,9 +58,29 @@ struct abyss_ens_s {
     // its flagged targets are recomputed) once every block is done
     std::vector<int> chk_done;
 
+    // Pinned host staging of a launch's descriptors, work items, header and
+    // prefixes.  Two sets alternate: a launch fills one while the copies of
+    // the launch before it may still read the other, so launches need no
+    // host wait between them.  `done` is recorded after the launch that
+    // used the set (its number `seq`, 0: none).  h_cd ... h_ck_off below
+    // point into the set of the launch being formed.
+    struct Staging {
+        ClusterDesc *h_cd = nullptr;
+        WorkItem *h_items = nullptr;
+        size_t items_cap = 0;
+        BatchHeader *h_hdr = nullptr;
+        int *h_fin_cl = nullptr, *h_fin_off = nullptr, *h_ck_off = nullptr;
+        hipEvent_t done = nullptr;
+        long long seq = 0;
+    };
+    Staging stg[2];
+    int cur = 0;
+    long long seq = 0;               // launches enqueued
+    std::vector<long long> last_seq; // per cluster: the last launch it was in
+
     ClusterDesc *d_cd = nullptr, *h_cd = nullptr;
     WorkItem *d_items = nullptr, *h_items = nullptr;
-    size_t items_cap = 0;
+    size_t items_cap = 0;            // device work items
     BatchHeader *d_hdr = nullptr, *h_hdr = nullptr;
     int *d_counter = nullptr;
     int *d_fin_cl = nullptr, *h_fin_cl = nullptr;     // [max_clusters]
@@ -118,20 +140,69 @@ void drop_graph(abyss_ens_t e)
     e->graph = nullptr;
 }
 
+// Work-item room for n items: the current staging set's pinned buffer (its
+// previous launch has finished, see begin_stage) and the device buffer.
 int ensure_items(abyss_ens_t e, size_t n)
 {
+    abyss_ens_s::Staging &st = e->stg[e->cur];
+    if (n > st.items_cap) {
+        const size_t cap = std::max<size_t>(n + n / 2, 4096);
+        if (st.h_items) (void)hipHostFree(st.h_items);
+        st.h_items = nullptr;
+        st.items_cap = 0;
+        ENSCHK(e, hipHostMalloc(&st.h_items, cap * sizeof(WorkItem), hipHostMallocDefault));
+        st.items_cap = cap;
+        e->h_items = st.h_items;
+    }
     if (n <= e->items_cap) return ABYSS_OK;
     const size_t cap = std::max<size_t>(n + n / 2, 4096);
     ENSCHK(e, hipStreamSynchronize(e->stream));
     if (e->d_items) (void)hipFree(e->d_items);
-    if (e->h_items) (void)hipHostFree(e->h_items);
     e->d_items = nullptr;
-    e->h_items = nullptr;
     e->items_cap = 0;
     drop_graph(e);   // the graph's kernel arguments point at the old buffer
     ENSCHK(e, hipMalloc(&e->d_items, cap * sizeof(WorkItem)));
-    ENSCHK(e, hipHostMalloc(&e->h_items, cap * sizeof(WorkItem), hipHostMallocDefault));
     e->items_cap = cap;
+    return ABYSS_OK;
+}
+
+// Start forming a launch: switch to the other staging set, wait until the
+// launch that last used it has finished (two launches back), and point the
+// h_* buffers into it.
+int begin_stage(abyss_ens_t e)
+{
+    e->cur ^= 1;
+    abyss_ens_s::Staging &st = e->stg[e->cur];
+    if (st.seq > 0) ENSCHK(e, hipEventSynchronize(st.done));
+    e->h_cd = st.h_cd;
+    e->h_items = st.h_items;
+    e->h_hdr = st.h_hdr;
+    e->h_fin_cl = st.h_fin_cl;
+    e->h_fin_off = st.h_fin_off;
+    e->h_ck_off = st.h_ck_off;
+    return ABYSS_OK;
+}
+
+// A launch is enqueued: number it, mark the current set with its completion
+// and remember it as the last launch of each of its clusters.
+int end_stage(abyss_ens_t e, const std::vector<int> &clusters)
+{
+    abyss_ens_s::Staging &st = e->stg[e->cur];
+    e->seq += 1;
+    ENSCHK(e, hipEventRecord(st.done, e->stream));
+    st.seq = e->seq;
+    for (int k : clusters) e->last_seq[k] = e->seq;
+    return ABYSS_OK;
+}
+
+// Waits until every launch that included `cluster` has finished.  Launches
+// up to seq - 2 have (a launch waits for the one two back before it forms).
+int wait_cluster(abyss_ens_t e, int cluster)
+{
+    const long long ls = e->last_seq[cluster];
+    if (ls <= 0 || ls < e->seq - 1) return ABYSS_OK;
+    for (auto &st : e->stg)
+        if (st.seq == ls) ENSCHK(e, hipEventSynchronize(st.done));
     return ABYSS_OK;
 }
 
@@ -188,6 +259,7 @@ int abyss_ens_create(int device, int cap, int max_clusters, abyss_ens_t *out)
     e->blk_first.assign(max_clusters, 0);
     e->blk_count.assign(max_clusters, -1);
     e->chk_done.assign(max_clusters, 0);
+    e->last_seq.assign(max_clusters, 0);
     int rc = ABYSS_OK;
     do {
         if (hipSetDevice(device) != hipSuccess) { rc = ABYSS_ERR_NO_DEVICE; break; }
@@ -201,26 +273,35 @@ int abyss_ens_create(int device, int cap, int max_clusters, abyss_ens_t *out)
         e->grid = std::max(1, e->cu_count) * abyss::batch_blocks_per_cu();
         e->fin_grid = std::max(1, e->cu_count) * 8;
         if (hipMalloc(&e->d_cd, (size_t)max_clusters * sizeof(ClusterDesc)) != hipSuccess ||
-            hipHostMalloc(&e->h_cd, (size_t)max_clusters * sizeof(ClusterDesc),
-                          hipHostMallocDefault) != hipSuccess ||
             hipMalloc(&e->d_hdr, sizeof(BatchHeader)) != hipSuccess ||
-            hipHostMalloc(&e->h_hdr, sizeof(BatchHeader), hipHostMallocDefault) != hipSuccess ||
             hipMalloc(&e->d_counter, 4 * sizeof(int)) != hipSuccess ||
             hipMalloc(&e->d_fin_cl, (size_t)max_clusters * sizeof(int)) != hipSuccess ||
-            hipHostMalloc(&e->h_fin_cl, (size_t)max_clusters * sizeof(int),
-                          hipHostMallocDefault) != hipSuccess ||
             hipMalloc(&e->d_fin_off, (size_t)(max_clusters + 1) * sizeof(int)) != hipSuccess ||
-            hipHostMalloc(&e->h_fin_off, (size_t)(max_clusters + 1) * sizeof(int),
-                          hipHostMallocDefault) != hipSuccess ||
             hipMalloc(&e->d_ckcount, (size_t)max_clusters * sizeof(int)) != hipSuccess ||
             hipHostMalloc(&e->h_ckcount, (size_t)max_clusters * sizeof(int),
                           hipHostMallocDefault) != hipSuccess ||
-            hipMalloc(&e->d_ck_off, (size_t)(max_clusters + 1) * sizeof(int)) != hipSuccess ||
-            hipHostMalloc(&e->h_ck_off, (size_t)(max_clusters + 1) * sizeof(int),
-                          hipHostMallocDefault) != hipSuccess) {
+            hipMalloc(&e->d_ck_off, (size_t)(max_clusters + 1) * sizeof(int)) != hipSuccess) {
             rc = ABYSS_ERR_OUT_OF_MEMORY; break;
         }
-        memset(e->h_hdr, 0, sizeof(BatchHeader));
+        for (auto &st : e->stg) {
+            if (hipHostMalloc(&st.h_cd, (size_t)max_clusters * sizeof(ClusterDesc),
+                              hipHostMallocDefault) != hipSuccess ||
+                hipHostMalloc(&st.h_hdr, sizeof(BatchHeader), hipHostMallocDefault) !=
+                    hipSuccess ||
+                hipHostMalloc(&st.h_fin_cl, (size_t)max_clusters * sizeof(int),
+                              hipHostMallocDefault) != hipSuccess ||
+                hipHostMalloc(&st.h_fin_off, (size_t)(max_clusters + 1) * sizeof(int),
+                              hipHostMallocDefault) != hipSuccess ||
+                hipHostMalloc(&st.h_ck_off, (size_t)(max_clusters + 1) * sizeof(int),
+                              hipHostMallocDefault) != hipSuccess) {
+                rc = ABYSS_ERR_OUT_OF_MEMORY; break;
+            }
+            if (hipEventCreateWithFlags(&st.done, hipEventDisableTiming) != hipSuccess) {
+                rc = ABYSS_ERR_HIP; break;
+            }
+            memset(st.h_hdr, 0, sizeof(BatchHeader));
+        }
+        if (rc != ABYSS_OK) break;
     } while (0);
     if (rc != ABYSS_OK) {
         abyss_ens_destroy(e);
@@ -239,21 +320,24 @@ int abyss_ens_destroy(abyss_ens_t e)
         if (c) abyss_ctx_destroy(c);
     drop_graph(e);
     if (e->d_cd) (void)hipFree(e->d_cd);
-    if (e->h_cd) (void)hipHostFree(e->h_cd);
     if (e->d_items) (void)hipFree(e->d_items);
-    if (e->h_items) (void)hipHostFree(e->h_items);
     if (e->d_hdr) (void)hipFree(e->d_hdr);
-    if (e->h_hdr) (void)hipHostFree(e->h_hdr);
     if (e->d_counter) (void)hipFree(e->d_counter);
     if (e->d_fin_cl) (void)hipFree(e->d_fin_cl);
-    if (e->h_fin_cl) (void)hipHostFree(e->h_fin_cl);
     if (e->d_fin_off) (void)hipFree(e->d_fin_off);
-    if (e->h_fin_off) (void)hipHostFree(e->h_fin_off);
     if (e->d_pool) (void)hipFree(e->d_pool);
     if (e->d_ckcount) (void)hipFree(e->d_ckcount);
     if (e->h_ckcount) (void)hipHostFree(e->h_ckcount);
     if (e->d_ck_off) (void)hipFree(e->d_ck_off);
-    if (e->h_ck_off) (void)hipHostFree(e->h_ck_off);
+    for (auto &st : e->stg) {
+        if (st.h_cd) (void)hipHostFree(st.h_cd);
+        if (st.h_items) (void)hipHostFree(st.h_items);
+        if (st.h_hdr) (void)hipHostFree(st.h_hdr);
+        if (st.h_fin_cl) (void)hipHostFree(st.h_fin_cl);
+        if (st.h_fin_off) (void)hipHostFree(st.h_fin_off);
+        if (st.h_ck_off) (void)hipHostFree(st.h_ck_off);
+        if (st.done) (void)hipEventDestroy(st.done);
+    }
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return ABYSS_OK;
@@ -344,7 +428,11 @@ int abyss_ens_set_targets(abyss_ens_t e, int cluster, int nt, const int *targets
         e->active[cluster] = 0;
         return ABYSS_OK;
     }
-    int rc = abyss_ctx_set_targets(c, nt, targets);
+    // the cluster's target and output buffers are free once its own last
+    // launch has finished; other clusters' launches may still run
+    int rc = wait_cluster(e, cluster);
+    if (rc != ABYSS_OK) return rc;
+    rc = abyss::ctx_set_targets_idle(c, nt, targets);
     e->active[cluster] = rc == ABYSS_OK ? 1 : 0;
     e->chk_done[cluster] = 0;
     e->blk_first[cluster] = 0;
@@ -423,6 +511,18 @@ int abyss_ens_launch(abyss_ens_t e)
     std::sort(act.begin(), act.end(), [](const Entry &a, const Entry &b) {
         return a.item_cost != b.item_cost ? a.item_cost > b.item_cost : a.cluster < b.cluster;
     });
+    // this launch's staging set: the launch two back (which used it) has
+    // finished; whatever is enqueued below is marked with this launch's
+    // completion on every return path
+    int rc = begin_stage(e);
+    if (rc != ABYSS_OK) return rc;
+    std::vector<int> members;
+    for (const Entry &en : act) members.push_back(en.cluster);
+    struct StageEnd {
+        abyss_ens_t e;
+        const std::vector<int> &members;
+        ~StageEnd() { (void)end_stage(e, members); }
+    } stage_end{e, members};
     if (e->checked) {
         for (const Entry &en : act) {
             abyss_ctx_t c = e->cl[en.cluster];
@@ -432,12 +532,10 @@ int abyss_ens_launch(abyss_ens_t e)
             c->n_rechecked = 0;
         }
     }
-    int rc = ensure_items(e, nitems);
+    rc = ensure_items(e, nitems);
     if (rc != ABYSS_OK) return rc;
     rc = ensure_pool(e, pool);
     if (rc != ABYSS_OK) return rc;
-    // the previous launch may still read the pinned descriptor buffers
-    ENSCHK(e, hipStreamSynchronize(e->stream));
     size_t it = 0;
     char *bump = e->d_pool;
     auto take = [&](size_t bytes) {
@@ -709,7 +807,7 @@ struct abyss_srv_s {
     long long batches = 0, completed = 0;
     // GPU time of the batches (events around each on the ensemble stream)
     // and the wall-clock start of the statistics
-    hipEvent_t ev_b0 = nullptr, ev_b1 = nullptr;
+    hipEvent_t ev_b0[2] = {}, ev_b1[2] = {};   // two batches in flight
     double busy_ms = 0.0;
     Clock::time_point t_stats = Clock::now();
 
@@ -725,23 +823,58 @@ namespace {
 
 // Worker: forms batches.  Every running request gets at least one target
 // block per batch; requests with the least remaining work go first, then
-// blocks are handed out until the batch's pair budget is used.  A request
-// whose blocks are all done goes to the completion thread.
+// blocks are handed out until the batch's pair budget is used.  Batches are
+// pipelined: batch k + 1 is formed and enqueued while batch k runs (the
+// ensemble's staging sets alternate, and a cluster's new targets wait only
+// for the cluster's own last batch), and the requests batch k completes go
+// to the completion thread once it has finished.  At most two batches are
+// in flight.
 void serve_loop(abyss_srv_t s)
 {
     (void)hipSetDevice(s->device);
     std::vector<ReqPtr> running;
+    struct Flight {
+        int slot = 0;                // event pair of the batch
+        bool recorded = false;       // its end event is on the stream
+        bool timed = false;          // both events are
+        std::vector<ReqPtr> done;    // requests complete after this batch
+    };
+    std::deque<Flight> flights;
+    int slot = 0;
+    // Waits for the oldest batch in flight, adds its GPU time and hands the
+    // requests it completed to the completion thread.
+    auto retire = [&]() {
+        Flight f = std::move(flights.front());
+        flights.pop_front();
+        bool ok = f.recorded ? hipEventSynchronize(s->ev_b1[f.slot]) == hipSuccess
+                             : hipStreamSynchronize(s->ens->stream) == hipSuccess;
+        float ms = 0.0f;
+        if (!ok || !f.timed ||
+            hipEventElapsedTime(&ms, s->ev_b0[f.slot], s->ev_b1[f.slot]) != hipSuccess)
+            ms = 0.0f;
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->busy_ms += ms;
+        for (auto &r : f.done) {
+            if (!ok && r->status == ABYSS_OK) r->status = ABYSS_ERR_HIP;
+            s->to_fetch.push_back(r);
+        }
+        if (!f.done.empty()) s->cv_fetch.notify_one();
+    };
     for (;;) {
         std::vector<ReqPtr> starting;
+        bool idle = false;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv_work.wait(lk, [&] {
-                if (!running.empty()) return true;
+            auto startable = [&] {
                 for (auto &r : s->pending)
                     if (!s->busy[r->cluster]) return true;
-                return s->stop && s->pending.empty();
-            });
-            if (s->stop && s->pending.empty() && running.empty()) {
+                return false;
+            };
+            if (flights.empty())
+                s->cv_work.wait(lk, [&] {
+                    return !running.empty() || startable() || (s->stop && s->pending.empty());
+                });
+            if (s->stop && s->pending.empty() && running.empty() && flights.empty()) {
                 s->worker_done = true;
                 s->cv_fetch.notify_all();
                 return;
@@ -756,6 +889,11 @@ void serve_loop(abyss_srv_t s)
                     ++it;
                 }
             }
+            idle = running.empty() && starting.empty();
+        }
+        if (idle) {
+            retire();   // nothing to add: wait for the batch in flight
+            continue;
         }
         std::vector<ReqPtr> failed;
         {
@@ -793,34 +931,29 @@ void serve_loop(abyss_srv_t s)
                     take[k] = nb;
                     abyss_ens_set_block_range(s->ens, r->cluster, r->next_block, nb);
                 }
-                const bool timed = s->ev_b0 && s->ev_b1 &&
-                                   hipEventRecord(s->ev_b0, s->ens->stream) == hipSuccess;
-                int rc = abyss_ens_launch(s->ens);
-                const bool timed_end =
-                    timed && hipEventRecord(s->ev_b1, s->ens->stream) == hipSuccess;
-                if (rc == ABYSS_OK) rc = abyss_ens_sync(s->ens);
-                float batch_ms = 0.0f;
-                if (!timed_end || hipEventElapsedTime(&batch_ms, s->ev_b0, s->ev_b1) != hipSuccess)
-                    batch_ms = 0.0f;
-                double inter = 0.0;
+                Flight f;
+                f.slot = slot;
+                slot ^= 1;
+                const bool t0 = hipEventRecord(s->ev_b0[f.slot], s->ens->stream) == hipSuccess;
+                const int rc = abyss_ens_launch(s->ens);
+                f.recorded = hipEventRecord(s->ev_b1[f.slot], s->ens->stream) == hipSuccess;
+                f.timed = t0 && f.recorded;
                 std::vector<ReqPtr> still;
                 for (size_t k = 0; k < running.size(); ++k) {
                     const ReqPtr &r = running[k];
                     if (rc != ABYSS_OK) r->status = rc;
                     r->next_block += take[k];
-                    inter += take[k] * r->block_cost;
                     if (r->status != ABYSS_OK || r->next_block >= r->nblocks) {
                         abyss_ens_set_targets(s->ens, r->cluster, 0, nullptr);
-                        failed.push_back(r);   // to the completion thread
+                        f.done.push_back(r);   // to the completion thread after the batch
                     } else {
                         still.push_back(r);
                     }
                 }
                 running.swap(still);
+                flights.push_back(std::move(f));
                 std::lock_guard<std::mutex> lk(s->mu);
                 s->batches += 1;
-                s->busy_ms += batch_ms;
-                (void)inter;
             }
         }
         if (!failed.empty()) {
@@ -828,6 +961,7 @@ void serve_loop(abyss_srv_t s)
             for (auto &r : failed) s->to_fetch.push_back(r);
             s->cv_fetch.notify_one();
         }
+        while (flights.size() > 1) retire();   // one batch queued behind the running one
     }
 }
 
@@ -912,9 +1046,15 @@ int abyss_srv_create(int device, int cap, int max_clusters, unsigned long long h
     s->footprint.assign(max_clusters, 0);
     s->nparticles.assign(max_clusters, 0);
     s->busy.assign(max_clusters, 0);
-    if (hipSetDevice(device) != hipSuccess || hipEventCreate(&s->ev_b0) != hipSuccess ||
-        hipEventCreate(&s->ev_b1) != hipSuccess) {
-        if (s->ev_b0) (void)hipEventDestroy(s->ev_b0);
+    bool events_ok = hipSetDevice(device) == hipSuccess;
+    for (int k = 0; k < 2 && events_ok; ++k)
+        events_ok = hipEventCreate(&s->ev_b0[k]) == hipSuccess &&
+                    hipEventCreate(&s->ev_b1[k]) == hipSuccess;
+    if (!events_ok) {
+        for (int k = 0; k < 2; ++k) {
+            if (s->ev_b0[k]) (void)hipEventDestroy(s->ev_b0[k]);
+            if (s->ev_b1[k]) (void)hipEventDestroy(s->ev_b1[k]);
+        }
         abyss_ens_destroy(e);
         delete s;
         return ABYSS_ERR_HIP;
@@ -936,8 +1076,10 @@ int abyss_srv_destroy(abyss_srv_t s)
     if (s->worker.joinable()) s->worker.join();   // drains every submitted request
     if (s->completer.joinable()) s->completer.join();
     abyss_ens_destroy(s->ens);
-    if (s->ev_b0) (void)hipEventDestroy(s->ev_b0);
-    if (s->ev_b1) (void)hipEventDestroy(s->ev_b1);
+    for (int k = 0; k < 2; ++k) {
+        if (s->ev_b0[k]) (void)hipEventDestroy(s->ev_b0[k]);
+        if (s->ev_b1[k]) (void)hipEventDestroy(s->ev_b1[k]);
+    }
     delete s;
     return ABYSS_OK;
 }

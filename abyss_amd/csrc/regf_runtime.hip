@@ -692,6 +692,23 @@ int abyss_ctx_upload(abyss_ctx_t c, int n, const double *m, const double *x, con
 
 int abyss_ctx_set_targets(abyss_ctx_t c, int nt, const int *targets)
 {
+    return abyss::ctx_set_targets_sync(c, nt, targets, true);
+}
+
+}  // extern "C"
+
+namespace abyss {
+
+int ctx_set_targets_idle(abyss_ctx_t c, int nt, const int *targets)
+{
+    return ctx_set_targets_sync(c, nt, targets, false);
+}
+
+// `sync`: wait for the context's stream first (the previous launch may
+// still read the target list and outputs); without it the caller
+// guarantees that no launch using this context is in flight.
+int ctx_set_targets_sync(abyss_ctx_t c, int nt, const int *targets, bool sync)
+{
     if (!c) return ABYSS_ERR_INVALID_ARG;
     if (c->n <= 0) return set_detail(c, ABYSS_ERR_NO_JSET, "no j-set uploaded");
     if (!targets || nt <= 0 || nt > c->n)
@@ -702,13 +719,13 @@ int abyss_ctx_set_targets(abyss_ctx_t c, int nt, const int *targets)
             return set_detail(c, ABYSS_ERR_TARGET_RANGE, "target[%d]=%d outside [0,%d)", k,
                               targets[k], c->n);
     HIPCHK(c, hipSetDevice(c->device));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (sync) HIPCHK(c, hipStreamSynchronize(c->stream));
     int rc = ensure_outputs(c, nt);
     if (rc != ABYSS_OK) return rc;
     memcpy(c->h_tgt, targets, (size_t)nt * sizeof(int));
     if (nt <= kMappedTargets && !c->batch_member) {
-        // small block: kernels read the (coherent) pinned copy directly; the
-        // stream is idle, so no launch still reads the previous list
+        // small block: kernels read the (coherent) pinned copy directly; no
+        // launch still reads the previous list
         c->tgt = c->h_tgt_dev;
     } else {
         HIPCHK(c, hipMemcpyAsync(c->d_tgt, c->h_tgt, (size_t)nt * sizeof(int),
@@ -725,6 +742,10 @@ int abyss_ctx_set_targets(abyss_ctx_t c, int nt, const int *targets)
     c->plan_valid = true;
     return ABYSS_OK;
 }
+
+}  // namespace abyss
+
+extern "C" {
 
 int abyss_ctx_plan(abyss_ctx_t c, int *ipl, int *splits, int *grid_x, int *tile, int *unroll,
                    int *jl)
