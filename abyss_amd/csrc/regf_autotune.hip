@@ -1,16 +1,25 @@
 // Plan autotuner (include/abyss_regf.h, abyss_autotune*).
 //
-// For target counts m = 1, 4, 16, ... up to n on a synthetic n-particle
-// cluster (uniform cube, r_nb for ~48 neighbours), every candidate plan is
-// timed with HIP events (median of 3 launches):
+// For target counts m = 1, 4, 16, ... up to n on an n-particle cluster (a
+// synthetic uniform cube with r_nb for ~48 neighbours, or the caller's own
+// j-set), every candidate plan is timed with HIP events (median of 3
+// launches):
 //   lanes per target  JL in {1, 4, 16, 64} (JL = 1 also with j-unroll 2, 4)
+//   shape (JL > 1)    j per LDS tile and wave64s per workgroup (targets per
+//                     workgroup = waves * 64 / JL): (256, 4) (128, 4)
+//                     (64, 4) (128, 2) (64, 2) (64, 1)
 //   workgroups per CU W in {1, 2, 4, 8, 16, 32, 48, 64, 96}, which sets the j-split
 //                     count S = ceil(W * CUs / target blocks)
-// The fastest (JL, unroll, W) per power-of-two target-count bucket is kept in
-// a per-device table that plan selection consults for every later query on
-// that device (buckets between measured points take the nearest one).  The
-// table can be written to and read from a small text file.  Batched
-// ensembles take the lanes-per-target choice from the same table.
+// These are the compile-time BatchSize (j per shared-memory load),
+// NNB_per_block (targets per block) and GridDimY (j-splits) of the
+// reference, src_mpi/def.h:77-79.  The fastest (JL, unroll, W, tile, waves)
+// per power-of-two target-count bucket is kept in a per-device table that
+// plan selection consults for every later query on that device (buckets
+// between measured points take the nearest one).  The table can be written
+// to and read from a small text file.  Batched ensembles take the lanes per
+// target and the tile of each cluster from the same table (their persistent
+// workgroups are 4 waves).  No choice changes a result bit: the plan's
+// j-split count is the only one that changes the order of the FP64 sums.
 #include "regf_context.h"
 
 #include <math.h>
@@ -87,11 +96,14 @@ int abyss_autotune_load(int device, const char *path)
     int got = 0;
     char line[256];
     while (fgets(line, sizeof(line), f)) {
-        int b, jl, un, w;
+        int b, jl, un, w, tile = abyss::kTile, wpb = 4;
         if (line[0] == '#') continue;
-        if (sscanf(line, "%d %d %d %d", &b, &jl, &un, &w) == 4 && b >= 0 && b < abyss::kBuckets &&
-            (jl == 1 || jl == 4 || jl == 16 || jl == 64) && (un == 2 || un == 4) && w > 0) {
-            t.e[b] = abyss::TuneEntry{jl, un, w};
+        // six columns; a four-column line (an older table) has the default shape
+        const int k = sscanf(line, "%d %d %d %d %d %d", &b, &jl, &un, &w, &tile, &wpb);
+        if ((k == 4 || k == 6) && b >= 0 && b < abyss::kBuckets &&
+            (jl == 1 || jl == 4 || jl == 16 || jl == 64) && (un == 2 || un == 4) && w > 0 &&
+            abyss::shape_supported(tile, wpb)) {
+            t.e[b] = abyss::TuneEntry{jl, un, w, tile, wpb};
             ++got;
         }
     }
@@ -103,22 +115,16 @@ int abyss_autotune_load(int device, const char *path)
     return ABYSS_OK;
 }
 
-int abyss_autotune(int device, int n, const char *path)
+int abyss_autotune_jset(int device, int n, const double *mass, const double *pos,
+                        const double *vel, const double *r2, const char *path)
 {
-    if (device < 0 || device >= abyss::kMaxDevices || n < 256) return ABYSS_ERR_INVALID_ARG;
+    if (device < 0 || device >= abyss::kMaxDevices || n < 256 || !mass || !pos || !vel || !r2)
+        return ABYSS_ERR_INVALID_ARG;
     abyss_ctx_t c = nullptr;
     int rc = abyss_ctx_create(device, 256, &c);
     if (rc != ABYSS_OK) return rc;
-    // synthetic cluster: uniform unit cube, r_nb for ~48 neighbours
-    std::vector<double> m(n, 1.0 / n), x(3 * (size_t)n), v(3 * (size_t)n), r2(n);
     abyss::Rng rng{0x9E3779B97F4A7C15ull};
-    for (size_t k = 0; k < 3 * (size_t)n; ++k) {
-        x[k] = rng.next();
-        v[k] = rng.next() - 0.5;
-    }
-    const double rnb = cbrt(48.0 / (n * 4.18879020478639));
-    std::fill(r2.begin(), r2.end(), rnb * rnb);
-    rc = abyss_ctx_upload(c, n, m.data(), x.data(), v.data(), r2.data());
+    rc = abyss_ctx_upload(c, n, mass, pos, vel, r2);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (rc == ABYSS_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
         rc = ABYSS_ERR_HIP;
@@ -131,7 +137,16 @@ int abyss_autotune(int device, int n, const char *path)
         abyss::TuneEntry best;
     };
     std::vector<Meas> meas;
-    const int cu = c->cu_count > 0 ? c->cu_count : 256;
+    const int cu = c ? (c->cu_count > 0 ? c->cu_count : 256) : 256;
+    // candidates: one target per lane with j-unroll 2 or 4, and every
+    // (tile, waves per workgroup) shape of the lanes-per-target kernels
+    struct Cand {
+        int jl, un, tile, wpb;
+    };
+    std::vector<Cand> cands{{1, 2, abyss::kTile, 4}, {1, 4, abyss::kTile, 4}};
+    constexpr int kShapes[][2] = {{256, 4}, {128, 4}, {64, 4}, {128, 2}, {64, 2}, {64, 1}};
+    for (int jl : {4, 16, 64})
+        for (const auto &sh : kShapes) cands.push_back({jl, 2, sh[0], sh[1]});
     std::vector<int> counts;
     for (int mt = 1; mt < n; mt *= 4) counts.push_back(mt);
     counts.push_back(n);
@@ -141,42 +156,36 @@ int abyss_autotune(int device, int n, const char *path)
         if (rc != ABYSS_OK) break;
         double best_ms = 1e30;
         abyss::TuneEntry best{1, 2, 16};
-        const int jls[] = {1, 4, 16, 64};
-        for (int jl : jls) {
-            for (int un : {2, 4}) {
-                if (jl > 1 && un == 4) continue;
-                int prev_S = -1;
-                for (int w : {1, 2, 4, 8, 16, 32, 48, 64, 96}) {
-                    const int tpb = jl > 1 ? 4 * (64 / jl) : abyss::kBlock;
-                    const int gx = (mt + tpb - 1) / tpb;
-                    const int S = std::max(1, std::min((w * cu + gx - 1) / gx,
-                                                       std::min(n / abyss::kTile,
-                                                                abyss::kMaxSplits)));
-                    if (S == prev_S) continue;   // same plan as the previous w
-                    prev_S = S;
-                    rc = abyss_ctx_set_tuning(c, 1, S, un, jl);
-                    if (rc != ABYSS_OK) break;
-                    rc = abyss_ctx_launch(c);   // warm-up (also sizes the scratch)
-                    if (rc != ABYSS_OK) break;
-                    float t[3];
-                    for (int r = 0; r < 3 && rc == ABYSS_OK; ++r) {
-                        if (hipEventRecord(e0, c->stream) != hipSuccess) rc = ABYSS_ERR_HIP;
-                        if (rc == ABYSS_OK) rc = abyss_ctx_launch(c);
-                        if (rc == ABYSS_OK && hipEventRecord(e1, c->stream) != hipSuccess)
-                            rc = ABYSS_ERR_HIP;
-                        if (rc == ABYSS_OK && hipEventSynchronize(e1) != hipSuccess)
-                            rc = ABYSS_ERR_HIP;
-                        if (rc == ABYSS_OK && hipEventElapsedTime(&t[r], e0, e1) != hipSuccess)
-                            rc = ABYSS_ERR_HIP;
-                    }
-                    if (rc != ABYSS_OK) break;
-                    std::sort(t, t + 3);
-                    if (t[1] < best_ms) {
-                        best_ms = t[1];
-                        best = abyss::TuneEntry{jl, un, w};
-                    }
+        for (const Cand &cd : cands) {
+            int prev_S = -1;
+            for (int w : {1, 2, 4, 8, 16, 32, 48, 64, 96}) {
+                const int tpb = cd.jl > 1 ? cd.wpb * (64 / cd.jl) : abyss::kBlock;
+                const int gx = (mt + tpb - 1) / tpb;
+                const int S = std::max(1, std::min((w * cu + gx - 1) / gx,
+                                                   std::min(n / abyss::kTile, abyss::kMaxSplits)));
+                if (S == prev_S) continue;   // same plan as the previous w
+                prev_S = S;
+                rc = abyss_ctx_set_tuning(c, 1, S, cd.un, cd.jl);
+                if (rc == ABYSS_OK)
+                    rc = abyss_ctx_set_shape(c, cd.jl > 1 ? cd.tile : 0, cd.jl > 1 ? cd.wpb : 0);
+                if (rc == ABYSS_OK) rc = abyss_ctx_launch(c);   // warm-up (sizes the scratch)
+                float t[3];
+                for (int r = 0; r < 3 && rc == ABYSS_OK; ++r) {
+                    if (hipEventRecord(e0, c->stream) != hipSuccess) rc = ABYSS_ERR_HIP;
+                    if (rc == ABYSS_OK) rc = abyss_ctx_launch(c);
+                    if (rc == ABYSS_OK && hipEventRecord(e1, c->stream) != hipSuccess)
+                        rc = ABYSS_ERR_HIP;
+                    if (rc == ABYSS_OK && hipEventSynchronize(e1) != hipSuccess)
+                        rc = ABYSS_ERR_HIP;
+                    if (rc == ABYSS_OK && hipEventElapsedTime(&t[r], e0, e1) != hipSuccess)
+                        rc = ABYSS_ERR_HIP;
                 }
                 if (rc != ABYSS_OK) break;
+                std::sort(t, t + 3);
+                if (t[1] < best_ms) {
+                    best_ms = t[1];
+                    best = abyss::TuneEntry{cd.jl, cd.un, w, cd.tile, cd.wpb};
+                }
             }
             if (rc != ABYSS_OK) break;
         }
@@ -203,12 +212,28 @@ int abyss_autotune(int device, int n, const char *path)
         FILE *f = fopen(path, "w");
         if (!f) return ABYSS_ERR_INVALID_ARG;
         fprintf(f, "# abyss_regf plan table: bucket(floor log2 targets) lanes_per_target "
-                   "j_unroll workgroups_per_cu  (tuned at n=%d)\n", n);
+                   "j_unroll workgroups_per_cu j_tile waves_per_workgroup  (tuned at n=%d)\n", n);
         for (int b = 0; b < abyss::kBuckets; ++b)
-            fprintf(f, "%d %d %d %d\n", b, t.e[b].jl, t.e[b].unroll, t.e[b].wg_per_cu);
+            fprintf(f, "%d %d %d %d %d %d\n", b, t.e[b].jl, t.e[b].unroll, t.e[b].wg_per_cu,
+                    t.e[b].tile, t.e[b].wpb);
         fclose(f);
     }
     return ABYSS_OK;
 }
 
 }  // extern "C"
+
+extern "C" int abyss_autotune(int device, int n, const char *path)
+{
+    if (device < 0 || device >= abyss::kMaxDevices || n < 256) return ABYSS_ERR_INVALID_ARG;
+    // synthetic cluster: uniform unit cube, r_nb for ~48 neighbours
+    std::vector<double> m(n, 1.0 / n), x(3 * (size_t)n), v(3 * (size_t)n), r2(n);
+    abyss::Rng rng{0x9E3779B97F4A7C15ull};
+    for (size_t k = 0; k < 3 * (size_t)n; ++k) {
+        x[k] = rng.next();
+        v[k] = rng.next() - 0.5;
+    }
+    const double rnb = cbrt(48.0 / (n * 4.18879020478639));
+    std::fill(r2.begin(), r2.end(), rnb * rnb);
+    return abyss_autotune_jset(device, n, m.data(), x.data(), v.data(), r2.data(), path);
+}

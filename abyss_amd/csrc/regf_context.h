@@ -13,6 +13,7 @@ struct abyss_ctx_s {
     int cap = ABYSS_NUM_NEIGHBOR_MAX;
     int precision = ABYSS_PREC_CHECKED;
     int force_ipl = 0, force_splits = 0, force_unroll = 0, force_jl = 0;
+    int force_tile = 0, force_wpb = 0;   // lanes-per-target shape (0: automatic)
     bool batch_member = false;   // an ensemble's cluster: lane-group plans only
     unsigned long long scratch_limit = 2ull << 30;   // split-j scratch bound
 

@@ -451,6 +451,17 @@ int abyss_ens_blocks(abyss_ens_t e, int cluster, int *nblocks, int *targets_per_
     return ABYSS_OK;
 }
 
+int abyss_ens_plan(abyss_ens_t e, int cluster, int *lanes_per_target, int *tile)
+{
+    if (!e || cluster < 0 || cluster >= e->max_clusters) return ABYSS_ERR_INVALID_ARG;
+    abyss_ctx_t c = e->cl[cluster];
+    if (!c || !e->active[cluster] || !c->plan_valid)
+        return ens_detail(e, ABYSS_ERR_INVALID_ARG, "cluster %d has no targets", cluster);
+    if (lanes_per_target) *lanes_per_target = c->plan.jl;
+    if (tile) *tile = c->plan.tile;
+    return ABYSS_OK;
+}
+
 int abyss_ens_set_block_range(abyss_ens_t e, int cluster, int first, int count)
 {
     if (!e || cluster < 0 || cluster >= e->max_clusters || first < 0)
@@ -564,6 +575,7 @@ int abyss_ens_launch(abyss_ens_t e)
         d.tgt = c->tgt;
         d.nt = c->nt;
         d.jl = c->plan.jl;
+        d.tile = c->plan.tile;
         d.splits = en.splits;
         d.jspan = en.jspan;
         d.acc = c->d_acc;
@@ -691,6 +703,7 @@ int abyss_ens_launch(abyss_ens_t e)
                 d.tgt = c->chk_fi;
                 d.nt = x.m;
                 d.jl = x.jl;
+                d.tile = abyss::kTile;
                 d.splits = x.splits;
                 d.jspan = x.jspan;
                 d.nt_stride = x.m;

@@ -52,11 +52,12 @@ struct CheckFlag {
 struct Plan {
     int ipl = 1;     // targets per lane
     int splits = 1;  // j-splits (grid.y)
-    int jspan = 0;   // j per split (multiple of the tile)
+    int jspan = 0;   // j per split (a multiple of kTile whatever the tile)
     int grid_x = 0;  // target blocks
     int tile = 256;  // j per LDS tile
     int unroll = 2;  // j per unrolled inner-loop step
     int jl = 1;      // lanes per target (1: one target per lane; 4/16/64)
+    int wpb = 4;     // wave64s per workgroup (lanes-per-target kernels)
 };
 
 // Plan choice measured by the autotuner for one target-count bucket.
@@ -64,6 +65,8 @@ struct TuneEntry {
     int jl;         // lanes per target
     int unroll;     // j-unroll of the one-target-per-lane kernel
     int wg_per_cu;  // workgroups per CU the j-split count aims for
+    int tile = 256; // j per LDS tile          } lanes-per-target kernels,
+    int wpb = 4;    // wave64s per workgroup   } split and checked precision
 };
 // The tuned entry for nt targets on `device`, if a table is installed.
 bool tuned_plan(int device, int nt, TuneEntry *out);
@@ -73,9 +76,17 @@ bool tuned_plan(int device, int nt, TuneEntry *out);
 bool variant_supported(int prec, int ipl, int unroll, int jl);
 // Targets handled by one workgroup under plan p.
 int targets_per_block(const Plan &p);
+// Whether the lanes-per-target kernels of term mode 0 (split and checked
+// precision) exist with j-tile `tile` and `wpb` wave64s per workgroup: tile
+// 64, 128 or 256, wpb 1, 2 or 4, and tile <= 64 wpb (every thread loads at
+// most one j record of a tile).  The other modes run (256, 4) only.
+bool shape_supported(int tile, int wpb);
 
 constexpr int kBlock = 256;   // threads per workgroup (4 wave64)
-constexpr int kTile = 256;    // j per LDS tile
+// j per LDS tile of the default shape, the largest tile, and the unit of
+// every j-split span: with splits a multiple of every tile, a lane's FP32
+// runs fold at the same j whatever the tile, so the shape changes no result.
+constexpr int kTile = 256;
 constexpr int kFold = 64;     // j per FP32 partial before the FP64 fold
 constexpr int kMaxSplits = 2048;  // bound of the finalize kernel's LDS prefix
 
@@ -89,6 +100,7 @@ struct ClusterDesc {
     JState js;
     const int *tgt;
     int nt, jl;
+    int tile;        // j per LDS tile (64, 128 or 256)
     int splits, jspan;
     int q_base, nt_stride;
     QueryOut part;

@@ -360,15 +360,16 @@ __device__ __forceinline__ void jl_near(JLLane &L, bool near, unsigned long long
     L.cnt += __popcll(gb);
 }
 
-// Steps of one tile.  FULL: all kTile records are valid; SELF: the tile may
-// hold a lane's own target.  The common case (FULL, !SELF) lists exactly the
-// near pairs, so its only extra instruction over the force is the compare.
-template <int JL, int TM, bool FULL, bool SELF>
+// Steps of one tile of TILE records.  FULL: all TILE records are valid;
+// SELF: the tile may hold a lane's own target.  The common case (FULL,
+// !SELF) lists exactly the near pairs, so its only extra instruction over
+// the force is the compare.
+template <int JL, int TM, bool FULL, bool SELF, int TILE>
 __device__ __forceinline__ void jl_tile(JLLane &L, const double2 *__restrict__ pos,
                                         const typename TermMass<TM>::type *__restrict__ mass,
                                         int count, int jt)
 {
-    const int steps = FULL ? kTile / JL : (count + JL - 1) / JL;
+    const int steps = FULL ? TILE / JL : (count + JL - 1) / JL;
     constexpr int kUnroll = FULL ? 2 : 1;   // partial tiles: run-time trip count
 #pragma unroll kUnroll
     for (int st = 0; st < steps; ++st) {
@@ -454,7 +455,7 @@ __device__ __forceinline__ void jl_tile(JLLane &L, const double2 *__restrict__ p
     }
 }
 
-// The common tile of term mode 0 (all kTile records valid, no lane's own
+// The common tile of term mode 0 (all TILE records valid, no lane's own
 // target in the tile).  G consecutive steps share one neighbour test: the
 // minimum of their r^2 (v_min3 + v_min) against r_nb^2, one compare and one
 // branch instead of G of each.  r^2 is a sum of squares of finite
@@ -463,11 +464,11 @@ __device__ __forceinline__ void jl_tile(JLLane &L, const double2 *__restrict__ p
 // order, so lists stay ascending.  Every pair is evaluated with the same
 // instructions and summed in the same order as in jl_tile, so the results
 // are bitwise those of jl_tile.
-template <int JL, int G>
+template <int JL, int G, int TILE>
 __device__ __forceinline__ void jl_tile_common(JLLane &L, const double2 *__restrict__ pos,
                                                const float *__restrict__ mass, int jt)
 {
-    constexpr int steps = kTile / JL;
+    constexpr int steps = TILE / JL;
     static_assert(steps % G == 0, "the group divides the tile");
 #pragma unroll 1
     for (int st = 0; st < steps; st += G) {
@@ -537,21 +538,31 @@ __device__ __forceinline__ void jl_tile_common(JLLane &L, const double2 *__restr
 // root of its total (the largest terms dominate both), which the checked
 // precision compares with |a| and |jdot|.  The pair arithmetic and the sums
 // are those of EST = false.
-template <int JL, int TM, bool EST = false>
+// Shape: WPB wave64s per workgroup (WPB * 64 / JL targets) stream the
+// j-range through double-buffered LDS tiles of TILE records; the first TILE
+// threads load one record each.  A full tile is TILE / JL steps, a divisor
+// of kFold, so the FP32 runs fold at the same j for every TILE (spans are
+// multiples of kTile): the shape changes neither a sum nor a list.
+template <int JL, int TM, bool EST = false, int TILE = kTile, int WPB = 4>
 __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__restrict__ tgt, int nt,
                                              int jspan, const QueryOut &out, int cap,
                                              int final_counts, int bx, int s, int q_base,
-                                             int nt_stride, double2 (*s_pos)[kTile * 3],
-                                             typename TermMass<TM>::type (*s_m)[kTile])
+                                             int nt_stride, double2 (*s_pos)[TILE * 3],
+                                             typename TermMass<TM>::type (*s_m)[TILE])
 {
+    static_assert(TILE <= 64 * WPB, "one j record per thread per tile load");
+    static_assert(TILE % JL == 0 && kFold % (TILE / JL) == 0, "runs fold at tile ends");
+    static_assert(kTile % TILE == 0, "split spans are whole tiles");
     using MT = typename TermMass<TM>::type;
     constexpr int TW = 64 / JL;            // targets per wave
+    constexpr int G = TILE / JL < 4 ? TILE / JL : 4;   // steps per neighbour test
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int g = lane / JL;
     const int j0 = s * jspan;
     const int j1 = min(js.n, j0 + jspan);
-    const int q = bx * (4 * TW) + (tid >> 6) * TW + g;
+    const int q = bx * (WPB * TW) + (tid >> 6) * TW + g;
+    const bool loader = TILE == 64 * WPB || tid < TILE;
 
     JLLane L;
     L.sub = lane % JL;
@@ -588,33 +599,33 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
         }
     };
 
-    const int ntile = (j1 - j0 + kTile - 1) / kTile;
+    const int ntile = (j1 - j0 + TILE - 1) / TILE;
     // a wave without targets loads its share of every tile and meets every
     // barrier, but evaluates nothing
     const bool wave_busy = __any(q < nt);
     TileLoader<MT> ld;
-    if (ntile > 0) {
+    if (ntile > 0 && loader) {
         ld.load(js, j0 + tid, j1);
         ld.store(s_pos[0], s_m[0], tid);
     }
     __syncthreads();
     for (int t = 0; t < ntile; ++t) {
         const int b = t & 1;
-        if (t + 1 < ntile) ld.load(js, j0 + (t + 1) * kTile + tid, j1);
-        const int jt = j0 + t * kTile;
-        const int count = min(kTile, j1 - jt);
+        if (t + 1 < ntile && loader) ld.load(js, j0 + (t + 1) * TILE + tid, j1);
+        const int jt = j0 + t * TILE;
+        const int count = min(TILE, j1 - jt);
         if (!wave_busy) {
             // no target in this wave (the tail of a small block)
         } else if (__any((unsigned)(L.idx - jt) < (unsigned)count))
-            jl_tile<JL, TM, false, true>(L, s_pos[b], s_m[b], count, jt);
-        else if (count == kTile) {
+            jl_tile<JL, TM, false, true, TILE>(L, s_pos[b], s_m[b], count, jt);
+        else if (count == TILE) {
             if constexpr (TM == 0)
-                jl_tile_common<JL, 4>(L, s_pos[b], s_m[b], jt);
+                jl_tile_common<JL, G, TILE>(L, s_pos[b], s_m[b], jt);
             else
-                jl_tile<JL, TM, true, false>(L, s_pos[b], s_m[b], count, jt);
+                jl_tile<JL, TM, true, false, TILE>(L, s_pos[b], s_m[b], count, jt);
         }
         else
-            jl_tile<JL, TM, false, false>(L, s_pos[b], s_m[b], count, jt);
+            jl_tile<JL, TM, false, false, TILE>(L, s_pos[b], s_m[b], count, jt);
         if constexpr (TM == 0) {
             // fold the FP32 partials once a lane has summed >= kFold pairs
             // (every tile for JL = 4, every 4 / 16 tiles for JL = 16 / 64)
@@ -629,7 +640,7 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
                 since_fold = 0;
             }
         }
-        if (t + 1 < ntile) ld.store(s_pos[b ^ 1], s_m[b ^ 1], tid);
+        if (t + 1 < ntile && loader) ld.store(s_pos[b ^ 1], s_m[b ^ 1], tid);
         __syncthreads();
     }
     double acc[6];
@@ -669,16 +680,16 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
 // 98 registers and drop to 4.
 // reset (checked precision, may be null): the flagged-target counter, zeroed
 // here so that no separate launch is needed before the flagging after it.
-template <int JL, int TM, bool EST>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+template <int JL, int TM, bool EST, int TILE = kTile, int WPB = 4>
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(5, 8)))
 void regf_kernel_jl(JState js, const int *__restrict__ tgt, int nt, int jspan, QueryOut out,
                     int cap, int final_counts, int *__restrict__ reset)
 {
     if (reset && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *reset = 0;
-    __shared__ double2 s_pos[2][kTile * 3];
-    __shared__ typename TermMass<TM>::type s_m[2][kTile];
-    regf_body_jl<JL, TM, EST>(js, tgt, nt, jspan, out, cap, final_counts, blockIdx.x,
-                              blockIdx.y, 0, nt, s_pos, s_m);
+    __shared__ double2 s_pos[2][TILE * 3];
+    __shared__ typename TermMass<TM>::type s_m[2][TILE];
+    regf_body_jl<JL, TM, EST, TILE, WPB>(js, tgt, nt, jspan, out, cap, final_counts, blockIdx.x,
+                                         blockIdx.y, 0, nt, s_pos, s_m);
 }
 
 // One wave per target.  Partial sums: lane l adds splits l, l+64, ... in
@@ -914,6 +925,19 @@ __global__ void batch_reset_kernel(int *counters)
     if (threadIdx.x < 2) counters[threadIdx.x] = 0;
 }
 
+// One work item of a batch with the cluster's lanes per target and j-tile
+// in 4-wave workgroups: smaller tiles use the start of the LDS buffers.
+template <int JL, bool EST, int TILE>
+__device__ __forceinline__ void batch_body(const ClusterDesc &d, const WorkItem &w,
+                                           const QueryOut &out, int cap, int fc,
+                                           double2 (*s_pos)[kTile * 3], float (*s_m)[kTile])
+{
+    regf_body_jl<JL, 0, EST, TILE, 4>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock,
+                                      w.split, d.q_base, d.nt_stride,
+                                      reinterpret_cast<double2 (*)[TILE * 3]>(s_pos),
+                                      reinterpret_cast<float (*)[TILE]>(s_m));
+}
+
 // Persistent: each workgroup takes work items from a device queue until it
 // is empty, so small and large clusters share one launch and the chip stays
 // busy.  A cluster's items use the same plan and body as a single-cluster
@@ -942,19 +966,16 @@ void batch_kernel(const ClusterDesc *__restrict__ cd, const WorkItem *__restrict
         const int fc = d.splits == 1 ? 1 : 0;
         const QueryOut out =
             fc ? QueryOut{d.acc, d.adot, d.nnb, d.list, d.spread} : d.part;
-        switch (d.jl) {
-            case 4:
-                regf_body_jl<4, 0, EST>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock,
-                                        w.split, d.q_base, d.nt_stride, s_pos, s_m);
-                break;
-            case 16:
-                regf_body_jl<16, 0, EST>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock,
-                                         w.split, d.q_base, d.nt_stride, s_pos, s_m);
-                break;
-            default:
-                regf_body_jl<64, 0, EST>(d.js, d.tgt, d.nt, d.jspan, out, cap, fc, w.tblock,
-                                         w.split, d.q_base, d.nt_stride, s_pos, s_m);
-                break;
+        switch (d.jl * 1024 + d.tile) {
+            case 4 * 1024 + 64: batch_body<4, EST, 64>(d, w, out, cap, fc, s_pos, s_m); break;
+            case 4 * 1024 + 128: batch_body<4, EST, 128>(d, w, out, cap, fc, s_pos, s_m); break;
+            case 4 * 1024 + 256: batch_body<4, EST, 256>(d, w, out, cap, fc, s_pos, s_m); break;
+            case 16 * 1024 + 64: batch_body<16, EST, 64>(d, w, out, cap, fc, s_pos, s_m); break;
+            case 16 * 1024 + 128: batch_body<16, EST, 128>(d, w, out, cap, fc, s_pos, s_m); break;
+            case 16 * 1024 + 256: batch_body<16, EST, 256>(d, w, out, cap, fc, s_pos, s_m); break;
+            case 64 * 1024 + 64: batch_body<64, EST, 64>(d, w, out, cap, fc, s_pos, s_m); break;
+            case 64 * 1024 + 128: batch_body<64, EST, 128>(d, w, out, cap, fc, s_pos, s_m); break;
+            default: batch_body<64, EST, 256>(d, w, out, cap, fc, s_pos, s_m); break;
         }
     }
 }
@@ -1104,6 +1125,31 @@ hipError_t launch_t(const JState &js, const int *tgt, int nt, const Plan &p, con
     return hipGetLastError();
 }
 
+// Term mode 0 with the plan's shape (j-tile, wave64s per workgroup).
+template <int JL, bool EST, int TILE, int WPB>
+void launch_shape(const JState &js, const int *tgt, int nt, const Plan &p, const QueryOut &out,
+                  int cap, int fc, int *reset, hipStream_t st)
+{
+    hipLaunchKernelGGL((regf_kernel_jl<JL, 0, EST, TILE, WPB>), dim3(p.grid_x, p.splits),
+                       dim3(64 * WPB), 0, st, js, tgt, nt, p.jspan, out, cap, fc, reset);
+}
+
+template <int JL, bool EST>
+hipError_t launch_mode0(const JState &js, const int *tgt, int nt, const Plan &p,
+                        const QueryOut &out, int cap, int fc, int *reset, hipStream_t st)
+{
+    switch (p.tile * 8 + p.wpb) {
+        case 256 * 8 + 4: launch_shape<JL, EST, 256, 4>(js, tgt, nt, p, out, cap, fc, reset, st); break;
+        case 128 * 8 + 4: launch_shape<JL, EST, 128, 4>(js, tgt, nt, p, out, cap, fc, reset, st); break;
+        case 64 * 8 + 4: launch_shape<JL, EST, 64, 4>(js, tgt, nt, p, out, cap, fc, reset, st); break;
+        case 128 * 8 + 2: launch_shape<JL, EST, 128, 2>(js, tgt, nt, p, out, cap, fc, reset, st); break;
+        case 64 * 8 + 2: launch_shape<JL, EST, 64, 2>(js, tgt, nt, p, out, cap, fc, reset, st); break;
+        case 64 * 8 + 1: launch_shape<JL, EST, 64, 1>(js, tgt, nt, p, out, cap, fc, reset, st); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 // tm: term mode; est: the spread estimate of checked precision (mode 0)
 template <int JL>
 hipError_t launch_jl(const JState &js, const int *tgt, int nt, const Plan &p, const QueryOut &out,
@@ -1111,12 +1157,11 @@ hipError_t launch_jl(const JState &js, const int *tgt, int nt, const Plan &p, co
 {
     dim3 grid(p.grid_x, p.splits);
     const int fc = p.splits == 1 ? 1 : 0;
+    if (tm != 0 && (p.tile != kTile || p.wpb != 4)) return hipErrorInvalidValue;
     if (tm == 0 && est)
-        hipLaunchKernelGGL((regf_kernel_jl<JL, 0, true>), grid, dim3(kBlock), 0, st, js, tgt, nt,
-                           p.jspan, out, cap, fc, reset);
+        return launch_mode0<JL, true>(js, tgt, nt, p, out, cap, fc, reset, st);
     else if (tm == 0)
-        hipLaunchKernelGGL((regf_kernel_jl<JL, 0, false>), grid, dim3(kBlock), 0, st, js, tgt,
-                           nt, p.jspan, out, cap, fc, reset);
+        return launch_mode0<JL, false>(js, tgt, nt, p, out, cap, fc, reset, st);
     else if (tm == 1)
         hipLaunchKernelGGL((regf_kernel_jl<JL, 1, false>), grid, dim3(kBlock), 0, st, js, tgt,
                            nt, p.jspan, out, cap, fc, reset);
@@ -1154,7 +1199,13 @@ bool variant_supported(int prec, int ipl, int unroll, int jl)
 
 int targets_per_block(const Plan &p)
 {
-    return p.jl > 1 ? 4 * (64 / p.jl) : kBlock * p.ipl;
+    return p.jl > 1 ? p.wpb * (64 / p.jl) : kBlock * p.ipl;
+}
+
+bool shape_supported(int tile, int wpb)
+{
+    return (tile == 64 || tile == 128 || tile == 256) && (wpb == 1 || wpb == 2 || wpb == 4) &&
+           tile <= 64 * wpb;
 }
 
 hipError_t launch_regf(const JState &js, const int *tgt, int nt, const Plan &p,
@@ -1164,10 +1215,12 @@ hipError_t launch_regf(const JState &js, const int *tgt, int nt, const Plan &p,
     const int tm = term_mode(prec);
     const bool est = prec == ABYSS_PREC_CHECKED;
     if (est && !out.spread) return hipErrorInvalidValue;
+    if (p.jl > 1 && !shape_supported(p.tile, p.wpb)) return hipErrorInvalidValue;
     if (p.jl == 4) return launch_jl<4>(js, tgt, nt, p, out, cap, tm, est, reset, st);
     if (p.jl == 16) return launch_jl<16>(js, tgt, nt, p, out, cap, tm, est, reset, st);
     if (p.jl == 64) return launch_jl<64>(js, tgt, nt, p, out, cap, tm, est, reset, st);
-    if (reset) return hipErrorInvalidValue;   // the lanes-per-target kernels only
+    if (reset || p.tile != kTile || p.wpb != 4)
+        return hipErrorInvalidValue;   // the lanes-per-target kernels only
     if (prec == ABYSS_PREC_FP64) {
         if (p.ipl == 1) return launch_t<1, 2, true>(js, tgt, nt, p, out, cap, st);
         return launch_t<2, 2, true>(js, tgt, nt, p, out, cap, st);
@@ -1218,7 +1271,8 @@ hipError_t launch_recheck_dev(const JState &js, const int *fi, const int *count,
                               double *adot, const FinalMirror &mirror, int *m_report,
                               hipStream_t st)
 {
-    if (p.jl != 4 && p.jl != 16 && p.jl != 64) return hipErrorInvalidValue;
+    if ((p.jl != 4 && p.jl != 16 && p.jl != 64) || p.tile != kTile || p.wpb != 4)
+        return hipErrorInvalidValue;
     dim3 grid(p.grid_x, p.splits);
     if (p.jl == 4)
         hipLaunchKernelGGL(recheck_kernel_dev<4>, grid, dim3(kBlock), 0, st, js, fi, count, nt,

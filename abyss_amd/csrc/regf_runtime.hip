@@ -169,11 +169,26 @@ Plan plan_for(abyss_ctx_t c, int nt, int n, int prec, int cap, bool forced)
     // peak at N = m = 262144); smaller blocks spread a target over 16 lanes
     // (16..255 targets) or 64 (below 16), measured best at N = 262144.
     abyss::TuneEntry te{0, 0, 16};
-    const bool tuned = forced &&
-                       (prec == ABYSS_PREC_SPLIT || prec == ABYSS_PREC_CHECKED) &&
-                       abyss::tuned_plan(c->device, nt, &te);
+    const bool mode0 = prec == ABYSS_PREC_SPLIT || prec == ABYSS_PREC_CHECKED;
+    const bool tuned = forced && mode0 && abyss::tuned_plan(c->device, nt, &te);
     const int force_jl = forced ? c->force_jl : 0;
     const int force_splits = forced ? c->force_splits : 0;
+    // Workgroup shape of the lanes-per-target kernels (term mode 0 only; a
+    // batch member runs in the 4-wave workgroups of the persistent batch
+    // kernel): forced, else tuned, else 256 j per tile and 4 waves.
+    int tile = abyss::kTile, wpb = 4;
+    if (mode0) {
+        if (forced && c->force_tile) tile = c->force_tile;
+        else if (tuned) tile = te.tile;
+        if (forced && c->force_wpb) wpb = c->force_wpb;
+        else if (tuned) wpb = te.wpb;
+        if (c->batch_member) wpb = 4;
+        if (tile > 64 * wpb) {
+            // a forced tile wins over a tuned or default block, and v.v.
+            if (forced && c->force_wpb && !c->force_tile) tile = 64 * wpb;
+            else wpb = tile / 64;
+        }
+    }
     int jl = force_jl;
     if (jl == 0) {
         if (tuned) jl = te.jl;
@@ -194,7 +209,7 @@ Plan plan_for(abyss_ctx_t c, int nt, int n, int prec, int cap, bool forced)
         const double per = (double)nt * ((double)cap * 4.0 + 6 * 8.0 + 4.0);
         s_max = std::min(s_max, (int)std::max(1.0, std::floor((double)c->scratch_limit / per)));
         auto groups = [&](int l) {
-            const int tpb = abyss::kBlock / l;
+            const int tpb = wpb * 64 / l;
             return (double)((nt + tpb - 1) / tpb) * s_max;
         };
         if (jl == 4 && groups(4) < 4.0 * cu) jl = 16;
@@ -215,6 +230,12 @@ Plan plan_for(abyss_ctx_t c, int nt, int n, int prec, int cap, bool forced)
         unroll = 2;
     }
     if (jl > 1) ipl = 1;
+    if (jl == 1 || !mode0) {
+        tile = abyss::kTile;
+        wpb = 4;
+    }
+    p.tile = tile;
+    p.wpb = wpb;
     p.jl = jl;
     p.ipl = ipl;
     p.unroll = unroll;
@@ -242,8 +263,10 @@ Plan plan_for(abyss_ctx_t c, int nt, int n, int prec, int cap, bool forced)
         S = std::min(S, max_by_mem);
     }
     S = std::max(1, std::min(S, n));
+    // spans are whole kTile blocks whatever the tile (every tile divides
+    // kTile), so the tile changes no fold point of a lane's FP32 runs
     int jspan = (n + S - 1) / S;
-    jspan = ((jspan + p.tile - 1) / p.tile) * p.tile;
+    jspan = ((jspan + abyss::kTile - 1) / abyss::kTile) * abyss::kTile;
     p.jspan = jspan;
     p.splits = (n + jspan - 1) / jspan;
     return p;
@@ -581,6 +604,23 @@ int abyss_ctx_set_tuning(abyss_ctx_t c, int ipl, int splits, int unroll, int jl)
     return ABYSS_OK;
 }
 
+int abyss_ctx_set_shape(abyss_ctx_t c, int tile, int waves_per_block)
+{
+    if (!c) return ABYSS_ERR_INVALID_ARG;
+    if (tile != 0 && tile != 64 && tile != 128 && tile != 256)
+        return set_detail(c, ABYSS_ERR_INVALID_ARG, "tile must be 0, 64, 128 or 256");
+    if (waves_per_block != 0 && waves_per_block != 1 && waves_per_block != 2 &&
+        waves_per_block != 4)
+        return set_detail(c, ABYSS_ERR_INVALID_ARG, "waves_per_block must be 0, 1, 2 or 4");
+    if (tile != 0 && waves_per_block != 0 && !abyss::shape_supported(tile, waves_per_block))
+        return set_detail(c, ABYSS_ERR_INVALID_ARG,
+                          "a tile of %d j needs at least %d waves per block", tile, tile / 64);
+    c->force_tile = tile;
+    c->force_wpb = waves_per_block;
+    c->plan_valid = false;
+    return ABYSS_OK;
+}
+
 int abyss_ctx_set_scratch_limit(abyss_ctx_t c, unsigned long long bytes)
 {
     if (!c || bytes == 0) return ABYSS_ERR_INVALID_ARG;
@@ -758,6 +798,15 @@ int abyss_ctx_plan(abyss_ctx_t c, int *ipl, int *splits, int *grid_x, int *tile,
     if (tile) *tile = c->plan.tile;
     if (unroll) *unroll = c->plan.unroll;
     if (jl) *jl = c->plan.jl;
+    return ABYSS_OK;
+}
+
+int abyss_ctx_plan_block(abyss_ctx_t c, int *waves_per_block, int *targets_per_block)
+{
+    if (!c) return ABYSS_ERR_INVALID_ARG;
+    if (!c->plan_valid) return set_detail(c, ABYSS_ERR_INVALID_ARG, "no targets set");
+    if (waves_per_block) *waves_per_block = c->plan.jl > 1 ? c->plan.wpb : abyss::kBlock / 64;
+    if (targets_per_block) *targets_per_block = abyss::targets_per_block(c->plan);
     return ABYSS_OK;
 }
 

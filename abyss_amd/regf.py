@@ -68,6 +68,7 @@ EXPORTS = [
     "abyss_srv_submit", "abyss_srv_wait", "abyss_srv_stats", "abyss_srv_reset_stats",
     "abyss_srv_cluster_footprint", "abyss_srv_set_batch_budget", "abyss_ens_blocks",
     "abyss_ens_set_block_range", "abyss_autotune", "abyss_autotune_load", "abyss_autotune_clear",
+    "abyss_autotune_jset", "abyss_ctx_set_shape", "abyss_ctx_plan_block", "abyss_ens_plan",
     "abyss_ctx_upload_state", "abyss_ctx_update_state", "abyss_ctx_predict",
     "abyss_ctx_keep_lists", "abyss_ctx_irregular", "abyss_ctx_regular_step",
     "abyss_ctx_derivs23", "abyss_ctx_update_radius", "abyss_srv_set_admission",
@@ -128,6 +129,11 @@ def load():
         "abyss_ctx_fetch": (ctypes.c_int, [_vp, _dp, _dp, _ip, _ip]),
         "abyss_ctx_stream": (_vp, [_vp]),
         "abyss_ctx_plan": (ctypes.c_int, [_vp, _ip, _ip, _ip, _ip, _ip, _ip]),
+        "abyss_ctx_plan_block": (ctypes.c_int, [_vp, _ip, _ip]),
+        "abyss_ens_plan": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip]),
+        "abyss_ctx_set_shape": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+        "abyss_autotune_jset": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                               ctypes.c_char_p]),
         "abyss_ctx_error_detail": (ctypes.c_char_p, [_vp]),
         "abyss_ctx_set_kernel_timing": (ctypes.c_int, [_vp, ctypes.c_int]),
         "abyss_srv_gpu_busy": (ctypes.c_int, [_vp, _dp, _dp]),
@@ -330,6 +336,22 @@ def autotune(device=0, n=65536, cache_path=None):
         raise AbyssError(rc)
 
 
+def autotune_jset(device, m, x, v, r2, cache_path=None):
+    """Measure and install the plan table on the caller's own j-set (see
+    abyss_autotune_jset): m, r2 as [n], x, v as [n][3]."""
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+    r2 = np.ascontiguousarray(r2, dtype=np.float64)
+    n = len(m)
+    if x.shape[0] != n or v.shape[0] != n or len(r2) != n:
+        raise ValueError("m, x, v and r2 must describe the same particles")
+    rc = load().abyss_autotune_jset(device, n, _d(m), _d(x), _d(v), _d(r2),
+                                    cache_path.encode() if cache_path else None)
+    if rc != OK:
+        raise AbyssError(rc)
+
+
 def autotune_load(device, cache_path):
     rc = load().abyss_autotune_load(device, cache_path.encode())
     if rc != OK:
@@ -409,6 +431,11 @@ class Context:
     def set_tuning(self, i_per_lane=0, j_splits=0, j_unroll=0, lanes_per_target=0):
         self._chk(self._lib.abyss_ctx_set_tuning(self.h, int(i_per_lane), int(j_splits),
                                                  int(j_unroll), int(lanes_per_target)))
+
+    def set_shape(self, tile=0, waves_per_block=0):
+        """Force the j-tile (64/128/256) and wave64s per workgroup (1/2/4) of
+        the lanes-per-target kernels; 0 = automatic."""
+        self._chk(self._lib.abyss_ctx_set_shape(self.h, int(tile), int(waves_per_block)))
 
     def set_scratch_limit(self, nbytes):
         self._chk(self._lib.abyss_ctx_set_scratch_limit(self.h, int(nbytes)))
@@ -552,8 +579,11 @@ class Context:
         self._chk(self._lib.abyss_ctx_plan(self.h, ctypes.byref(a), ctypes.byref(b),
                                            ctypes.byref(c), ctypes.byref(d), ctypes.byref(e),
                                            ctypes.byref(f)))
+        w, t = ctypes.c_int(), ctypes.c_int()
+        self._chk(self._lib.abyss_ctx_plan_block(self.h, ctypes.byref(w), ctypes.byref(t)))
         return {"i_per_lane": a.value, "j_splits": b.value, "grid_x": c.value, "tile": d.value,
-                "j_unroll": e.value, "lanes_per_target": f.value}
+                "j_unroll": e.value, "lanes_per_target": f.value, "waves_per_block": w.value,
+                "targets_per_block": t.value}
 
     @property
     def stream(self):
@@ -636,6 +666,12 @@ class Ensemble:
         nb, tpb = ctypes.c_int(), ctypes.c_int()
         self._chk(self._lib.abyss_ens_blocks(self.h, cluster, ctypes.byref(nb), ctypes.byref(tpb)))
         return nb.value, tpb.value
+
+    def plan(self, cluster):
+        """(lanes per target, j per LDS tile) of a cluster's work items."""
+        jl, tile = ctypes.c_int(), ctypes.c_int()
+        self._chk(self._lib.abyss_ens_plan(self.h, cluster, ctypes.byref(jl), ctypes.byref(tile)))
+        return jl.value, tile.value
 
     def set_block_range(self, cluster, first, count):
         self._chk(self._lib.abyss_ens_set_block_range(self.h, cluster, first, count))

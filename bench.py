@@ -333,6 +333,12 @@ def bench_ensemble(args):
     for k, (x, v, m, r2, t) in enumerate(cl):
         ens.upload(k, m, x, v, r2)
         ens.set_targets(k, t)
+    cluster_plans = {}
+    for k in range(len(cl)):
+        jl, tile = ens.plan(k)
+        key = f"jl{jl}/tile{tile}"
+        cluster_plans[key] = cluster_plans.get(key, 0) + 1
+    log(f"cluster plans (lanes per target / j-tile): {cluster_plans}")
     ms_batched = {}
     for prec, name in ((regf.PREC_CHECKED, "checked"), (regf.PREC_SPLIT, "split")):
         ens.set_precision(prec)
@@ -435,6 +441,7 @@ def bench_ensemble(args):
     return {
         "workload": "ensemble_256_plummer_n1024-65536_active0.01-1",
         "clusters": len(cl), "interactions_per_step": inter, "cap": cap,
+        "cluster_plans": cluster_plans,
         "batched_ms_per_step": ms_batched["checked"],
         "batched_split_ms_per_step": ms_batched["split"],
         "interactions_per_s": inter / (ms_batched["checked"] * 1e-3),
@@ -532,6 +539,81 @@ def check_served(cl, kept, nsample=4):
     return {"requests_checked": len(kept), "targets_per_request": nsample,
             "a_rel_max": worst_a, "jdot_rel_max": worst_j,
             "rows_differing_from_oracle": list_mismatch}
+
+
+def tuned_table(path):
+    """The measured buckets of a table written by abyss_autotune (the target
+    counts 1, 4, 16, ... fall in even buckets): bucket -> "lanes/tile/waves/
+    workgroups per CU"."""
+    rows = [ln.split() for ln in open(path) if not ln.startswith("#")]
+    return {f"2^{r[0]}": f"jl{r[1]}/tile{r[4]}/w{r[5]}/wg{r[3]}" for r in rows
+            if int(r[0]) % 2 == 0 and int(r[0]) <= 18}
+
+
+def bench_tuning(args):
+    """The plan table tuned on a uniform cube (N = 65536, what the default
+    run installs) against one tuned on each workload's own cluster, at the
+    library's default precision: per case, the device-resident launch time
+    under each table (HIP events, mean over repeated launches).  Cases: the
+    Plummer + Kroupa N = 65536 active blocks and the King W0 = 6 + binaries
+    N = 262144 blocks and full-N step."""
+    import tempfile
+    import numpy as np
+    from abyss_amd import regf, workloads
+    ev = HipEvents()
+    cases = [("plummer_kroupa65536", workloads.plummer_kroupa(65536),
+              (1, 16, 256, 4096, 65536)),
+             ("king_w6_binaries262144", workloads.king_binaries(262144),
+              (1, 256, 16384, 262144))]
+
+    def timed(js, sizes):
+        out = {}
+        ctx = regf.Context(device=0, cap=js.cap)
+        ctx.upload(js.m, js.x, js.v, js.r2)
+        rng = np.random.default_rng(11)
+        for m in sizes:
+            t = np.sort(rng.permutation(js.n)[:m]).astype(np.int32)
+            reps = int(min(400, max(5, 0.3 / max(m * (js.n - 1) / 1.0e13, 1e-5))))
+            ctx.set_targets(t)
+            ctx.launch()
+            ctx.sync()
+            e0, e1 = ev.create(), ev.create()
+            ev.record(e0, ctx.stream)
+            for _ in range(reps):
+                ctx.launch()
+            ev.record(e1, ctx.stream)
+            p = ctx.plan()
+            out[m] = {"us": ev.elapsed_ms(e0, e1) / reps * 1e3,
+                      "plan": f"jl{p['lanes_per_target']}/tile{p['tile']}/"
+                              f"w{p['waves_per_block']}/S{p['j_splits']}"}
+            ctx.sync()
+        ctx.close()
+        return out
+
+    res = {}
+    with tempfile.TemporaryDirectory() as td:
+        cube = os.path.join(td, "cube.txt")
+        t0 = time.perf_counter()
+        regf.autotune(0, 65536, cube)
+        log(f"cube table in {time.perf_counter() - t0:.1f} s: {tuned_table(cube)}")
+        res["cube_table"] = tuned_table(cube)
+        for name, js, sizes in cases:
+            regf.autotune_load(0, cube)
+            a = timed(js, sizes)
+            own = os.path.join(td, name + ".txt")
+            t0 = time.perf_counter()
+            regf.autotune_jset(0, js.m, js.x, js.v, js.r2, own)
+            secs = time.perf_counter() - t0
+            b = timed(js, sizes)
+            rows = [{"m": m, "cube_us": a[m]["us"], "cube_plan": a[m]["plan"],
+                     "own_us": b[m]["us"], "own_plan": b[m]["plan"],
+                     "cube_over_own": a[m]["us"] / b[m]["us"]} for m in sizes]
+            for r in rows:
+                log(f"{name} m={r['m']}: cube {r['cube_us']:.1f} us ({r['cube_plan']}) "
+                    f"own {r['own_us']:.1f} us ({r['own_plan']}) ratio {r['cube_over_own']:.3f}")
+            res[name] = {"own_table": tuned_table(own), "own_tuning_s": secs, "rows": rows}
+    regf.autotune_clear(0)
+    return res
 
 
 def bench_blocks(args, sizes=(1, 4, 16, 64, 256, 1024, 4096, 16384, 65536)):
@@ -754,7 +836,7 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter calibration)")
     ap.add_argument("--workload", choices=["king262144", "ensemble", "serve", "neighbour",
-                                           "blocks"],
+                                           "blocks", "tuning"],
                     default="king262144")
     ap.add_argument("--serve-rate", type=float, default=100.0,
                     help="sustained request rate of the served-ensemble leg (req/s)")
@@ -772,6 +854,10 @@ def main():
     if args.workload == "blocks":
         print(json.dumps({"metric": "regular-force active-block sweep (latency and rates)",
                           "blocks": bench_blocks(args)}), flush=True)
+        return
+    if args.workload == "tuning":
+        print(json.dumps({"metric": "plan tables: cube-tuned vs tuned on the workload",
+                          "tuning": bench_tuning(args)}), flush=True)
         return
     if args.workload == "neighbour":
         print(json.dumps({"metric": "irregular force and fused regular step (wall-clock)",
@@ -872,10 +958,14 @@ def main():
         # table (the full-N step above uses the built-in rules, which were
         # set from sweeps of this workload)
         if not args.no_autotune:
+            import tempfile
             t0 = time.perf_counter()
-            regf.autotune(0, 65536)
-            tuned = True
-            log(f"autotuned plan table in {time.perf_counter() - t0:.1f} s")
+            with tempfile.TemporaryDirectory() as td:
+                path = os.path.join(td, "plans.txt")
+                regf.autotune(0, 65536, path)
+                tuned = {"n": 65536, "seconds": time.perf_counter() - t0,
+                         "measured_buckets": tuned_table(path)}
+            log(f"autotuned plan table {tuned}")
         ens_args = argparse.Namespace(steps=4, warmup=1)
         ens_info = bench_ensemble(ens_args)
         ens_info["autotuned"] = tuned

@@ -183,6 +183,13 @@ int abyss_ctx_set_precision(abyss_ctx_t ctx, int mode);
  * or 64; > 1 implies one target per lane group); 0 = automatic for each. */
 int abyss_ctx_set_tuning(abyss_ctx_t ctx, int i_per_lane, int j_splits, int j_unroll,
                          int lanes_per_target);
+/* Forces the workgroup shape of the lanes-per-target kernels under split and
+ * checked precision: j per LDS tile (64, 128 or 256) and wave64s per
+ * workgroup (1, 2 or 4; a workgroup serves waves * 64 / lanes_per_target
+ * targets), with tile <= 64 * waves; 0 = automatic for each.  The
+ * reference's compile-time BatchSize and NNB_per_block (src_mpi/def.h:77,79).
+ * No shape changes a result bit. */
+int abyss_ctx_set_shape(abyss_ctx_t ctx, int tile, int waves_per_block);
 /* Upper bound on the split-j scratch, S * m * (52 + 4 * cap) bytes for S
  * j-splits of m targets (bytes; default 2 GiB).  A smaller bound caps S. */
 int abyss_ctx_set_scratch_limit(abyss_ctx_t ctx, unsigned long long bytes);
@@ -208,6 +215,8 @@ void *abyss_ctx_stream(abyss_ctx_t ctx);
  * blocks, j per LDS tile, j unroll, lanes per target. */
 int abyss_ctx_plan(abyss_ctx_t ctx, int *i_per_lane, int *j_splits,
                    int *grid_x, int *tile, int *j_unroll, int *lanes_per_target);
+/* Workgroup of the plan: wave64s per workgroup and targets per workgroup. */
+int abyss_ctx_plan_block(abyss_ctx_t ctx, int *waves_per_block, int *targets_per_block);
 /* Detail string of the context's last error. */
 const char *abyss_ctx_error_detail(abyss_ctx_t ctx);
 /* Force-kernel timing: with it on, every launch records a HIP event pair
@@ -333,14 +342,22 @@ int abyss_ctx_derivs23(abyss_ctx_t ctx, int nt, const int *targets, int referenc
  * nmax = NumNeighborMax (1000), src_mpi/def.h:18,20. */
 int abyss_ctx_update_radius(abyss_ctx_t ctx, int fix, int nmax, double *r2_out);
 
-/* Plan autotuner.  Times candidate plans (lanes per target, j-unroll,
- * workgroups per CU -> j-split count) for target counts 1, 4, 16, ... n on
- * a synthetic n-particle cluster (n >= 256) and installs the fastest per
- * power-of-two target-count bucket for every later automatic plan on
- * `device` (split precision; forced tunings still win).  Writes the table
- * to `cache_path` when not NULL.  Takes seconds; results of a query depend
- * on its plan only through the FP64 summation order. */
+/* Plan autotuner.  Times candidate plans (lanes per target, j-unroll, j
+ * per LDS tile, wave64s per workgroup, workgroups per CU -> j-split count)
+ * for target counts 1, 4, 16, ... n on a synthetic n-particle cluster
+ * (uniform cube, n >= 256) and installs the fastest per power-of-two
+ * target-count bucket for every later automatic plan on `device` (split
+ * and checked precision; forced tunings still win).  Writes the table to
+ * `cache_path` when not NULL (one line per bucket: bucket, lanes per
+ * target, j-unroll, workgroups per CU, tile, waves).  Takes seconds to a
+ * minute; results of a query depend on its plan only through the FP64
+ * summation order of its j-splits.  Replaces the reference's compile-time
+ * BatchSize / GridDimY / NNB_per_block (src_mpi/def.h:77-79). */
 int abyss_autotune(int device, int n, const char *cache_path);
+/* The same on the caller's own n-particle j-set (the abyss_ctx_upload
+ * arrays), so the table is measured on the workload it will serve. */
+int abyss_autotune_jset(int device, int n, const double *m, const double *x,
+                        const double *v, const double *r2, const char *cache_path);
 /* Installs a table written by abyss_autotune. */
 int abyss_autotune_load(int device, const char *cache_path);
 /* Removes the table: plans use the built-in rules again. */
@@ -377,6 +394,9 @@ int abyss_ens_sync(abyss_ens_t ens);
  * cluster's blocks over several launches gives the same bits. */
 int abyss_ens_blocks(abyss_ens_t ens, int cluster, int *nblocks, int *targets_per_block);
 int abyss_ens_set_block_range(abyss_ens_t ens, int cluster, int first, int count);
+/* Lanes per target and j per LDS tile of a cluster's work items (from the
+ * tuned table when one is installed; workgroups of the batch are 4 waves). */
+int abyss_ens_plan(abyss_ens_t ens, int cluster, int *lanes_per_target, int *tile);
 /* Results of a cluster from the last launch (list row stride = cap). */
 int abyss_ens_fetch(abyss_ens_t ens, int cluster, double *acc, double *adot, int *nnb,
                     int *list);
