@@ -2,10 +2,11 @@
 //
 // For target counts m = 1, 4, 16, ... up to n on an n-particle cluster (a
 // synthetic uniform cube with r_nb for ~48 neighbours, or the caller's own
-// j-set), every candidate plan is timed with HIP events (median of 3
-// launches):
-//   lanes per target  JL in {1, 4, 16, 64} (JL = 1 also with j-unroll 2, 4)
-//   shape (JL > 1)    j per LDS tile and wave64s per workgroup (targets per
+// j-set), every candidate plan is timed under the library's default
+// precision (checked) with HIP events (median of 5 launches, 3 above 1 ms):
+//   lanes per target  JL in {4, 16, 64} (the lane-group kernels; the
+//                     one-target-per-lane kernel runs only when forced)
+//   shape             j per LDS tile and wave64s per workgroup (targets per
 //                     workgroup = waves * 64 / JL): (256, 4) (128, 4)
 //                     (64, 4) (128, 2) (64, 2) (64, 1)
 //   workgroups per CU W in {1, 2, 4, 8, 16, 32, 48, 64, 96}, which sets the j-split
@@ -124,7 +125,8 @@ int abyss_autotune_jset(int device, int n, const double *mass, const double *pos
     int rc = abyss_ctx_create(device, 256, &c);
     if (rc != ABYSS_OK) return rc;
     abyss::Rng rng{0x9E3779B97F4A7C15ull};
-    rc = abyss_ctx_upload(c, n, mass, pos, vel, r2);
+    rc = abyss_ctx_set_precision(c, ABYSS_PREC_CHECKED);
+    if (rc == ABYSS_OK) rc = abyss_ctx_upload(c, n, mass, pos, vel, r2);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (rc == ABYSS_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
         rc = ABYSS_ERR_HIP;
@@ -138,12 +140,11 @@ int abyss_autotune_jset(int device, int n, const double *mass, const double *pos
     };
     std::vector<Meas> meas;
     const int cu = c ? (c->cu_count > 0 ? c->cu_count : 256) : 256;
-    // candidates: one target per lane with j-unroll 2 or 4, and every
-    // (tile, waves per workgroup) shape of the lanes-per-target kernels
+    // candidates: every (tile, waves per workgroup) shape of each lane group
     struct Cand {
         int jl, un, tile, wpb;
     };
-    std::vector<Cand> cands{{1, 2, abyss::kTile, 4}, {1, 4, abyss::kTile, 4}};
+    std::vector<Cand> cands;
     constexpr int kShapes[][2] = {{256, 4}, {128, 4}, {64, 4}, {128, 2}, {64, 2}, {64, 1}};
     for (int jl : {4, 16, 64})
         for (const auto &sh : kShapes) cands.push_back({jl, 2, sh[0], sh[1]});
@@ -155,7 +156,7 @@ int abyss_autotune_jset(int device, int n, const double *mass, const double *pos
         rc = abyss_ctx_set_targets(c, mt, idx.data());
         if (rc != ABYSS_OK) break;
         double best_ms = 1e30;
-        abyss::TuneEntry best{1, 2, 16};
+        abyss::TuneEntry best{4, 2, 16};
         for (const Cand &cd : cands) {
             int prev_S = -1;
             for (int w : {1, 2, 4, 8, 16, 32, 48, 64, 96}) {
@@ -169,8 +170,9 @@ int abyss_autotune_jset(int device, int n, const double *mass, const double *pos
                 if (rc == ABYSS_OK)
                     rc = abyss_ctx_set_shape(c, cd.jl > 1 ? cd.tile : 0, cd.jl > 1 ? cd.wpb : 0);
                 if (rc == ABYSS_OK) rc = abyss_ctx_launch(c);   // warm-up (sizes the scratch)
-                float t[3];
-                for (int r = 0; r < 3 && rc == ABYSS_OK; ++r) {
+                float t[5];
+                int reps = 5;
+                for (int r = 0; r < reps && rc == ABYSS_OK; ++r) {
                     if (hipEventRecord(e0, c->stream) != hipSuccess) rc = ABYSS_ERR_HIP;
                     if (rc == ABYSS_OK) rc = abyss_ctx_launch(c);
                     if (rc == ABYSS_OK && hipEventRecord(e1, c->stream) != hipSuccess)
@@ -179,11 +181,12 @@ int abyss_autotune_jset(int device, int n, const double *mass, const double *pos
                         rc = ABYSS_ERR_HIP;
                     if (rc == ABYSS_OK && hipEventElapsedTime(&t[r], e0, e1) != hipSuccess)
                         rc = ABYSS_ERR_HIP;
+                    if (r == 0 && t[0] > 1.0f) reps = 3;   // long launches: 3 suffice
                 }
                 if (rc != ABYSS_OK) break;
-                std::sort(t, t + 3);
-                if (t[1] < best_ms) {
-                    best_ms = t[1];
+                std::sort(t, t + reps);
+                if (t[reps / 2] < best_ms) {
+                    best_ms = t[reps / 2];
                     best = abyss::TuneEntry{cd.jl, cd.un, w, cd.tile, cd.wpb};
                 }
             }

@@ -342,12 +342,12 @@ int abyss_ctx_derivs23(abyss_ctx_t ctx, int nt, const int *targets, int referenc
  * nmax = NumNeighborMax (1000), src_mpi/def.h:18,20. */
 int abyss_ctx_update_radius(abyss_ctx_t ctx, int fix, int nmax, double *r2_out);
 
-/* Plan autotuner.  Times candidate plans (lanes per target, j-unroll, j
+/* Plan autotuner.  Times candidate plans (lanes per target 4, 16 or 64, j
  * per LDS tile, wave64s per workgroup, workgroups per CU -> j-split count)
- * for target counts 1, 4, 16, ... n on a synthetic n-particle cluster
- * (uniform cube, n >= 256) and installs the fastest per power-of-two
- * target-count bucket for every later automatic plan on `device` (split
- * and checked precision; forced tunings still win).  Writes the table to
+ * under checked precision for target counts 1, 4, 16, ... n on a synthetic
+ * n-particle cluster (uniform cube, n >= 256) and installs the fastest per
+ * power-of-two target-count bucket for every later automatic plan on
+ * `device` (split and checked precision; forced tunings still win).  Writes the table to
  * `cache_path` when not NULL (one line per bucket: bucket, lanes per
  * target, j-unroll, workgroups per CU, tile, waves).  Takes seconds to a
  * minute; results of a query depend on its plan only through the FP64
