@@ -464,24 +464,70 @@ def bench_ensemble(args):
 ADMIT_PAIRS = 3.0e11
 
 
-def serve_stream(srv, ids, sizes, rate, window_s, seed=9, check_every=0):
-    """Poisson request stream at `rate` req/s for `window_s` seconds; each
-    request is one regular step of a random cluster with an active block
-    log-uniform in [1, N].  With check_every = k, every k-th admitted
-    request's results are copied back and kept.  Returns (latencies_ms,
-    interactions, wall_s, rejected, kept) over the admitted requests, kept =
-    [(cluster index, targets, results)]."""
-    import threading
+def serve_requests(sizes, rate, window_s, seed=9):
+    """Arrival times of a Poisson stream at `rate` req/s over `window_s`
+    seconds, and its requests: (cluster, targets, N), one regular step of a
+    random cluster with an active block log-uniform in [1, N]."""
     import numpy as np
     rng = np.random.default_rng(seed)
     t_arr = np.cumsum(rng.exponential(1.0 / rate, int(rate * window_s * 1.5) + 16))
     t_arr = t_arr[t_arr < window_s]
     reqs = []
     for _ in t_arr:
-        c = int(rng.integers(len(ids)))
+        c = int(rng.integers(len(sizes)))
         n = sizes[c]
         mblk = int(min(n, max(1, np.exp(rng.uniform(0.0, np.log(n + 1))))))
         reqs.append((c, rng.choice(n, mblk, replace=False).astype(np.int32), n))
+    return t_arr, reqs
+
+
+def batched_same_mix(cl, reqs, cap):
+    """The requests of a served stream as batched-ensemble launches, with no
+    server: each launch takes the oldest pending request of every cluster
+    that has one (a cluster holds one target set per launch).  Returns the
+    interactions/s over the wall time of all launches (targets set, launch,
+    wait), checked precision."""
+    from abyss_amd import regf
+    ens = regf.Ensemble(device=0, cap=cap, max_clusters=len(cl))
+    for k, (x, v, m, r2) in enumerate(cl):
+        ens.upload(k, m, x, v, r2)
+    pending = [[] for _ in cl]
+    for (c, t, n) in reqs:
+        pending[c].append(t)
+    inter = 0.0
+    launches = 0
+    # warm-up launch of one request (not timed)
+    first = next(k for k in range(len(cl)) if pending[k])
+    ens.set_targets(first, pending[first][0])
+    ens.launch()
+    ens.sync()
+    t0 = time.perf_counter()
+    while any(pending):
+        for k in range(len(cl)):
+            if pending[k]:
+                t = pending[k].pop(0)
+                ens.set_targets(k, t)
+                inter += float(len(t)) * (len(cl[k][2]) - 1)
+            else:
+                ens.set_targets(k, [])
+        ens.launch()
+        ens.sync()
+        launches += 1
+    wall = time.perf_counter() - t0
+    ens.close()
+    return {"interactions_per_s": inter / wall, "launches": launches, "wall_s": wall,
+            "requests": len(reqs)}
+
+
+def serve_stream(srv, ids, sizes, rate, window_s, seed=9, check_every=0):
+    """Poisson request stream at `rate` req/s for `window_s` seconds (see
+    serve_requests).  With check_every = k, every k-th admitted request's
+    results are copied back and kept.  Returns (latencies_ms, interactions,
+    wall_s, rejected, kept) over the admitted requests, kept = [(cluster
+    index, targets, results)]."""
+    import threading
+    import numpy as np
+    t_arr, reqs = serve_requests(sizes, rate, window_s, seed)
     tickets = [None] * len(reqs)
     lat = [None] * len(reqs)
     kept = []
@@ -805,6 +851,14 @@ def bench_serve(args, rate=2000.0, window_s=1.0):
                      "gpu_busy_frac": st["gpu_busy_frac"]}
         log(f"serve {name}: {out[name]}")
     srv.close()
+    # the rate300 leg's requests as batched launches without the server: the
+    # batched rate of the served mix itself
+    _, reqs300 = serve_requests(sizes, 3 * args.serve_rate, 3.0)
+    out["batched_same_mix"] = batched_same_mix(cl, reqs300, cap)
+    out["rate300_over_batched_same_mix"] = (out["rate300"]["interactions_per_s"]
+                                            / out["batched_same_mix"]["interactions_per_s"])
+    log(f"serve batched same mix: {out['batched_same_mix']} "
+        f"(rate300 / batched {out['rate300_over_batched_same_mix']:.3f})")
     out["oracle_sample"] = check_served(cl, kept_all)
     log(f"serve oracle sample: {out['oracle_sample']}")
     return out
