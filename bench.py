@@ -313,6 +313,19 @@ def bench_ensemble(args):
     cl = workloads.ensemble_clusters()
     log(f"ensemble workload: {len(cl)} clusters, sum N = {sum(len(c[2]) for c in cl)}, "
         f"built in {time.perf_counter() - t0:.1f} s")
+    tuned = False
+    if not getattr(args, "no_autotune", False):
+        # the plan table, tuned on the workload's own largest cluster
+        import tempfile
+        big = max(range(len(cl)), key=lambda k: len(cl[k][2]))
+        x, v, m, r2, _ = cl[big]
+        t0 = time.perf_counter()
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "plans.txt")
+            regf.autotune_jset(0, m, x, v, r2, path)
+            tuned = {"tuned_on": f"ensemble cluster {big} (Plummer, N = {len(m)})",
+                     "seconds": time.perf_counter() - t0, "measured_buckets": tuned_table(path)}
+        log(f"autotuned plan table {tuned}")
     inter = float(sum(len(c[4]) * (len(c[2]) - 1) for c in cl))
     cap = 256
     ev = HipEvents()
@@ -442,6 +455,7 @@ def bench_ensemble(args):
         "workload": "ensemble_256_plummer_n1024-65536_active0.01-1",
         "clusters": len(cl), "interactions_per_step": inter, "cap": cap,
         "cluster_plans": cluster_plans,
+        "autotuned": tuned,
         "batched_ms_per_step": ms_batched["checked"],
         "batched_split_ms_per_step": ms_batched["split"],
         "interactions_per_s": inter / (ms_batched["checked"] * 1e-3),
@@ -1006,23 +1020,13 @@ def main():
     if not args.no_blocks:
         blocks = bench_blocks(args, sizes=(1, 64, 4096, 65536))
     ens_info = serve_info = None
-    tuned = False
     if not args.no_ensemble:
-        # the ensemble legs mix every block size: plan them from a measured
-        # table (the full-N step above uses the built-in rules, which were
-        # set from sweeps of this workload)
-        if not args.no_autotune:
-            import tempfile
-            t0 = time.perf_counter()
-            with tempfile.TemporaryDirectory() as td:
-                path = os.path.join(td, "plans.txt")
-                regf.autotune(0, 65536, path)
-                tuned = {"n": 65536, "seconds": time.perf_counter() - t0,
-                         "measured_buckets": tuned_table(path)}
-            log(f"autotuned plan table {tuned}")
-        ens_args = argparse.Namespace(steps=4, warmup=1)
+        # the ensemble legs mix every block size: they plan from a table
+        # tuned on the ensemble's own largest cluster (the full-N step above
+        # uses the built-in rules, which were set from sweeps of this
+        # workload); the served leg keeps that table
+        ens_args = argparse.Namespace(steps=4, warmup=1, no_autotune=args.no_autotune)
         ens_info = bench_ensemble(ens_args)
-        ens_info["autotuned"] = tuned
         log(f"batched ensemble {ens_info}")
         serve_info = bench_serve(args)
 
