@@ -8,7 +8,8 @@
 //                     one-target-per-lane kernel runs only when forced)
 //   shape             j per LDS tile and wave64s per workgroup (targets per
 //                     workgroup = waves * 64 / JL): (256, 4) (128, 4)
-//                     (64, 4) (128, 2) (64, 2) (64, 1)
+//                     (64, 4) (128, 2) (64, 2) (64, 1), and for JL >= 16
+//                     the one-shot (1024, 4) of splits up to 1024 j
 //   workgroups per CU W in {1, 2, 4, 8, 16, 32, 48, 64, 96}, which sets the j-split
 //                     count S = ceil(W * CUs / target blocks)
 // These are the compile-time BatchSize (j per shared-memory load),
@@ -146,8 +147,11 @@ int abyss_autotune_jset(int device, int n, const double *mass, const double *pos
     };
     std::vector<Cand> cands;
     constexpr int kShapes[][2] = {{256, 4}, {128, 4}, {64, 4}, {128, 2}, {64, 2}, {64, 1}};
-    for (int jl : {4, 16, 64})
+    for (int jl : {4, 16, 64}) {
         for (const auto &sh : kShapes) cands.push_back({jl, 2, sh[0], sh[1]});
+        // the one-shot tile (plans whose splits exceed it fall back to 256)
+        if (jl >= 16) cands.push_back({jl, 2, abyss::kTileOneShot, 4});
+    }
     std::vector<int> counts;
     for (int mt = 1; mt < n; mt *= 4) counts.push_back(mt);
     counts.push_back(n);

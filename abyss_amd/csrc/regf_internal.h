@@ -79,8 +79,11 @@ int targets_per_block(const Plan &p);
 // Whether the lanes-per-target kernels of term mode 0 (split and checked
 // precision) exist with j-tile `tile` and `wpb` wave64s per workgroup: tile
 // 64, 128 or 256, wpb 1, 2 or 4, and tile <= 64 wpb (every thread loads at
-// most one j record of a tile).  The other modes run (256, 4) only.
+// most one j record of a tile); or the one-shot tile (kTileOneShot, 4
+// waves), which holds a whole split of at most that many j and exists for
+// 16 and 64 lanes per target.  The other modes run (256, 4) only.
 bool shape_supported(int tile, int wpb);
+constexpr int kTileOneShot = 1024;
 
 constexpr int kBlock = 256;   // threads per workgroup (4 wave64)
 // j per LDS tile of the default shape, the largest tile, and the unit of

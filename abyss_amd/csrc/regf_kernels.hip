@@ -543,6 +543,10 @@ __device__ __forceinline__ void jl_tile_common(JLLane &L, const double2 *__restr
 // threads load one record each.  A full tile is TILE / JL steps, a divisor
 // of kFold, so the FP32 runs fold at the same j for every TILE (spans are
 // multiples of kTile): the shape changes neither a sum nor a list.
+// One-shot tiles (TILE > 64 * WPB, small blocks): the whole j-range of the
+// split is one tile (the host guarantees jspan <= TILE), loaded at once
+// with TILE / (64 WPB) records per thread into a single LDS buffer, so a
+// split waits for one load instead of one per 256 j.
 template <int JL, int TM, bool EST = false, int TILE = kTile, int WPB = 4>
 __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__restrict__ tgt, int nt,
                                              int jspan, const QueryOut &out, int cap,
@@ -550,9 +554,12 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
                                              int nt_stride, double2 (*s_pos)[TILE * 3],
                                              typename TermMass<TM>::type (*s_m)[TILE])
 {
-    static_assert(TILE <= 64 * WPB, "one j record per thread per tile load");
+    constexpr bool ONE_SHOT = TILE > 64 * WPB;
+    constexpr int R = ONE_SHOT ? TILE / (64 * WPB) : 1;   // records per thread
+    static_assert(!ONE_SHOT || TILE % (64 * WPB) == 0, "whole records per thread");
     static_assert(TILE % JL == 0 && kFold % (TILE / JL) == 0, "runs fold at tile ends");
-    static_assert(kTile % TILE == 0, "split spans are whole tiles");
+    static_assert(ONE_SHOT ? TILE % kTile == 0 : kTile % TILE == 0,
+                  "split spans are whole tiles, or one one-shot tile");
     using MT = typename TermMass<TM>::type;
     constexpr int TW = 64 / JL;            // targets per wave
     constexpr int G = TILE / JL < 4 ? TILE / JL : 4;   // steps per neighbour test
@@ -604,14 +611,21 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
     // barrier, but evaluates nothing
     const bool wave_busy = __any(q < nt);
     TileLoader<MT> ld;
-    if (ntile > 0 && loader) {
+    if constexpr (ONE_SHOT) {
+        // every load in flight before the first store (ntile <= 1)
+        TileLoader<MT> lr[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) lr[r].load(js, j0 + r * 64 * WPB + tid, j1);
+#pragma unroll
+        for (int r = 0; r < R; ++r) lr[r].store(s_pos[0], s_m[0], r * 64 * WPB + tid);
+    } else if (ntile > 0 && loader) {
         ld.load(js, j0 + tid, j1);
         ld.store(s_pos[0], s_m[0], tid);
     }
     __syncthreads();
     for (int t = 0; t < ntile; ++t) {
-        const int b = t & 1;
-        if (t + 1 < ntile && loader) ld.load(js, j0 + (t + 1) * TILE + tid, j1);
+        const int b = ONE_SHOT ? 0 : (t & 1);
+        if (!ONE_SHOT && t + 1 < ntile && loader) ld.load(js, j0 + (t + 1) * TILE + tid, j1);
         const int jt = j0 + t * TILE;
         const int count = min(TILE, j1 - jt);
         if (!wave_busy) {
@@ -640,7 +654,7 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
                 since_fold = 0;
             }
         }
-        if (t + 1 < ntile && loader) ld.store(s_pos[b ^ 1], s_m[b ^ 1], tid);
+        if (!ONE_SHOT && t + 1 < ntile && loader) ld.store(s_pos[b ^ 1], s_m[b ^ 1], tid);
         __syncthreads();
     }
     double acc[6];
@@ -680,14 +694,18 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
 // 98 registers and drop to 4.
 // reset (checked precision, may be null): the flagged-target counter, zeroed
 // here so that no separate launch is needed before the flagging after it.
+// A one-shot tile's 52 KB of LDS holds 3 workgroups per CU, so it asks for 3.
 template <int JL, int TM, bool EST, int TILE = kTile, int WPB = 4>
-__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(5, 8)))
+__global__ __launch_bounds__(64 * WPB)
+__attribute__((amdgpu_waves_per_eu(TILE > 64 * WPB ? 3 : 5, 8)))
 void regf_kernel_jl(JState js, const int *__restrict__ tgt, int nt, int jspan, QueryOut out,
                     int cap, int final_counts, int *__restrict__ reset)
 {
     if (reset && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *reset = 0;
-    __shared__ double2 s_pos[2][TILE * 3];
-    __shared__ typename TermMass<TM>::type s_m[2][TILE];
+    // a one-shot tile (TILE > 64 WPB) has a single buffer
+    constexpr int NB = TILE > 64 * WPB ? 1 : 2;
+    __shared__ double2 s_pos[NB][TILE * 3];
+    __shared__ typename TermMass<TM>::type s_m[NB][TILE];
     regf_body_jl<JL, TM, EST, TILE, WPB>(js, tgt, nt, jspan, out, cap, final_counts, blockIdx.x,
                                          blockIdx.y, 0, nt, s_pos, s_m);
 }
@@ -1145,6 +1163,14 @@ hipError_t launch_mode0(const JState &js, const int *tgt, int nt, const Plan &p,
         case 128 * 8 + 2: launch_shape<JL, EST, 128, 2>(js, tgt, nt, p, out, cap, fc, reset, st); break;
         case 64 * 8 + 2: launch_shape<JL, EST, 64, 2>(js, tgt, nt, p, out, cap, fc, reset, st); break;
         case 64 * 8 + 1: launch_shape<JL, EST, 64, 1>(js, tgt, nt, p, out, cap, fc, reset, st); break;
+        case 1024 * 8 + 4:
+            // one-shot tiles: every split within one tile (fold at 16 or 64 steps)
+            if constexpr (JL >= 16) {
+                if (p.jspan > 1024) return hipErrorInvalidValue;
+                launch_shape<JL, EST, 1024, 4>(js, tgt, nt, p, out, cap, fc, reset, st);
+                break;
+            }
+            return hipErrorInvalidValue;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1204,6 +1230,7 @@ int targets_per_block(const Plan &p)
 
 bool shape_supported(int tile, int wpb)
 {
+    if (tile == kTileOneShot) return wpb == 4;
     return (tile == 64 || tile == 128 || tile == 256) && (wpb == 1 || wpb == 2 || wpb == 4) &&
            tile <= 64 * wpb;
 }

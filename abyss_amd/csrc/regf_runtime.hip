@@ -175,7 +175,8 @@ Plan plan_for(abyss_ctx_t c, int nt, int n, int prec, int cap, bool forced)
     const int force_splits = forced ? c->force_splits : 0;
     // Workgroup shape of the lanes-per-target kernels (term mode 0 only; a
     // batch member runs in the 4-wave workgroups of the persistent batch
-    // kernel): forced, else tuned, else 256 j per tile and 4 waves.
+    // kernel): forced, else tuned, else 256 j per tile and 4 waves (the
+    // one-shot tile for 64 lanes per target, below).
     int tile = abyss::kTile, wpb = 4;
     if (mode0) {
         if (forced && c->force_tile) tile = c->force_tile;
@@ -183,7 +184,12 @@ Plan plan_for(abyss_ctx_t c, int nt, int n, int prec, int cap, bool forced)
         if (forced && c->force_wpb) wpb = c->force_wpb;
         else if (tuned) wpb = te.wpb;
         if (c->batch_member) wpb = 4;
-        if (tile > 64 * wpb) {
+        if (tile == abyss::kTileOneShot) {
+            // the one-shot tile's only workgroup is 4 waves; a forced other
+            // block wins over a tuned one-shot tile
+            if (wpb != 4 && forced && c->force_wpb && !c->force_tile) tile = 64 * wpb;
+            else wpb = 4;
+        } else if (tile > 64 * wpb) {
             // a forced tile wins over a tuned or default block, and v.v.
             if (forced && c->force_wpb && !c->force_tile) tile = 64 * wpb;
             else wpb = tile / 64;
@@ -234,6 +240,16 @@ Plan plan_for(abyss_ctx_t c, int nt, int n, int prec, int cap, bool forced)
         tile = abyss::kTile;
         wpb = 4;
     }
+    // Built-in shape of blocks below 16 targets (64 lanes per target): the
+    // one-shot tile, whose splits load their j at once (Plummer + Kroupa
+    // N = 65536, m = 1..16: 12.8 -> 12.1 ... 14.3 -> 13.3 us per split
+    // launch; 16 lanes, m = 64..256, gain nothing).  Kept only when the
+    // splits fit it (below).
+    if (mode0 && jl == 64 && !tuned && !c->batch_member &&
+        !(forced && (c->force_tile || c->force_wpb))) {
+        tile = abyss::kTileOneShot;
+        wpb = 4;
+    }
     p.tile = tile;
     p.wpb = wpb;
     p.jl = jl;
@@ -269,6 +285,11 @@ Plan plan_for(abyss_ctx_t c, int nt, int n, int prec, int cap, bool forced)
     jspan = ((jspan + abyss::kTile - 1) / abyss::kTile) * abyss::kTile;
     p.jspan = jspan;
     p.splits = (n + jspan - 1) / jspan;
+    // the one-shot tile holds a whole split, with 16 or 64 lanes per target
+    // (4 lanes would fold their FP32 runs inside it), outside batches
+    if (p.tile == abyss::kTileOneShot &&
+        (p.jl < 16 || p.jspan > abyss::kTileOneShot || c->batch_member))
+        p.tile = abyss::kTile;
     return p;
 }
 
@@ -607,8 +628,8 @@ int abyss_ctx_set_tuning(abyss_ctx_t c, int ipl, int splits, int unroll, int jl)
 int abyss_ctx_set_shape(abyss_ctx_t c, int tile, int waves_per_block)
 {
     if (!c) return ABYSS_ERR_INVALID_ARG;
-    if (tile != 0 && tile != 64 && tile != 128 && tile != 256)
-        return set_detail(c, ABYSS_ERR_INVALID_ARG, "tile must be 0, 64, 128 or 256");
+    if (tile != 0 && tile != 64 && tile != 128 && tile != 256 && tile != abyss::kTileOneShot)
+        return set_detail(c, ABYSS_ERR_INVALID_ARG, "tile must be 0, 64, 128, 256 or 1024");
     if (waves_per_block != 0 && waves_per_block != 1 && waves_per_block != 2 &&
         waves_per_block != 4)
         return set_detail(c, ABYSS_ERR_INVALID_ARG, "waves_per_block must be 0, 1, 2 or 4");

@@ -5,7 +5,9 @@ LDS tile, wave64s per workgroup) and of the autotuner that picks them.
   same lanes per target and j-split count, in split and in checked
   precision.  A full tile is a divisor of the 64-step FP32 run and split
   spans are whole 256-j blocks, so each lane folds its runs at the same j
-  whatever the tile; the workgroup only groups targets.
+  whatever the tile; the workgroup only groups targets.  The one-shot tile
+  (1024 j, a whole split of at most 1024 j) likewise, with 16 or 64 lanes
+  per target; plans it cannot hold fall back to 256.
 * A batched ensemble runs each cluster with the tile of the installed table
   and gives the bits it gives without the table (the table used keeps the
   built-in lanes-per-target rule, so only the shape differs).
@@ -87,6 +89,47 @@ def test_shapes_give_the_same_bits(plummer, prec):
         assert err["a_norm"]["max"] <= 1e-6 and err["jdot_norm"]["max"] <= 1e-6
 
 
+@pytest.mark.parametrize("prec", [regf.PREC_SPLIT, regf.PREC_CHECKED])
+def test_one_shot_tile_gives_the_same_bits(plummer, prec):
+    js = plummer
+    rng = np.random.default_rng(23)
+    cases = {"random1": rng.permutation(js.n)[:1].astype(np.int32),
+             "random16": rng.permutation(js.n)[:16].astype(np.int32),
+             "first64": np.arange(64, dtype=np.int32),
+             "random200": rng.permutation(js.n)[:200].astype(np.int32)}
+    with regf.Context(device=0, cap=js.cap, precision=prec) as c:
+        c.upload(js.m, js.x, js.v, js.r2)
+        for name, t in cases.items():
+            for jl in (16, 64):
+                say(f"case: one-shot tile prec={prec} targets={name} jl={jl}")
+                c.set_tuning(1, 64, 2, jl)     # splits of 1024 j
+                c.set_shape(256, 4)
+                base = c.query(t)
+                base_nre = c.last_recheck() if prec == regf.PREC_CHECKED else 0
+                c.set_shape(1024, 4)
+                r = c.query(t)
+                p = c.plan()
+                assert (p["tile"], p["j_splits"]) == (1024, 64), p
+                assert _same(r, base), (name, jl)
+                if prec == regf.PREC_CHECKED:
+                    assert c.last_recheck() == base_nre
+        # the built-in plan of a block below 16 targets uses it
+        c.set_tuning()
+        c.set_shape()
+        c.query(cases["random1"])
+        p = c.plan()
+        assert (p["lanes_per_target"], p["tile"]) == (64, 1024), p
+        # plans the one-shot tile cannot hold fall back to 256 j
+        for jl, splits in ((4, 64), (16, 7), (64, 1)):
+            c.set_tuning(1, splits, 2, jl)
+            c.set_shape(1024, 0)
+            c.query(cases["random16"])
+            p = c.plan()
+            assert p["tile"] == 256 and p["waves_per_block"] == 4, (jl, splits, p)
+        c.set_tuning()
+        c.set_shape()
+
+
 def test_shapes_full_n_king():
     """The full-N King step (N = 262144, every particle a target) in three
     shapes, split precision: identical bits."""
@@ -109,7 +152,7 @@ def test_shapes_full_n_king():
 
 def test_shape_arguments_are_checked():
     with regf.Context(device=0, cap=64) as c:
-        for tile, wpb in ((32, 4), (512, 4), (256, 2), (128, 1), (64, 3)):
+        for tile, wpb in ((32, 4), (512, 4), (256, 2), (128, 1), (64, 3), (1024, 2)):
             with pytest.raises(regf.AbyssError):
                 c.set_shape(tile, wpb)
         c.set_shape(256, 0)   # the other one automatic
@@ -173,7 +216,7 @@ def test_autotune_on_own_jset(tmp_path, plummer):
         assert len(rows) == 32 and all(len(r) == 6 for r in rows)
         shapes = {(int(r[4]), int(r[5])) for r in rows}
         say(f"  tuned shapes: {sorted(shapes)}")
-        assert shapes <= set(SHAPES)
+        assert shapes <= set(SHAPES) | {(1024, 4)}
         regf.autotune_clear(0)
         regf.autotune_load(0, path)
         with regf.Context(device=0, cap=js.cap) as c:
