@@ -2,8 +2,8 @@
 //
 // For target counts m = 1, 4, 16, ... up to n on an n-particle cluster (a
 // synthetic uniform cube with r_nb for ~48 neighbours, or the caller's own
-// j-set), every candidate plan is timed under the library's default
-// precision (checked) with HIP events (median of 5 launches, 3 above 1 ms):
+// j-set), every candidate plan of the FP32 pass is timed under split
+// precision with HIP events (median of 5 launches, 3 above 1 ms):
 //   lanes per target  JL in {4, 16, 64} (the lane-group kernels; the
 //                     one-target-per-lane kernel runs only when forced)
 //   shape             j per LDS tile and wave64s per workgroup (targets per
@@ -20,8 +20,11 @@
 // between measured points take the nearest one).  The table can be written
 // to and read from a small text file.  Batched ensembles take the lanes per
 // target and the tile of each cluster from the same table (their persistent
-// workgroups are 4 waves).  No choice changes a result bit: the plan's
-// j-split count is the only one that changes the order of the FP64 sums.
+// workgroups are 4 waves).  The table governs the FP32 pass: split
+// precision, and checked precision above 256 targets (smaller checked
+// blocks run in FP64 terms with the built-in rules).  No choice changes a
+// result bit: the plan's j-split count is the only one that changes the
+// order of the FP64 sums.
 #include "regf_context.h"
 
 #include <math.h>
@@ -126,7 +129,7 @@ int abyss_autotune_jset(int device, int n, const double *mass, const double *pos
     int rc = abyss_ctx_create(device, 256, &c);
     if (rc != ABYSS_OK) return rc;
     abyss::Rng rng{0x9E3779B97F4A7C15ull};
-    rc = abyss_ctx_set_precision(c, ABYSS_PREC_CHECKED);
+    rc = abyss_ctx_set_precision(c, ABYSS_PREC_SPLIT);
     if (rc == ABYSS_OK) rc = abyss_ctx_upload(c, n, mass, pos, vel, r2);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (rc == ABYSS_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))

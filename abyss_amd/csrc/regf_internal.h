@@ -58,7 +58,14 @@ struct Plan {
     int unroll = 2;  // j per unrolled inner-loop step
     int jl = 1;      // lanes per target (1: one target per lane; 4/16/64)
     int wpb = 4;     // wave64s per workgroup (lanes-per-target kernels)
+    // checked precision of a small block: every target in FP64 terms in one
+    // pass (kPrecRecheck arithmetic, with lists), no flag or recomputation
+    bool fp64_terms = false;
 };
+// Checked-precision blocks of at most this many targets (outside batches)
+// run in FP64 terms in one pass: below ~1000 targets that is faster than
+// the FP32 pass, its flag and its recomputation together.
+constexpr int kCheckedDirectMax = 256;
 
 // Plan choice measured by the autotuner for one target-count bucket.
 struct TuneEntry {

@@ -35,6 +35,9 @@ constexpr size_t kOneTripListBytes = 1u << 20;
 // Up to this many targets the kernels read the target list straight from
 // mapped host memory instead of a device copy (saves one copy per query).
 constexpr int kMappedTargets = 64;
+// a checked block with a mapped mirror runs in FP64 terms in one pass, so no
+// recomputation ever has to update the mirror
+static_assert(abyss::kCheckedDirectMax >= kMappedTargets, "mirrored blocks are direct");
 
 int set_detail(abyss_ctx_t c, int code, const char *fmt, ...)
 {
@@ -169,7 +172,15 @@ Plan plan_for(abyss_ctx_t c, int nt, int n, int prec, int cap, bool forced)
     // peak at N = m = 262144); smaller blocks spread a target over 16 lanes
     // (16..255 targets) or 64 (below 16), measured best at N = 262144.
     abyss::TuneEntry te{0, 0, 16};
-    const bool mode0 = prec == ABYSS_PREC_SPLIT || prec == ABYSS_PREC_CHECKED;
+    // a small checked block runs in FP64 terms in one pass (term mode 3,
+    // the recomputation's arithmetic, with lists): Plummer + Kroupa
+    // N = 65536, m = 1 / 64 / 256: 15.7 -> 12.7, 25.2 -> 21.1, 54.4 -> 35.3
+    // us; King N = 262144, m = 16 / 256: 42.2 -> 23.9, 111.6 -> 102.6 us;
+    // m = 1024 is a tie or slower
+    const bool direct = prec == ABYSS_PREC_CHECKED && nt <= abyss::kCheckedDirectMax &&
+                        !c->batch_member;
+    p.fp64_terms = direct;
+    const bool mode0 = prec == ABYSS_PREC_SPLIT || (prec == ABYSS_PREC_CHECKED && !direct);
     const bool tuned = forced && mode0 && abyss::tuned_plan(c->device, nt, &te);
     const int force_jl = forced ? c->force_jl : 0;
     const int force_splits = forced ? c->force_splits : 0;
@@ -344,12 +355,7 @@ int ensure_check(abyss_ctx_t c, int nt, int S)
         HIPCHK(c, hipMalloc(&c->d_chk, grow(need)));
         c->chk_bytes = grow(need);
     }
-    if (!c->h_chk) {
-        HIPCHK(c, hipHostMalloc(&c->h_chk, 256, hipHostMallocMapped | hipHostMallocCoherent));
-        void *dev = nullptr;
-        HIPCHK(c, hipHostGetDevicePointer(&dev, c->h_chk, 0));
-        c->h_chk_dev = static_cast<int *>(dev);
-    }
+    if (!c->h_chk) HIPCHK(c, hipHostMalloc(&c->h_chk, 256, hipHostMallocDefault));
     char *b = c->d_chk;
     c->chk_spread = reinterpret_cast<float *>(b);
     b += b_sp;
@@ -414,12 +420,11 @@ double check_k2(abyss_ctx_t c)
 // After a checked launch's force and finalize: flags the targets whose a or
 // jdot may miss the tolerance (unless the finalize of a split launch has
 // flagged them: `flagged`) and recomputes them with FP64 terms over the
-// same regular set.  Blocks of up to kMappedTargets targets plan the
-// recomputation for all of them and read the flagged count on the device
-// (no host wait; the count reaches h_chk[1]); larger blocks read the count
-// back first (a host wait) and plan for it.  `mirror`: the finalize wrote
-// the mapped host copy, which the recomputed targets must update too.
-int recheck(abyss_ctx_t c, bool mirror, bool flagged)
+// same regular set: the flagged count is read back (a host wait) and the
+// recomputation planned for it.  (Blocks small enough to skip that wait run
+// in FP64 terms in one pass instead, Plan::fp64_terms; a finalize mirror is
+// written only for them, so the recomputed targets never have to update it.)
+int recheck(abyss_ctx_t c, bool flagged)
 {
     if (!flagged) {
         // one-split launches: a flag pass over the final results (the force
@@ -427,18 +432,6 @@ int recheck(abyss_ctx_t c, bool mirror, bool flagged)
         HIPCHK(c, abyss::launch_check_flag(c->d_acc, c->d_adot, c->chk_spread, c->tgt, c->nt,
                                            check_k2(c), c->chk_counters, c->chk_fq, c->chk_fi,
                                            c->stream));
-    }
-    if (c->nt <= kMappedTargets) {
-        const Plan rp = plan_for(c, c->nt, c->n, ABYSS_PREC_REFINED, 0, false);
-        QueryOut rpart;
-        int rc = ensure_recheck_scratch(c, (size_t)rp.splits * c->nt, &rpart);
-        if (rc != ABYSS_OK) return rc;
-        HIPCHK(c, abyss::launch_recheck_dev(c->js, c->chk_fi, c->chk_counters, c->nt, rp, rpart,
-                                            c->chk_fq, c->d_acc, c->d_adot,
-                                            mirror ? c->mirror : abyss::FinalMirror{},
-                                            c->h_chk_dev + 1, c->stream));
-        c->recheck_pending = true;
-        return ABYSS_OK;
     }
     HIPCHK(c, hipMemcpyAsync(c->h_chk, c->chk_counters, sizeof(int), hipMemcpyDeviceToHost,
                              c->stream));
@@ -664,13 +657,6 @@ int abyss_ctx_set_check_tolerance(abyss_ctx_t c, double tol)
 int abyss_ctx_last_recheck(abyss_ctx_t c, int *count)
 {
     if (!c || !count) return ABYSS_ERR_INVALID_ARG;
-    if (c->recheck_pending) {
-        // the count of a small block arrives with the launch's last kernel
-        HIPCHK(c, hipSetDevice(c->device));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        c->n_rechecked = c->h_chk[1];
-        c->recheck_pending = false;
-    }
     *count = c->n_rechecked;
     return ABYSS_OK;
 }
@@ -679,7 +665,10 @@ int abyss_ctx_fetch_spread(abyss_ctx_t c, float *spread)
 {
     if (!c || !spread) return ABYSS_ERR_INVALID_ARG;
     if (!c->spread_valid)
-        return set_detail(c, ABYSS_ERR_INVALID_ARG, "no checked-precision launch to read");
+        return set_detail(c, ABYSS_ERR_INVALID_ARG,
+                          "no spread to read: the last launch was not a checked FP32 pass "
+                          "(checked blocks of at most %d targets run in FP64 terms)",
+                          abyss::kCheckedDirectMax);
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipMemcpyAsync(spread, c->chk_spread, 2 * (size_t)c->nt * sizeof(float),
                              hipMemcpyDeviceToHost, c->stream));
@@ -850,10 +839,12 @@ int ctx_launch_force(abyss_ctx_t c)
         c->plan_valid = true;
     }
     const Plan &p = c->plan;
-    const bool checked = c->precision == ABYSS_PREC_CHECKED;
+    // checked precision: the FP32 pass with its flag and recomputation, or
+    // for a small block every target in FP64 terms (p.fp64_terms)
+    const bool direct = c->precision == ABYSS_PREC_CHECKED && p.fp64_terms;
+    const bool checked = c->precision == ABYSS_PREC_CHECKED && !direct;
     c->spread_valid = false;
-    c->n_rechecked = 0;
-    c->recheck_pending = false;
+    c->n_rechecked = direct ? c->nt : 0;
     if (checked) {
         int rc = ensure_check(c, c->nt, p.splits);
         if (rc != ABYSS_OK) return rc;
@@ -879,7 +870,8 @@ int ctx_launch_force(abyss_ctx_t c)
     // and a split launch's finalize flags the targets (no launches of their
     // own)
     HIPCHK(c, abyss::launch_regf(c->js, c->tgt, c->nt, p, p.splits == 1 ? fin : part, c->cap,
-                                 c->precision, c->stream, checked ? c->chk_counters : nullptr));
+                                 direct ? abyss::kPrecRecheck : c->precision, c->stream,
+                                 checked ? c->chk_counters : nullptr));
     if (c->ktiming) {
         HIPCHK(c, hipEventRecord(c->kev[2 * c->kev_used + 1], c->stream));
         c->kev_used += 1;
@@ -905,7 +897,7 @@ int ctx_launch_force(abyss_ctx_t c)
         c->mirror_valid = mirror;
     }
     if (checked) {
-        int rc = recheck(c, c->mirror_valid, p.splits > 1);
+        int rc = recheck(c, p.splits > 1);
         if (rc != ABYSS_OK) return rc;
         c->spread_valid = true;
     }
@@ -1185,7 +1177,10 @@ int abyss_query_noalloc(int device, int n, const double *jset, int nt, const int
     tmp.precision = precision;
     tmp.scratch_limit = ABYSS_NOALLOC_SPLIT_BYTES;
     const Plan p = plan_for(&tmp, nt, n, precision, cap, true);
-    const bool checked = precision == ABYSS_PREC_CHECKED;
+    // a small checked block: every target in FP64 terms, as a context does
+    const bool direct = precision == ABYSS_PREC_CHECKED && p.fp64_terms;
+    const bool checked = precision == ABYSS_PREC_CHECKED && !direct;
+    const int kprec = direct ? abyss::kPrecRecheck : precision;
     const size_t S = (size_t)p.splits, t = (size_t)nt;
 
     Carve cv{base};
@@ -1249,9 +1244,9 @@ int abyss_query_noalloc(int device, int n, const double *jset, int nt, const int
     QueryOut fin{acc, adot, nnb, list};
     fin.spread = spread;
     if (S == 1) {
-        NACHK(abyss::launch_regf(js, tgt_s, nt, p, fin, cap, precision, st));
+        NACHK(abyss::launch_regf(js, tgt_s, nt, p, fin, cap, kprec, st));
     } else {
-        NACHK(abyss::launch_regf(js, tgt_s, nt, p, part, cap, precision, st));
+        NACHK(abyss::launch_regf(js, tgt_s, nt, p, part, cap, kprec, st));
         NACHK(abyss::launch_finalize(part, p.splits, nt, cap, acc, adot, nnb, list,
                                      abyss::FinalMirror{}, spread, st));
     }

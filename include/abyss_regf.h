@@ -237,15 +237,17 @@ int abyss_ctx_kernel_timing(abyss_ctx_t ctx, double *force_ms, long long *launch
  * terms dominate both) and recomputes with FP64 terms every target with
  *   ABYSS_CHECK_ERR_SCALE * sqrt(Sa) > tol * |a|  or
  *   ABYSS_CHECK_ERR_SCALE * sqrt(Sj) > tol * |adot|,
- * before the results are fetched.  A launch of more than 64 targets under
- * this mode waits once on the host (for the flagged count, to plan the
- * recomputation); smaller blocks read the count on the device and do not
- * wait.  No counterpart in the reference (whose FP32 build has no accuracy
- * control).
+ * before the results are fetched; such a launch waits once on the host (for
+ * the flagged count, to plan the recomputation).  A block of at most 256
+ * targets (outside batched ensembles) instead evaluates every target with
+ * FP64 terms in one pass, which is faster there and waits for nothing; its
+ * lists are those of the FP32 pass as well.  No counterpart in the
+ * reference (whose FP32 build has no accuracy control).
  *   set_check_tolerance: tol (> 0; default 1e-6);
- *   last_recheck: targets recomputed by the last launch (waits for it);
+ *   last_recheck: targets recomputed by the last launch (all of a block of
+ *     at most 256);
  *   fetch_spread: [nt][2] = sqrt(Sa), sqrt(Sj) per target of the last
- *     launch. */
+ *     launch (a launch of more than 256 targets). */
 #define ABYSS_CHECK_ERR_SCALE 1.0e-6
 int abyss_ctx_set_check_tolerance(abyss_ctx_t ctx, double tol);
 int abyss_ctx_last_recheck(abyss_ctx_t ctx, int *count);
@@ -344,12 +346,13 @@ int abyss_ctx_derivs23(abyss_ctx_t ctx, int nt, const int *targets, int referenc
  * nmax = NumNeighborMax (1000), src_mpi/def.h:18,20. */
 int abyss_ctx_update_radius(abyss_ctx_t ctx, int fix, int nmax, double *r2_out);
 
-/* Plan autotuner.  Times candidate plans (lanes per target 4, 16 or 64, j
- * per LDS tile, wave64s per workgroup, workgroups per CU -> j-split count)
- * under checked precision for target counts 1, 4, 16, ... n on a synthetic
+/* Plan autotuner.  Times candidate plans of the FP32 pass (lanes per target
+ * 4, 16 or 64, j per LDS tile, wave64s per workgroup, workgroups per CU ->
+ * j-split count) for target counts 1, 4, 16, ... n on a synthetic
  * n-particle cluster (uniform cube, n >= 256) and installs the fastest per
- * power-of-two target-count bucket for every later automatic plan on
- * `device` (split and checked precision; forced tunings still win).  Writes the table to
+ * power-of-two target-count bucket for every later automatic plan of that
+ * pass on `device` (split precision, and checked precision above 256
+ * targets; forced tunings still win).  Writes the table to
  * `cache_path` when not NULL (one line per bucket: bucket, lanes per
  * target, j-unroll, workgroups per CU, tile, waves).  Takes seconds to a
  * minute; results of a query depend on its plan only through the FP64

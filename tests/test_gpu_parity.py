@@ -181,16 +181,20 @@ def _same_set_rows(nnb, lst, rn, rl):
 
 
 CHECK_TOL = 1e-6
+# checked blocks of at most this many targets run in FP64 terms in one pass
+CHECKED_DIRECT_MAX = 256
 
 
 @pytest.mark.parametrize("wl,size", [("plummer1024", 1024), ("kroupa", 1), ("kroupa", 100),
                                      ("kroupa", 4096), ("kroupa", 65536)])
 def test_checked_precision(ctx, wl, size):
     """Checked precision: every result within 1e-6 relative of the FP64
-    oracle (a and jdot, per particle, maximum), lists and counts those of
-    split precision, unflagged targets bitwise split precision, recomputed
-    targets FP64-accurate, and the recomputed set the one the flag rule
-    names."""
+    oracle (a and jdot, per particle, maximum) and lists and counts those of
+    split precision.  Blocks above 256 targets: unflagged targets bitwise
+    split precision, recomputed targets FP64-accurate, and the recomputed
+    set the one the flag rule names.  Smaller blocks: every target in FP64
+    terms in one pass, bitwise refined precision where its neighbour test
+    agrees."""
     say(f"case: checked precision {wl} block={size}")
     js = workloads.plummer_equal(1024) if wl == "plummer1024" else workloads.plummer_kroupa(65536)
     ctx.upload(js.m, js.x, js.v, js.r2)
@@ -200,10 +204,33 @@ def test_checked_precision(ctx, wl, size):
     try:
         chk = ctx.query(t)
         nre = ctx.last_recheck()
-        spr = ctx.spread().astype(np.float64)
+        if size <= CHECKED_DIRECT_MAX:
+            with pytest.raises(regf.AbyssError):
+                ctx.spread()
+            ctx.set_precision(regf.PREC_REFINED)
+            refined = ctx.query(t)
+        else:
+            spr = ctx.spread().astype(np.float64)
     finally:
         ctx.set_precision(regf.PREC_SPLIT)
     assert np.array_equal(chk[2], split[2]) and np.array_equal(chk[3], split[3])
+    ra, rj, rn, rl = oracle.regf(js.x, js.v, js.m, js.r2, t, js.cap)
+    rows = _same_set_rows(chk[2], chk[3], rn, rl)
+    err = force_errors(chk[0][rows], chk[1][rows], ra[rows], rj[rows])
+    say(f"[checked {wl}/{size}] recomputed={nre} a_rel={err['a_rel']['max']:.2e} "
+        f"jdot_rel={err['jdot_rel']['max']:.2e}")
+    assert err["a_rel"]["max"] <= CHECK_TOL and err["jdot_rel"]["max"] <= CHECK_TOL
+    if size <= CHECKED_DIRECT_MAX:
+        # a small block: every target in FP64 terms in one pass, the sums of
+        # refined precision wherever the two neighbour tests agree (the
+        # refined test compares the FP64 r^2; rows with a pair in that band
+        # sum a different regular set)
+        assert nre == size
+        same = np.all(refined[3] == split[3], axis=1) & (refined[2] == split[2])
+        assert same.sum() >= size - 2
+        assert np.array_equal(chk[0][same], refined[0][same])
+        assert np.array_equal(chk[1][same], refined[1][same])
+        return
     changed = np.any(chk[0] != split[0], axis=1) | np.any(chk[1] != split[1], axis=1)
     assert changed.sum() <= nre <= size
     # the flag rule, evaluated on the host from the split results (a 1 %
@@ -226,12 +253,12 @@ def test_checked_precision(ctx, wl, size):
         assert rerr["a_rel"]["max"] <= 1e-11 and rerr["jdot_rel"]["max"] <= 1e-11
 
 
-@pytest.mark.parametrize("size", [8, 64])
-def test_checked_precision_small_block_with_recomputation(ctx, size):
-    """Blocks of up to 64 targets read the flagged count on the device (no
-    host wait) and update the mapped copy of the results.  A block made of
-    targets that a full-N checked launch had to recompute exercises that
-    path with a non-zero count."""
+@pytest.mark.parametrize("size", [8, 64, 256])
+def test_checked_precision_small_block_in_fp64_terms(ctx, size):
+    """Blocks of up to 256 targets evaluate every target in FP64 terms in one
+    pass: no spread, every target counted as recomputed, lists those of
+    split precision, and FP64 accuracy on the targets that a full-N checked
+    launch had to recompute as on the others."""
     say(f"case: checked precision, small block of {size} flagged-prone targets")
     js = workloads.plummer_kroupa(65536)
     ctx.upload(js.m, js.x, js.v, js.r2)
@@ -246,21 +273,20 @@ def test_checked_precision_small_block_with_recomputation(ctx, size):
         t = np.random.default_rng(size).permutation(t).astype(np.int32)
         chk = ctx.query(t)
         nre = ctx.last_recheck()
-        spr = ctx.spread()
+        with pytest.raises(regf.AbyssError):
+            ctx.spread()
     finally:
         ctx.set_precision(regf.PREC_SPLIT)
     split = ctx.query(t)
     assert len(t) == size and ctx.plan()["j_splits"] > 1
-    changed = np.any(chk[0] != split[0], axis=1) | np.any(chk[1] != split[1], axis=1)
-    say(f"[checked small {size}] recomputed={nre} changed={changed.sum()}")
-    assert nre > 0 and changed.sum() <= nre
+    say(f"[checked small {size}] recomputed={nre}")
+    assert nre == size
     assert np.array_equal(chk[2], split[2]) and np.array_equal(chk[3], split[3])
-    assert spr.shape == (size, 2) and np.all(np.isfinite(spr))
     ra, rj, rn, rl = oracle.regf(js.x, js.v, js.m, js.r2, t, js.cap)
-    err = force_errors(chk[0], chk[1], ra, rj)
-    assert err["a_rel"]["max"] <= CHECK_TOL and err["jdot_rel"]["max"] <= CHECK_TOL
-    rerr = force_errors(chk[0][changed], chk[1][changed], ra[changed], rj[changed])
-    assert rerr["a_rel"]["max"] <= 1e-11 and rerr["jdot_rel"]["max"] <= 1e-11
+    rows = _same_set_rows(chk[2], chk[3], rn, rl)
+    err = force_errors(chk[0][rows], chk[1][rows], ra[rows], rj[rows])
+    say(f"  a_rel={err['a_rel']['max']:.2e} jdot_rel={err['jdot_rel']['max']:.2e}")
+    assert err["a_rel"]["max"] <= 1e-11 and err["jdot_rel"]["max"] <= 1e-11
     # the legacy entry points (their own context, list stride 1000) give the
     # same results for the same block
     regf.OpenDevice(0)
