@@ -690,24 +690,45 @@ __device__ __forceinline__ void regf_body_jl(const JState &js, const int *__rest
     }
 }
 
+// XCD-aware placement of a launch's gx * S (target block, split) pairs on
+// its 1-D grid.  The hardware deals workgroup ids round-robin to the 8 XCDs,
+// each with its own L2: id runs on XCD id % 8 as that XCD's (id / 8)-th
+// workgroup.  XCD x takes the contiguous run [start(x), start(x) + n(x)) of
+// the split-major order L = s * gx + b (n(x) = total / 8, plus one for the
+// first total % 8 XCDs), so the workgroups one XCD runs at a time share one
+// or two splits' j-ranges, which its L2 keeps, instead of every XCD
+// streaming every split.  A bijection of [0, total): every pair runs once,
+// with the same arithmetic as on a 2-D grid.
+__device__ __forceinline__ void xcd_place(int id, int total, int gx, int &bx, int &s)
+{
+    const int x = id & 7, k = id >> 3;
+    const int q = total >> 3, r = total & 7;
+    const int L = x * q + min(x, r) + k;
+    s = L / gx;
+    bx = L - s * gx;
+}
+
 // 5 waves per SIMD (<= 96 VGPRs); the FP64-term modes would otherwise take
 // 98 registers and drop to 4.
 // reset (checked precision, may be null): the flagged-target counter, zeroed
 // here so that no separate launch is needed before the flagging after it.
 // A one-shot tile's 52 KB of LDS holds 3 workgroups per CU, so it asks for 3.
+// Grid: gx * splits workgroups on x (xcd_place).
 template <int JL, int TM, bool EST, int TILE = kTile, int WPB = 4>
 __global__ __launch_bounds__(64 * WPB)
 __attribute__((amdgpu_waves_per_eu(TILE > 64 * WPB ? 3 : 5, 8)))
 void regf_kernel_jl(JState js, const int *__restrict__ tgt, int nt, int jspan, QueryOut out,
-                    int cap, int final_counts, int *__restrict__ reset)
+                    int cap, int final_counts, int *__restrict__ reset, int gx, int splits)
 {
-    if (reset && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *reset = 0;
+    if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;
     // a one-shot tile (TILE > 64 WPB) has a single buffer
     constexpr int NB = TILE > 64 * WPB ? 1 : 2;
     __shared__ double2 s_pos[NB][TILE * 3];
     __shared__ typename TermMass<TM>::type s_m[NB][TILE];
-    regf_body_jl<JL, TM, EST, TILE, WPB>(js, tgt, nt, jspan, out, cap, final_counts, blockIdx.x,
-                                         blockIdx.y, 0, nt, s_pos, s_m);
+    int bx, s;
+    xcd_place(blockIdx.x, gx * splits, gx, bx, s);
+    regf_body_jl<JL, TM, EST, TILE, WPB>(js, tgt, nt, jspan, out, cap, final_counts, bx, s, 0,
+                                         nt, s_pos, s_m);
 }
 
 // One wave per target.  Partial sums: lane l adds splits l, l+64, ... in
@@ -897,14 +918,15 @@ __global__ __launch_bounds__(256) void check_flag_kernel(const double *__restric
 template <int JL>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
 void recheck_kernel_dev(JState js, const int *__restrict__ tgt, const int *__restrict__ count,
-                        int nt_stride, int jspan, QueryOut out)
+                        int nt_stride, int jspan, QueryOut out, int gx, int splits)
 {
     __shared__ double2 s_pos[2][kTile * 3];
     __shared__ double s_m[2][kTile];
     const int m = *count;
-    if ((int)blockIdx.x * (4 * (64 / JL)) >= m) return;   // the whole workgroup
-    regf_body_jl<JL, 3>(js, tgt, m, jspan, out, 0, 0, blockIdx.x, blockIdx.y, 0, nt_stride, s_pos,
-                        s_m);
+    int bx, s;
+    xcd_place(blockIdx.x, gx * splits, gx, bx, s);
+    if (bx * (4 * (64 / JL)) >= m) return;   // the whole workgroup
+    regf_body_jl<JL, 3>(js, tgt, m, jspan, out, 0, 0, bx, s, 0, nt_stride, s_pos, s_m);
 }
 
 // One wave per flagged target k < m (m = *m_dev when m_dev is set; then
@@ -1148,8 +1170,9 @@ template <int JL, bool EST, int TILE, int WPB>
 void launch_shape(const JState &js, const int *tgt, int nt, const Plan &p, const QueryOut &out,
                   int cap, int fc, int *reset, hipStream_t st)
 {
-    hipLaunchKernelGGL((regf_kernel_jl<JL, 0, EST, TILE, WPB>), dim3(p.grid_x, p.splits),
-                       dim3(64 * WPB), 0, st, js, tgt, nt, p.jspan, out, cap, fc, reset);
+    hipLaunchKernelGGL((regf_kernel_jl<JL, 0, EST, TILE, WPB>), dim3(p.grid_x * p.splits),
+                       dim3(64 * WPB), 0, st, js, tgt, nt, p.jspan, out, cap, fc, reset, p.grid_x,
+                       p.splits);
 }
 
 template <int JL, bool EST>
@@ -1181,7 +1204,9 @@ template <int JL>
 hipError_t launch_jl(const JState &js, const int *tgt, int nt, const Plan &p, const QueryOut &out,
                      int cap, int tm, bool est, int *reset, hipStream_t st)
 {
-    dim3 grid(p.grid_x, p.splits);
+    // one workgroup per (target block, split) on a 1-D grid (xcd_place)
+    if ((long long)p.grid_x * p.splits > 0x7fffffffLL) return hipErrorInvalidValue;
+    dim3 grid(p.grid_x * p.splits);
     const int fc = p.splits == 1 ? 1 : 0;
     if (tm != 0 && (p.tile != kTile || p.wpb != 4)) return hipErrorInvalidValue;
     if (tm == 0 && est)
@@ -1190,13 +1215,13 @@ hipError_t launch_jl(const JState &js, const int *tgt, int nt, const Plan &p, co
         return launch_mode0<JL, false>(js, tgt, nt, p, out, cap, fc, reset, st);
     else if (tm == 1)
         hipLaunchKernelGGL((regf_kernel_jl<JL, 1, false>), grid, dim3(kBlock), 0, st, js, tgt,
-                           nt, p.jspan, out, cap, fc, reset);
+                           nt, p.jspan, out, cap, fc, reset, p.grid_x, p.splits);
     else if (tm == 2)
         hipLaunchKernelGGL((regf_kernel_jl<JL, 2, false>), grid, dim3(kBlock), 0, st, js, tgt,
-                           nt, p.jspan, out, cap, fc, reset);
+                           nt, p.jspan, out, cap, fc, reset, p.grid_x, p.splits);
     else
         hipLaunchKernelGGL((regf_kernel_jl<JL, 3, false>), grid, dim3(kBlock), 0, st, js, tgt,
-                           nt, p.jspan, out, cap, fc, reset);
+                           nt, p.jspan, out, cap, fc, reset, p.grid_x, p.splits);
     return hipGetLastError();
 }
 
@@ -1300,16 +1325,17 @@ hipError_t launch_recheck_dev(const JState &js, const int *fi, const int *count,
 {
     if ((p.jl != 4 && p.jl != 16 && p.jl != 64) || p.tile != kTile || p.wpb != 4)
         return hipErrorInvalidValue;
-    dim3 grid(p.grid_x, p.splits);
+    if ((long long)p.grid_x * p.splits > 0x7fffffffLL) return hipErrorInvalidValue;
+    dim3 grid(p.grid_x * p.splits);
     if (p.jl == 4)
         hipLaunchKernelGGL(recheck_kernel_dev<4>, grid, dim3(kBlock), 0, st, js, fi, count, nt,
-                           p.jspan, out);
+                           p.jspan, out, p.grid_x, p.splits);
     else if (p.jl == 16)
         hipLaunchKernelGGL(recheck_kernel_dev<16>, grid, dim3(kBlock), 0, st, js, fi, count, nt,
-                           p.jspan, out);
+                           p.jspan, out, p.grid_x, p.splits);
     else
         hipLaunchKernelGGL(recheck_kernel_dev<64>, grid, dim3(kBlock), 0, st, js, fi, count, nt,
-                           p.jspan, out);
+                           p.jspan, out, p.grid_x, p.splits);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(recheck_finalize_kernel, dim3((nt + 3) / 4), dim3(256), 0, st, out,

@@ -1036,8 +1036,13 @@ def main():
              "split": "f64 i-j differences, f32 r^2 / rsqrt / terms, f32 runs of 64 pairs "
                       "folded into f64 sums",
              "fp64": "f64 throughout"}[args.precision]
-    kname = {"checked": "regf_kernel_jl<JL, 0, true>", "split": "regf_kernel_jl<JL, 0, false>",
-             "fp64": "regf_kernel<IPL, 2, true>"}[args.precision]
+    # the dominant kernel as the trace names it (template arguments from the plan)
+    if args.precision == "fp64":
+        kname = f"regf_kernel<{plan['i_per_lane']}, 2, true>"
+    else:
+        kname = (f"regf_kernel_jl<{plan['lanes_per_target']}, 0, "
+                 f"{'true' if args.precision == 'checked' else 'false'}, {plan['tile']}, "
+                 f"{plan['waves_per_block']}>")
     line = {
         "metric": "regular-force pair-interactions/s",
         "value": value,
