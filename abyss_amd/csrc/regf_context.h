@@ -103,11 +103,8 @@ struct abyss_ctx_s {
     abyss::Plan plan{};
     bool plan_valid = false;
 
-    // timing / counters (ProfileDevice); events are recorded around the
-    // force kernels only when `timed` (the legacy entry points' context)
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool timed = false;
-    bool timing_pending = false;
+    // counters (ProfileDevice); t_grav_ms is the legacy entry points'
+    // wall-clock time in CalculateAccelerationOnDevice
     long long n_send = 0, n_grav = 0, sum_ni = 0;
     double interactions = 0.0, t_grav_ms = 0.0;
 

@@ -22,6 +22,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 namespace {
@@ -72,7 +73,6 @@ int legacy_init(int irank)
         return rc;
     }
     g_init_detail[0] = 0;
-    g_ctx->timed = true;
     abyss_ctx_set_precision(g_ctx, g_prec);
     return ABYSS_OK;
 }
@@ -222,6 +222,10 @@ void CalculateAccelerationOnDevice(int *NumTarget, int *h_target_list,
         legacy_status(set_ctx_detail("NumTarget is null"));
         return;
     }
+    // grav(s) of ProfileDevice: the wall-clock time of the call, as the
+    // reference's host timers measured it (src/cuda/old/
+    // cuda_acceleration.cu_nbody:756)
+    const auto t0 = std::chrono::steady_clock::now();
 #ifdef ABYSS_LEGACY_FLOAT
     const int nt = *NumTarget;
     if (nt <= 0 || !acc || !adot) {
@@ -239,11 +243,14 @@ void CalculateAccelerationOnDevice(int *NumTarget, int *h_target_list,
                 adot[k][d] = (float)rj[3 * (size_t)k + d];
             }
     }
-    legacy_status(rc);
 #else
-    legacy_status(abyss_ctx_query(g_ctx, *NumTarget, h_target_list, acc ? &acc[0][0] : nullptr,
-                                  adot ? &adot[0][0] : nullptr, NumNeighbor, NeighborList));
+    const int rc = abyss_ctx_query(g_ctx, *NumTarget, h_target_list,
+                                   acc ? &acc[0][0] : nullptr, adot ? &adot[0][0] : nullptr,
+                                   NumNeighbor, NeighborList);
 #endif
+    g_ctx->t_grav_ms +=
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    legacy_status(rc);
 }
 
 }  // extern "C"

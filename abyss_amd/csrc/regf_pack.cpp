@@ -11,6 +11,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <atomic>
 #include <condition_variable>
 #include <functional>
 #include <mutex>
@@ -130,7 +131,88 @@ PackStatus pack_range(int n, int j0, int j1, const double *m, const double *x, c
     return {};
 }
 
+// Particles [j0, j1): positions only (POS) or the rest only (!POS).
+template <bool POS>
+PackStatus pack_part(int n, int j0, int j1, const double *m, const double *x, const double *v,
+                     const double *r2, double *soa, float *m32)
+{
+    const size_t N = (size_t)n;
+    for (int j = j0; j < j1; ++j) {
+        if constexpr (POS) {
+            const double *xj = x + 3 * (size_t)j;
+            if (!(isfinite(xj[0]) && isfinite(xj[1]) && isfinite(xj[2])))
+                return {j, kPackBadState};
+            soa[0 * N + j] = xj[0];
+            soa[1 * N + j] = xj[1];
+            soa[2 * N + j] = xj[2];
+        } else {
+            const double *vj = v + 3 * (size_t)j;
+            if (!(isfinite(vj[0]) && isfinite(vj[1]) && isfinite(vj[2])))
+                return {j, kPackBadState};
+            if (!isfinite(m[j]) || m[j] < 0.0) return {j, kPackBadMass};
+            if (isnan(r2[j]) || r2[j] < 0.0) return {j, kPackBadRadius};
+            soa[3 * N + j] = vj[0];
+            soa[4 * N + j] = vj[1];
+            soa[5 * N + j] = vj[2];
+            soa[6 * N + j] = m[j];
+            soa[7 * N + j] = r2[j];
+            m32[j] = (float)m[j];
+        }
+    }
+    return {};
+}
+
+// The first failure of the two field groups of the same range, as one pass
+// over all fields would report it: the lower index, and at an equal index
+// the position (the first field checked).
+PackStatus first_of(const PackStatus &pos, const PackStatus &rest)
+{
+    if (pos.index < 0) return rest;
+    if (rest.index < 0 || pos.index <= rest.index) return pos;
+    return rest;
+}
+
+// The first failure over parts in index order.
+PackStatus first_part(const std::vector<PackStatus> &st)
+{
+    for (const PackStatus &s : st)
+        if (s.index >= 0) return s;
+    return {};
+}
+
 }  // namespace
+
+PackStatus pack_jset_staged(int n, const double *m, const double *x, const double *v,
+                            const double *r2, double *soa, float *m32,
+                            const std::function<void()> &positions_done)
+{
+    const auto serial = [&] {
+        const PackStatus a = pack_part<true>(n, 0, n, m, x, v, r2, soa, m32);
+        if (a.index < 0) positions_done();
+        return first_of(a, pack_part<false>(n, 0, n, m, x, v, r2, soa, m32));
+    };
+    if (n < kPackParallelMin) return serial();
+    const int w = pool().workers();
+    const int parts = std::min(w + 1, (n + kPackParallelMin / 2 - 1) / (kPackParallelMin / 2));
+    if (parts <= 1) return serial();
+    std::vector<PackStatus> pos(parts), rest(parts);
+    std::atomic<int> packed{0};
+    const auto job = [&](int p) {
+        const int a = (int)((long long)n * p / parts);
+        const int b = (int)((long long)n * (p + 1) / parts);
+        pos[p] = pack_part<true>(n, a, b, m, x, v, r2, soa, m32);
+        packed.fetch_add(1, std::memory_order_release);
+        if (p == 0) {
+            // only part 0 waits: the others go on to the rest, so every
+            // part it waits for is running or done
+            while (packed.load(std::memory_order_acquire) < parts) std::this_thread::yield();
+            if (first_part(pos).index < 0) positions_done();
+        }
+        rest[p] = pack_part<false>(n, a, b, m, x, v, r2, soa, m32);
+    };
+    if (!pool().run(parts, job)) return serial();
+    return first_of(first_part(pos), first_part(rest));
+}
 
 PackStatus pack_jset(int n, int j0, int j1, const double *m, const double *x, const double *v,
                      const double *r2, double *soa, float *m32)

@@ -2,6 +2,8 @@
 // Not part of the public ABI.
 #pragma once
 
+#include <functional>
+
 namespace abyss {
 
 constexpr int kPackBadState = 1;    // non-finite position or velocity
@@ -23,5 +25,15 @@ struct PackStatus {
 // soa / m32 hold partial contents.
 PackStatus pack_jset(int n, int j0, int j1, const double *m, const double *x, const double *v,
                      const double *r2, double *soa, float *m32);
+
+// The same over all n particles, positions first: once every position is
+// packed and valid, `positions_done` runs (once, on the calling thread)
+// while the other threads pack the rest (velocities, masses, radii), so a
+// caller can copy the three position arrays meanwhile.  It does not run
+// when a position is invalid.  Returns the first invalid particle as
+// pack_jset does.
+PackStatus pack_jset_staged(int n, const double *m, const double *x, const double *v,
+                            const double *r2, double *soa, float *m32,
+                            const std::function<void()> &positions_done);
 
 }  // namespace abyss

@@ -454,16 +454,6 @@ int collect_kernel_timing(abyss_ctx_t c)
     return ABYSS_OK;
 }
 
-int collect_timing(abyss_ctx_t c)
-{
-    if (!c->timing_pending) return ABYSS_OK;
-    float ms = 0.0f;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->t_grav_ms += ms;
-    c->timing_pending = false;
-    return ABYSS_OK;
-}
-
 void release_all(abyss_ctx_t c)
 {
     if (c->device >= 0) (void)hipSetDevice(c->device);
@@ -483,10 +473,7 @@ void release_all(abyss_ctx_t c)
             (void)hipEventDestroy(e);
             e = nullptr;
         }
-    if (c->ev0) (void)hipEventDestroy(c->ev0);
-    if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
-    c->ev0 = c->ev1 = nullptr;
     c->stream = nullptr;
 }
 
@@ -551,9 +538,6 @@ int ctx_create_on_stream(int device, int cap, hipStream_t stream, abyss_ctx_t *o
         }
         if (hipDeviceGetAttribute(&c->cu_count, hipDeviceAttributeMultiprocessorCount, device) !=
             hipSuccess) { rc = ABYSS_ERR_HIP; break; }
-        if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-            rc = ABYSS_ERR_HIP; break;
-        }
     } while (0);
     if (rc != ABYSS_OK) {
         release_all(c);
@@ -707,16 +691,23 @@ int abyss_ctx_upload(abyss_ctx_t c, int n, const double *m, const double *x, con
         HIPCHK(c, hipHostMalloc(&c->h_j, cap_b, hipHostMallocDefault));
         c->j_bytes = cap_b;
     }
-    // Validation and AoS -> SoA pack in one pass over the caller's arrays,
-    // on a few host threads for large N, before anything is copied
+    // Validation and AoS -> SoA pack of the caller's arrays, on a few host
+    // threads for large N, before anything of them is used
     // (src/cuda/cuda_my_acceleration.cu:399-405 packs the same arrays with
-    // stride N).  Packing and copying in chunks, so that the copy of one
-    // chunk overlaps the pack of the next, was slower: the strided chunk
-    // copies run as blit kernels (192 vs 168 us for a 65536-particle
-    // upload + query).
+    // stride N).  The positions are packed first; their three arrays are
+    // copied while the other threads pack the rest, then the rest is copied
+    // (both copies contiguous).  Packing and copying in particle chunks was
+    // slower: the strided chunk copies run as blit kernels (192 vs 168 us
+    // for a 65536-particle upload + query).  After a failure an enqueued
+    // position copy is harmless: the j-set is dropped, and the next upload
+    // waits for the stream before it writes the staging buffer.
     double *h = reinterpret_cast<double *>(c->h_j);
-    const abyss::PackStatus st =
-        abyss::pack_jset(n, 0, n, m, x, v, r2, h, reinterpret_cast<float *>(h + 8 * (size_t)n));
+    float *h32 = reinterpret_cast<float *>(h + 8 * (size_t)n);
+    const size_t pos_bytes = 3 * (size_t)n * sizeof(double);
+    hipError_t pos_copy = hipSuccess;
+    const abyss::PackStatus st = abyss::pack_jset_staged(n, m, x, v, r2, h, h32, [&] {
+        pos_copy = hipMemcpyAsync(c->d_j, c->h_j, pos_bytes, hipMemcpyHostToDevice, c->stream);
+    });
     if (st.index >= 0) {
         const int j = st.index;
         if (st.kind == abyss::kPackBadState)
@@ -727,7 +718,9 @@ int abyss_ctx_upload(abyss_ctx_t c, int n, const double *m, const double *x, con
                               j);
         return set_detail(c, ABYSS_ERR_INVALID_INPUT, "upload: bad r_nb^2 %g at j=%d", r2[j], j);
     }
-    HIPCHK(c, hipMemcpyAsync(c->d_j, c->h_j, need, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, pos_copy);
+    HIPCHK(c, hipMemcpyAsync(c->d_j + pos_bytes, c->h_j + pos_bytes, need - pos_bytes,
+                             hipMemcpyHostToDevice, c->stream));
     bind_jstate(c, c->d_j, n);
     c->jset_version += 1;
     c->n = n;
@@ -855,10 +848,7 @@ int ctx_launch_force(abyss_ctx_t c)
         fin.spread = c->chk_spread;
         part.spread = c->chk_pspread;
     }
-    // GPU time for ProfileDevice: only the legacy context records events
-    // (two event records and their readback cost a small query ~7 us)
     c->mirror_valid = false;
-    if (c->timed) HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (c->ktiming) {
         if (c->kev_used == abyss_ctx_s::kTimingRing) {
             int rc = collect_kernel_timing(c);
@@ -901,8 +891,6 @@ int ctx_launch_force(abyss_ctx_t c)
         if (rc != ABYSS_OK) return rc;
         c->spread_valid = true;
     }
-    if (c->timed) HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    c->timing_pending = c->timed;
     c->n_grav += 1;
     c->counts_valid = true;
     c->sum_ni += c->nt;
@@ -928,9 +916,7 @@ int abyss_ctx_sync(abyss_ctx_t c)
     if (!c) return ABYSS_ERR_INVALID_ARG;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    int rc = collect_kernel_timing(c);
-    if (rc != ABYSS_OK) return rc;
-    return collect_timing(c);
+    return collect_kernel_timing(c);
 }
 
 int abyss_ctx_set_kernel_timing(abyss_ctx_t c, int enable)
@@ -986,10 +972,6 @@ int ctx_fetch_on_stream(abyss_ctx_t c, hipStream_t st, double *acc, double *adot
                                           : (size_t)nt * (6 * sizeof(double) + sizeof(int)),
                                  hipMemcpyDeviceToHost, st));
         HIPCHK(c, hipStreamSynchronize(st));
-    }
-    if (st == c->stream) {
-        int rc = collect_timing(c);
-        if (rc != ABYSS_OK) return rc;
     }
     // populated width and overflow count, from the counts on the host
     int width = 0, overflowed = 0;

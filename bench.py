@@ -697,8 +697,10 @@ def bench_blocks(args, sizes=(1, 4, 16, 64, 256, 1024, 4096, 16384, 65536)):
     ev = HipEvents()
     rng = np.random.default_rng(7)
     rows = []
+    blocks = []
     for m in sizes:
         t = np.sort(rng.permutation(n)[:m]).astype(np.int32)
+        blocks.append(t)
         inter = float(m) * (n - 1)
         reps = int(min(2000, max(10, 0.2 / max(inter / 1.2e12, 1e-5))))
         ctx.set_targets(t)
@@ -747,8 +749,45 @@ def bench_blocks(args, sizes=(1, 4, 16, 64, 256, 1024, 4096, 16384, 65536)):
     regf.CloseDevice()
     regf.set_num_neighbor_max(1000)
     regf.set_precision(regf.PREC_DEFAULT)
+    from_c = legacy_from_c(js, blocks, prec)
+    for row, c in zip(rows, from_c or []):
+        row["legacy_from_c_us"] = c
+    if from_c:
+        log("legacy upload+call from C: " +
+            ", ".join(f"m={r['m']} {r['legacy_from_c_us']['total_us']:.1f} us" for r in rows))
     return {"workload": "plummer_kroupa_n65536_cap256", "precision": args.precision,
             "rows": rows}
+
+
+def legacy_from_c(js, blocks, prec):
+    """The legacy calling pattern (SendToDevice of the whole j-set, then
+    CalculateAccelerationOnDevice of the block) timed from C, as ABYSS calls
+    it, by tests/harness/legacy_timing on the same j-set and blocks: per
+    block {send_us, call_us, total_us} (medians), or None when the harness
+    is not built."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    exe = os.path.join(ROOT, "tests", "harness", "legacy_timing")
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "workload.bin")
+        with open(path, "wb") as f:
+            np.array([js.n], np.int32).tofile(f)
+            for a in (js.m, js.x, js.v, js.r2):
+                np.ascontiguousarray(a, np.float64).tofile(f)
+            np.array([len(blocks)], np.int32).tofile(f)
+            for t in blocks:
+                np.array([len(t)], np.int32).tofile(f)
+                np.ascontiguousarray(t, np.int32).tofile(f)
+        p = subprocess.run([exe, path, str(js.cap), str(prec), "100"], capture_output=True,
+                           text=True, timeout=120)
+    if p.returncode != 0:
+        log(f"legacy timing harness failed ({p.returncode}): {p.stderr[-300:]}")
+        return None
+    out = [json.loads(ln) for ln in p.stdout.splitlines() if ln.startswith("{")]
+    return out if len(out) == len(blocks) else None
 
 
 def bench_neighbour(args, reps=5):

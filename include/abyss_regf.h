@@ -120,7 +120,9 @@ void OpenDevice(const int *irank);
 void CloseDevice(void);
 
 /* Prints and resets the call/interaction/time counters, with the 60-flop
- * interaction convention (src_mpi/cuda/cuda_my_acceleration.cu:1051-1062). */
+ * interaction convention (src_mpi/cuda/cuda_my_acceleration.cu:1051-1062).
+ * grav(s) is the wall-clock time spent in CalculateAccelerationOnDevice, as
+ * the reference's host timers measured it. */
 void ProfileDevice(int *irank);
 
 /* Replaces the whole j-set: N = *NNB particles, masses m[N], positions
@@ -226,8 +228,8 @@ const char *abyss_ctx_error_detail(abyss_ctx_t ctx);
  * the j-splits, nor checked precision's flagging and recomputation);
  * kernel_timing returns the summed milliseconds and the number of launches
  * since it was last switched (it waits for the last recorded launch).
- * Counterpart of the GPU time ProfileDevice reports
- * (src_mpi/cuda/cuda_my_acceleration.cu:1051-1062), per kernel. */
+ * The GPU-time, per-kernel form of the timing ProfileDevice reports
+ * (src_mpi/cuda/cuda_my_acceleration.cu:1051-1062). */
 int abyss_ctx_set_kernel_timing(abyss_ctx_t ctx, int enable);
 int abyss_ctx_kernel_timing(abyss_ctx_t ctx, double *force_ms, long long *launches);
 

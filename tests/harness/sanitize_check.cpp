@@ -90,11 +90,49 @@ static void check_pack(int n, unsigned seed)
         return abyss::pack_jset(n, 0, n, s.m.data(), s.x.data(), s.v.data(), s.r2.data(),
                                 soa.data(), m32.data());
     };
+    // the staged form: the callback sees every position packed (it reads
+    // them, as the upload's copy does, while the rest is being packed)
+    int calls = 0;
+    auto staged = [&] {
+        calls = 0;
+        return abyss::pack_jset_staged(n, s.m.data(), s.x.data(), s.v.data(), s.r2.data(),
+                                       soa.data(), m32.data(), [&] {
+                                           ++calls;
+                                           for (int j = 0; j < n; ++j)
+                                               for (int d = 0; d < 3; ++d)
+                                                   CHECK(soa[(size_t)d * n + j] ==
+                                                         s.x[3 * j + d]);
+                                       });
+    };
     for (int k = 0; k < 3; ++k) {
         spoil(k, true);
         st = pack();
         CHECK(st.index == bad[k] && st.kind == kinds[k]);
+        const abyss::PackStatus st2 = staged();
+        CHECK(st2.index == st.index && st2.kind == st.kind && calls == 1);
         spoil(k, false);
+    }
+    // a bad position: no callback, and the lowest index over both groups
+    s.x[3 * (n / 4) + 1] = INFINITY;
+    s.m[n / 2] = -1.0;
+    st = staged();
+    CHECK(st.index == n / 4 && st.kind == abyss::kPackBadState && calls == 0);
+    s.m[n / 4] = -1.0;   // the same index: the position is reported
+    st = staged();
+    CHECK(st.index == n / 4 && st.kind == abyss::kPackBadState && calls == 0);
+    s.x[3 * (n / 4) + 1] = 0.0;
+    st = staged();
+    CHECK(st.index == n / 4 && st.kind == abyss::kPackBadMass && calls == 1);
+    s.m[n / 4] = 1.0;
+    s.m[n / 2] = 1.0;
+    {
+        CHECK(pack().index == -1);
+        const std::vector<double> one = soa;
+        const std::vector<float> one32 = m32;
+        std::fill(soa.begin(), soa.end(), -1.0);
+        std::fill(m32.begin(), m32.end(), -1.0f);
+        CHECK(staged().index == -1 && calls == 1);
+        CHECK(soa == one && m32 == one32);   // the contents of one pass
     }
     for (int k = 0; k < 3; ++k) spoil(k, true);
     st = pack();
