@@ -759,6 +759,38 @@ def bench_blocks(args, sizes=(1, 4, 16, 64, 256, 1024, 4096, 16384, 65536)):
             "rows": rows}
 
 
+def config1_hermite(t_end=1.0, eta=0.01):
+    """The plain-method workload's dynamical check: the equal-mass Plummer
+    sphere N = 1024 (seed 1, ~48 neighbours) integrated for one N-body time
+    by tests/harness/ac_hermite_gpu, a block-timestep Hermite integrator
+    whose regular force and neighbour lists come from the six legacy entry
+    points (checked precision), as ABYSS's caller drives them.  Returns its
+    JSON line (energy error dE/E, steps, wall time) or None when the harness
+    is not built."""
+    import struct
+    import subprocess
+    import tempfile
+    from abyss_amd import workloads
+    exe = os.path.join(ROOT, "tests", "harness", "ac_hermite_gpu")
+    if not os.path.exists(exe):
+        return None
+    js = workloads.plummer_equal(1024)
+    with tempfile.TemporaryDirectory() as td:
+        ic = os.path.join(td, "ic.bin")
+        with open(ic, "wb") as f:
+            f.write(struct.pack("<i", js.n))
+            for a in (js.x, js.v, js.m, js.r2):
+                f.write(a.astype("<f8").tobytes())
+        p = subprocess.run([exe, "gpu", ic, str(t_end), str(eta)], capture_output=True,
+                           text=True, timeout=300)
+    if p.returncode != 0:
+        log(f"Hermite harness failed ({p.returncode}): {p.stderr[-300:]}")
+        return None
+    out = json.loads(p.stdout.strip().splitlines()[-1])
+    out["workload"] = "plummer_equal_n1024_seed1, 1 N-body time"
+    return out
+
+
 def legacy_from_c(js, blocks, prec):
     """The legacy calling pattern (SendToDevice of the whole j-set, then
     CalculateAccelerationOnDevice of the block) timed from C, as ABYSS calls
@@ -1058,6 +1090,11 @@ def main():
     blocks = None
     if not args.no_blocks:
         blocks = bench_blocks(args, sizes=(1, 64, 4096, 65536))
+    hermite = None
+    if not args.no_check:
+        # the dynamical check of the plain-method workload (an accuracy leg)
+        hermite = config1_hermite()
+        log(f"Hermite run, Plummer N = 1024: {hermite}")
     ens_info = serve_info = None
     if not args.no_ensemble:
         # the ensemble legs mix every block size: they plan from a table
@@ -1114,6 +1151,7 @@ def main():
         "accuracy": acc_info,
         "other_precisions": legs or None,
         "active_blocks": blocks,
+        "config1_hermite": hermite,
         "batched_ensemble": ens_info,
         "served_ensemble": serve_info,
     }

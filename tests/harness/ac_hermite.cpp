@@ -23,6 +23,8 @@
 //
 // usage: ac_hermite <gpu|resident|oracle> <ic.bin> <t_end> <eta>
 //   ic.bin: int32 n, then x[n][3], v[n][3], m[n], r2[n] as float64
+// Built with -DAC_NO_ORACLE (tests/harness/ac_hermite_gpu, which bench.py
+// runs) it has no oracle provider and does not link the oracle library.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -34,10 +36,12 @@
 
 #include "abyss_regf.h"
 
+#ifndef AC_NO_ORACLE
 extern "C" int oracle_regf(int n, const double *x, const double *v, const double *m,
                            const double *rnb2, int nt, const int *targets, double *acc,
                            double *adot, int *nnb, int *list, int cap, double *scale_a,
                            double *scale_j, int nthreads);
+#endif
 
 namespace {
 
@@ -89,8 +93,13 @@ void total_force(const System &s, double tn, const std::vector<double> &xp,
             exit(2);
         }
     } else {
+#ifdef AC_NO_ORACLE
+        fprintf(stderr, "built without the oracle provider\n");
+        exit(2);
+#else
         oracle_regf(s.n, xp.data(), vp.data(), s.m.data(), s.r2.data(), nt, tg.data(), ar.data(),
                     jr.data(), nnb.data(), list.data(), cap, nullptr, nullptr, 0);
+#endif
     }
     reg_calls += 1;
     reg_targets += nt;
