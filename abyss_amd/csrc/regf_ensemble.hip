@@ -673,7 +673,11 @@ int abyss_ens_launch(abyss_ens_t e)
             const int tile = abyss::kTile;
             const int nround = ((c->n + tile - 1) / tile) * tile;
             int jspan = (int)std::ceil(kItemPairs / std::min(tpb, m) / tile) * tile;
-            if (jl >= 16) jspan = std::min(jspan, kItemTilesSmall * tile);
+            // every lane group: the recomputation is a few percent of the
+            // batch's pairs, so its items must be short to spread over the
+            // chip (a 4-lane item of kItemPairs FP64-term pairs took ~5 ms,
+            // the whole recomputation's length)
+            jspan = std::min(jspan, kItemTilesSmall * tile);
             jspan = std::max(tile, std::min(jspan, nround));
             int S = (c->n + jspan - 1) / jspan;
             if (S > abyss::kMaxSplitsBatch) {
@@ -694,6 +698,32 @@ int abyss_ens_launch(abyss_ens_t e)
             bump = e->d_pool;
             size_t it2 = 0;
             int total = 0;
+            // items longest first (as the main pass), so short ones fill the
+            // tail; an item's slot of the partials does not depend on when
+            // it runs
+            struct Item2 {
+                double cost;
+                int r, b, sp;
+            };
+            std::vector<Item2> order;
+            order.reserve(nitems2);
+            for (size_t r = 0; r < rs.size(); ++r) {
+                const Rec &x = rs[r];
+                const int tpb = 4 * (64 / x.jl);
+                abyss_ctx_t c = e->cl[act[x.a].cluster];
+                for (int b = 0; b < x.nb; ++b)
+                    for (int sp = 0; sp < x.splits; ++sp) {
+                        const int tb = std::min(tpb, x.m - b * tpb);
+                        const int js = std::min(x.jspan, c->n - sp * x.jspan);
+                        order.push_back({(double)tb * js, (int)r, b, sp});
+                    }
+            }
+            std::sort(order.begin(), order.end(), [](const Item2 &p, const Item2 &q) {
+                if (p.cost != q.cost) return p.cost > q.cost;
+                if (p.r != q.r) return p.r < q.r;
+                return p.b != q.b ? p.b < q.b : p.sp < q.sp;
+            });
+            for (const Item2 &o : order) e->h_items[it2++] = WorkItem{o.r, o.b, o.sp, 0};
             for (size_t r = 0; r < rs.size(); ++r) {
                 const Rec &x = rs[r];
                 abyss_ctx_t c = e->cl[act[x.a].cluster];
@@ -716,9 +746,6 @@ int abyss_ens_launch(abyss_ens_t e)
                 d.fq = c->chk_fq;
                 e->h_ck_off[r] = total;
                 total += x.m;
-                for (int b = 0; b < x.nb; ++b)
-                    for (int sp = 0; sp < x.splits; ++sp)
-                        e->h_items[it2++] = WorkItem{(int)r, b, sp, 0};
             }
             e->h_ck_off[rs.size()] = total;
             e->h_hdr->nitems = (int)nitems2;
