@@ -165,3 +165,42 @@ def test_legacy_error_mode_setter_validates():
     lib = regf.load()
     assert lib.abyss_set_legacy_errors(7) == regf.ERR_INVALID_ARG
     assert lib.abyss_set_legacy_errors(regf.LEGACY_RETURN) == regf.OK
+
+
+def _table(path, rows):
+    with open(path, "w") as f:
+        f.write("# bucket jl unroll wg_per_cu [tile waves]\n")
+        for r in rows:
+            f.write(" ".join(str(v) for v in r) + "\n")
+
+
+def test_plan_table_files_load_and_validate(tmp_path):
+    """abyss_autotune_load parses on the host: six-column tables (lanes per
+    target, j-unroll, workgroups per CU, j-tile, waves per workgroup) and
+    four-column ones written before the shape columns load; a table with an
+    unsupported shape, lanes count or a missing bucket is refused."""
+    lib = regf.load()
+    shapes = [(256, 4), (128, 4), (64, 4), (128, 2), (64, 2), (64, 1), (1024, 4)]
+    good6 = str(tmp_path / "six.txt")
+    _table(good6, [(b, (4, 16, 64)[b % 3], 2, 16, *shapes[b % len(shapes)]) for b in range(32)])
+    good4 = str(tmp_path / "four.txt")
+    _table(good4, [(b, 4, 2, 48) for b in range(32)])
+    try:
+        assert lib.abyss_autotune_load(0, good6.encode()) == regf.OK
+        assert lib.abyss_autotune_load(0, good4.encode()) == regf.OK
+        for bad_row, name in (((5, 4, 2, 16, 512, 4), "tile 512"),
+                              ((5, 4, 2, 16, 256, 2), "tile above 64 x waves"),
+                              ((5, 4, 2, 16, 1024, 2), "one-shot tile with 2 waves"),
+                              ((5, 8, 2, 16, 256, 4), "8 lanes per target"),
+                              ((5, 4, 3, 16, 256, 4), "unroll 3")):
+            path = str(tmp_path / "bad.txt")
+            _table(path, [bad_row if b == 5 else (b, 4, 2, 16, 256, 4) for b in range(32)])
+            assert lib.abyss_autotune_load(0, path.encode()) == regf.ERR_INVALID_INPUT, name
+        short = str(tmp_path / "short.txt")
+        _table(short, [(b, 4, 2, 16, 256, 4) for b in range(31)])
+        assert lib.abyss_autotune_load(0, short.encode()) == regf.ERR_INVALID_INPUT
+        assert lib.abyss_autotune_load(0, str(tmp_path / "missing.txt").encode()) == \
+            regf.ERR_INVALID_ARG
+        assert lib.abyss_autotune_load(-1, good6.encode()) == regf.ERR_INVALID_ARG
+    finally:
+        assert lib.abyss_autotune_clear(0) == regf.OK
