@@ -463,7 +463,11 @@ __device__ __forceinline__ void jl_tile(JLLane &L, const double2 *__restrict__ p
 // tests is.  Inside the rare branch each step is tested and listed in step
 // order, so lists stay ascending.  Every pair is evaluated with the same
 // instructions and summed in the same order as in jl_tile, so the results
-// are bitwise those of jl_tile.
+// are bitwise those of jl_tile.  Between the test and the terms a step keeps
+// only its FP32 differences (x velocity's taken with the positions) and
+// rsqrt; the rare branch recomputes r^2 from the same differences with the
+// same expression, hence the same bits.  That keeps the group's state at 5
+// registers per step.
 template <int JL, int G, int TILE>
 __device__ __forceinline__ void jl_tile_common(JLLane &L, const double2 *__restrict__ pos,
                                                const float *__restrict__ mass, int jt)
@@ -472,8 +476,8 @@ __device__ __forceinline__ void jl_tile_common(JLLane &L, const double2 *__restr
     static_assert(steps % G == 0, "the group divides the tile");
 #pragma unroll 1
     for (int st = 0; st < steps; st += G) {
-        float dx[G], dy[G], dz[G], r2[G], rinv[G];
-        double vxj[G];
+        float dx[G], dy[G], dz[G], rinv[G], dvxg[G];
+        float mn = 0.0f;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const int r = (st + g) * JL + L.sub;
@@ -482,17 +486,16 @@ __device__ __forceinline__ void jl_tile_common(JLLane &L, const double2 *__restr
             dx[g] = (float)(p0.x - L.xi);
             dy[g] = (float)(p0.y - L.yi);
             dz[g] = (float)(p1.x - L.zi);
-            vxj[g] = p1.y;
-            r2[g] = fmaf(dz[g], dz[g], fmaf(dy[g], dy[g], dx[g] * dx[g]));
-            rinv[g] = __builtin_amdgcn_rsqf(r2[g]);
+            dvxg[g] = (float)(p1.y - L.vxi);
+            const float r2 = fmaf(dz[g], dz[g], fmaf(dy[g], dy[g], dx[g] * dx[g]));
+            rinv[g] = __builtin_amdgcn_rsqf(r2);
+            mn = g == 0 ? r2 : fminf(mn, r2);
         }
-        float mn = r2[0];
-#pragma unroll
-        for (int g = 1; g < G; ++g) mn = fminf(mn, r2[g]);
         if (__any(!(mn > L.rnbf))) {
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                const bool near = !(r2[g] > L.rnbf);
+                const float r2g = fmaf(dz[g], dz[g], fmaf(dy[g], dy[g], dx[g] * dx[g]));
+                const bool near = !(r2g > L.rnbf);
                 const unsigned long long nb = __ballot(near);
                 if (nb) {
                     // The lane offset passes through an empty asm so that the
@@ -515,7 +518,7 @@ __device__ __forceinline__ void jl_tile_common(JLLane &L, const double2 *__restr
             const int r = (st + g) * JL + L.sub;
             const double2 p2 = pos[3 * r + 2];
             const float mj = mass[r];
-            const float dvx = (float)(vxj[g] - L.vxi);
+            const float dvx = dvxg[g];
             const float dvy = (float)(p2.x - L.vyi);
             const float dvz = (float)(p2.y - L.vzi);
             const float rinv2 = rinv[g] * rinv[g];
