@@ -95,9 +95,12 @@ struct abyss_ctx_s {
     int *chk_counters = nullptr, *chk_fq = nullptr, *chk_fi = nullptr;
     char *d_rchk = nullptr;
     size_t rchk_bytes = 0;
-    int *h_chk = nullptr;       // pinned: [0] flagged-count readback
+    int *h_chk = nullptr;       // pinned, mapped: [0] flagged-count readback,
+                                // [1] the count a device-read recomputation reports
+    int *h_chk_dev = nullptr;   // device address of h_chk
     double check_tol = 1.0e-6;
     int n_rechecked = 0;        // targets recomputed by the last launch
+    bool recheck_pending = false;   // n_rechecked is h_chk[1] once the stream drains
     bool spread_valid = false;  // chk_spread holds the last launch's spread
 
     abyss::Plan plan{};

@@ -66,6 +66,11 @@ struct Plan {
 // run in FP64 terms in one pass: below ~1000 targets that is faster than
 // the FP32 pass, its flag and its recomputation together.
 constexpr int kCheckedDirectMax = 256;
+// Checked-precision blocks above that and up to this many targets (outside
+// batches) read the flagged count on the device: the recomputation is a
+// grid-stride loop over the flagged targets' work items, queued right behind
+// the flag, instead of a host wait for the count and a launch planned for it.
+constexpr int kRecheckOnDeviceMax = 4096;
 
 // Plan choice measured by the autotuner for one target-count bucket.
 struct TuneEntry {
@@ -218,5 +223,14 @@ hipError_t launch_recheck_dev(const JState &js, const int *fi, const int *count,
                               const Plan &p, const QueryOut &out, const int *fq, double *acc,
                               double *adot, const FinalMirror &mirror, int *m_report,
                               hipStream_t st);
+
+// The recomputation of the first *count targets of fi (count on the device,
+// at most nt): `grid` workgroups loop over the (target block, split) items of
+// plan p (lanes per target, splits and span; 256-record tiles, 4 waves), then
+// the merge.  Partials: rows s * nt + k of out.  Writes the count to
+// *m_report (mapped host memory) when it is set.
+hipError_t launch_recheck_loop(const JState &js, const int *fi, const int *count, int nt,
+                               const Plan &p, int grid, const QueryOut &out, const int *fq,
+                               double *acc, double *adot, int *m_report, hipStream_t st);
 
 }  // namespace abyss

@@ -932,6 +932,27 @@ void recheck_kernel_dev(JState js, const int *__restrict__ tgt, const int *__res
     regf_body_jl<JL, 3>(js, tgt, m, jspan, out, 0, 0, bx, s, 0, nt_stride, s_pos, s_m);
 }
 
+// The recomputation with the flagged count read on the device, as a
+// grid-stride loop: workgroup b takes items b, b + gridDim.x, ... of the
+// (target block, split) items of the first m = *count targets, split-minor.
+// The trip count is the same for every thread of a workgroup, so the
+// barriers inside each item are met by all of them; the last barrier of an
+// item precedes the next item's first LDS store.
+template <int JL>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5, 8)))
+void recheck_kernel_loop(JState js, const int *__restrict__ tgt, const int *__restrict__ count,
+                         int nt_stride, int jspan, QueryOut out, int splits)
+{
+    __shared__ double2 s_pos[2][kTile * 3];
+    __shared__ double s_m[2][kTile];
+    const int m = min(*count, nt_stride);
+    constexpr int TPB = 4 * (64 / JL);   // targets per workgroup
+    const int items = (m + TPB - 1) / TPB * splits;
+    for (int it = blockIdx.x; it < items; it += gridDim.x)
+        regf_body_jl<JL, 3>(js, tgt, m, jspan, out, 0, 0, it / splits, it % splits, 0, nt_stride,
+                            s_pos, s_m);
+}
+
 // One wave per flagged target k < m (m = *m_dev when m_dev is set; then
 // thread 0 also writes m to *m_report, host memory the caller reads after
 // the stream has drained).  The mirror, when set, receives the same values.
@@ -945,7 +966,7 @@ __global__ __launch_bounds__(256) void recheck_finalize_kernel(QueryOut part, in
                                                                FinalMirror mir)
 {
     const int lane = threadIdx.x & 63;
-    const int m = m_dev ? *m_dev : m_host;
+    const int m = min(m_dev ? *m_dev : m_host, nt_stride);
     if (m_report && blockIdx.x == 0 && threadIdx.x == 0) *m_report = m;
     const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (k >= m) return;   // the whole wave
@@ -1343,6 +1364,29 @@ hipError_t launch_recheck_dev(const JState &js, const int *fi, const int *count,
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(recheck_finalize_kernel, dim3((nt + 3) / 4), dim3(256), 0, st, out,
                        p.splits, nt, 0, count, m_report, fq, acc, adot, mirror);
+    return hipGetLastError();
+}
+
+hipError_t launch_recheck_loop(const JState &js, const int *fi, const int *count, int nt,
+                               const Plan &p, int grid, const QueryOut &out, const int *fq,
+                               double *acc, double *adot, int *m_report, hipStream_t st)
+{
+    if ((p.jl != 4 && p.jl != 16 && p.jl != 64) || p.tile != kTile || p.wpb != 4 || grid <= 0 ||
+        nt <= 0 || p.splits <= 0 || (long long)nt * p.splits > 0x7fffffffLL)
+        return hipErrorInvalidValue;
+    if (p.jl == 4)
+        hipLaunchKernelGGL(recheck_kernel_loop<4>, dim3(grid), dim3(kBlock), 0, st, js, fi, count,
+                           nt, p.jspan, out, p.splits);
+    else if (p.jl == 16)
+        hipLaunchKernelGGL(recheck_kernel_loop<16>, dim3(grid), dim3(kBlock), 0, st, js, fi,
+                           count, nt, p.jspan, out, p.splits);
+    else
+        hipLaunchKernelGGL(recheck_kernel_loop<64>, dim3(grid), dim3(kBlock), 0, st, js, fi,
+                           count, nt, p.jspan, out, p.splits);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(recheck_finalize_kernel, dim3((nt + 3) / 4), dim3(256), 0, st, out,
+                       p.splits, nt, 0, count, m_report, fq, acc, adot, FinalMirror{});
     return hipGetLastError();
 }
 

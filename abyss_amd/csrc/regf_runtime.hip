@@ -355,7 +355,12 @@ int ensure_check(abyss_ctx_t c, int nt, int S)
         HIPCHK(c, hipMalloc(&c->d_chk, grow(need)));
         c->chk_bytes = grow(need);
     }
-    if (!c->h_chk) HIPCHK(c, hipHostMalloc(&c->h_chk, 256, hipHostMallocDefault));
+    if (!c->h_chk) {
+        HIPCHK(c, hipHostMalloc(&c->h_chk, 256, hipHostMallocMapped | hipHostMallocCoherent));
+        void *dev = nullptr;
+        HIPCHK(c, hipHostGetDevicePointer(&dev, c->h_chk, 0));
+        c->h_chk_dev = static_cast<int *>(dev);
+    }
     char *b = c->d_chk;
     c->chk_spread = reinterpret_cast<float *>(b);
     b += b_sp;
@@ -432,6 +437,23 @@ int recheck(abyss_ctx_t c, bool flagged)
         HIPCHK(c, abyss::launch_check_flag(c->d_acc, c->d_adot, c->chk_spread, c->tgt, c->nt,
                                            check_k2(c), c->chk_counters, c->chk_fq, c->chk_fi,
                                            c->stream));
+    }
+    if (c->nt <= abyss::kRecheckOnDeviceMax && !c->batch_member) {
+        // The count stays on the device: a plan for a typical flagged count
+        // (1/16 of the block), a grid-stride loop over the actual one.
+        const Plan rp = plan_for(c, std::max(1, c->nt / 16), c->n, ABYSS_PREC_REFINED, 0, false);
+        const int tpb = 4 * (64 / rp.jl);
+        const int cu = c->cu_count > 0 ? c->cu_count : 256;
+        const long long items = (long long)((c->nt + tpb - 1) / tpb) * rp.splits;
+        const int grid = (int)std::min<long long>(items, 5LL * cu);
+        QueryOut rpart;
+        int rc = ensure_recheck_scratch(c, (size_t)rp.splits * c->nt, &rpart);
+        if (rc != ABYSS_OK) return rc;
+        HIPCHK(c, abyss::launch_recheck_loop(c->js, c->chk_fi, c->chk_counters, c->nt, rp, grid,
+                                             rpart, c->chk_fq, c->d_acc, c->d_adot,
+                                             c->h_chk_dev + 1, c->stream));
+        c->recheck_pending = true;
+        return ABYSS_OK;
     }
     HIPCHK(c, hipMemcpyAsync(c->h_chk, c->chk_counters, sizeof(int), hipMemcpyDeviceToHost,
                              c->stream));
@@ -641,6 +663,13 @@ int abyss_ctx_set_check_tolerance(abyss_ctx_t c, double tol)
 int abyss_ctx_last_recheck(abyss_ctx_t c, int *count)
 {
     if (!c || !count) return ABYSS_ERR_INVALID_ARG;
+    if (c->recheck_pending) {
+        // the recomputation read its count on the device and reported it
+        HIPCHK(c, hipSetDevice(c->device));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->n_rechecked = c->h_chk[1];
+        c->recheck_pending = false;
+    }
     *count = c->n_rechecked;
     return ABYSS_OK;
 }
@@ -838,6 +867,7 @@ int ctx_launch_force(abyss_ctx_t c)
     const bool checked = c->precision == ABYSS_PREC_CHECKED && !direct;
     c->spread_valid = false;
     c->n_rechecked = direct ? c->nt : 0;
+    c->recheck_pending = false;
     if (checked) {
         int rc = ensure_check(c, c->nt, p.splits);
         if (rc != ABYSS_OK) return rc;

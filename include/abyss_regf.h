@@ -239,15 +239,18 @@ int abyss_ctx_kernel_timing(abyss_ctx_t ctx, double *force_ms, long long *launch
  * terms dominate both) and recomputes with FP64 terms every target with
  *   ABYSS_CHECK_ERR_SCALE * sqrt(Sa) > tol * |a|  or
  *   ABYSS_CHECK_ERR_SCALE * sqrt(Sj) > tol * |adot|,
- * before the results are fetched; such a launch waits once on the host (for
- * the flagged count, to plan the recomputation).  A block of at most 256
- * targets (outside batched ensembles) instead evaluates every target with
- * FP64 terms in one pass, which is faster there and waits for nothing; its
- * lists are those of the FP32 pass as well.  No counterpart in the
- * reference (whose FP32 build has no accuracy control).
+ * before the results are fetched.  A block of at most 256 targets (outside
+ * batched ensembles) instead evaluates every target with FP64 terms in one
+ * pass, which is faster there and waits for nothing; its lists are those of
+ * the FP32 pass as well.  A block of 257 to 4096 targets reads the flagged
+ * count on the device (the recomputation loops over it), so its launch does
+ * not wait either.  A larger launch waits once on the host (for the flagged
+ * count, to plan the recomputation).  No counterpart in the reference (whose
+ * FP32 build has no accuracy control).
  *   set_check_tolerance: tol (> 0; default 1e-6);
  *   last_recheck: targets recomputed by the last launch (all of a block of
- *     at most 256);
+ *     at most 256); waits for the launch when its count was read on the
+ *     device;
  *   fetch_spread: [nt][2] = sqrt(Sa), sqrt(Sj) per target of the last
  *     launch (a launch of more than 256 targets). */
 #define ABYSS_CHECK_ERR_SCALE 1.0e-6

@@ -301,6 +301,32 @@ def test_checked_precision_small_block_in_fp64_terms(ctx, size):
     assert np.array_equal(nnb, chk[2])
 
 
+def test_checked_count_read_on_device(ctx):
+    """Checked blocks of 257..4096 targets read the flagged count on the
+    device: the launch queues the recomputation without a host wait, and
+    last_recheck waits for the reported count.  Two launches in a row with
+    no sync, then the count and the results equal a synchronous query's."""
+    say("case: checked count read on the device")
+    js = workloads.plummer_kroupa(65536)
+    ctx.upload(js.m, js.x, js.v, js.r2)
+    t = np.random.default_rng(41).permutation(js.n)[:2048].astype(np.int32)
+    ctx.set_precision(regf.PREC_CHECKED)
+    try:
+        ref = ctx.query(t)
+        nref = ctx.last_recheck()
+        ctx.set_targets(t)
+        ctx.launch()
+        ctx.launch()
+        n2 = ctx.last_recheck()
+        out = ctx.fetch()
+    finally:
+        ctx.set_precision(regf.PREC_SPLIT)
+    say(f"  recomputed {nref} / {n2} of {len(t)}")
+    assert 0 < nref < len(t) and n2 == nref
+    for a, b in zip(out, ref):
+        assert np.array_equal(a, b)
+
+
 def test_checked_precision_tolerance_and_legacy(ctx):
     """A tighter tolerance recomputes more targets; the legacy entry points
     give the context's checked results bit for bit."""
