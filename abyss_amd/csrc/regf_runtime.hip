@@ -38,6 +38,9 @@ constexpr int kMappedTargets = 64;
 // a checked block with a mapped mirror runs in FP64 terms in one pass, so no
 // recomputation ever has to update the mirror
 static_assert(abyss::kCheckedDirectMax >= kMappedTargets, "mirrored blocks are direct");
+// Most j-splits of the recomputation that reads its count on the device
+// (scratch: this many rows per target of the block, 52 bytes each).
+constexpr int kRecheckLoopSplits = 256;
 
 int set_detail(abyss_ctx_t c, int code, const char *fmt, ...)
 {
@@ -439,9 +442,16 @@ int recheck(abyss_ctx_t c, bool flagged)
                                            c->stream));
     }
     if (c->nt <= abyss::kRecheckOnDeviceMax && !c->batch_member) {
-        // The count stays on the device: a plan for a typical flagged count
-        // (1/16 of the block), a grid-stride loop over the actual one.
-        const Plan rp = plan_for(c, std::max(1, c->nt / 16), c->n, ABYSS_PREC_REFINED, 0, false);
+        // The count stays on the device: lanes per target planned for a
+        // typical flagged count (1/16 of the block), a grid-stride loop over
+        // the actual one.  Its items are short (j-ranges of 1 tile where N
+        // allows, up to 256 splits), since a few flagged targets give few
+        // items and each item's tiles run one after another.
+        Plan rp = plan_for(c, std::max(1, c->nt / 16), c->n, ABYSS_PREC_REFINED, 0, false);
+        const int tiles = (c->n + abyss::kTile - 1) / abyss::kTile;
+        rp.splits = std::min(tiles, kRecheckLoopSplits);
+        rp.jspan = (tiles + rp.splits - 1) / rp.splits * abyss::kTile;
+        rp.splits = (c->n + rp.jspan - 1) / rp.jspan;
         const int tpb = 4 * (64 / rp.jl);
         const int cu = c->cu_count > 0 ? c->cu_count : 256;
         const long long items = (long long)((c->nt + tpb - 1) / tpb) * rp.splits;
