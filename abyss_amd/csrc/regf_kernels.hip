@@ -715,11 +715,12 @@ __device__ __forceinline__ void xcd_place(int id, int total, int gx, int &bx, in
 // 98 registers and drop to 4.
 // reset (checked precision, may be null): the flagged-target counter, zeroed
 // here so that no separate launch is needed before the flagging after it.
-// A one-shot tile's 52 KB of LDS holds 3 workgroups per CU, so it asks for 3.
+// A one-shot tile's 52 KB of LDS (56 KB with FP64 masses) holds 3 (2)
+// workgroups per CU, so it asks for 3 (2).
 // Grid: gx * splits workgroups on x (xcd_place).
 template <int JL, int TM, bool EST, int TILE = kTile, int WPB = 4>
 __global__ __launch_bounds__(64 * WPB)
-__attribute__((amdgpu_waves_per_eu(TILE > 64 * WPB ? 3 : 5, 8)))
+__attribute__((amdgpu_waves_per_eu(TILE > 64 * WPB ? (TM == 0 ? 3 : 2) : 5, 8)))
 void regf_kernel_jl(JState js, const int *__restrict__ tgt, int nt, int jspan, QueryOut out,
                     int cap, int final_counts, int *__restrict__ reset, int gx, int splits)
 {
@@ -1232,6 +1233,17 @@ hipError_t launch_jl(const JState &js, const int *tgt, int nt, const Plan &p, co
     if ((long long)p.grid_x * p.splits > 0x7fffffffLL) return hipErrorInvalidValue;
     dim3 grid(p.grid_x * p.splits);
     const int fc = p.splits == 1 ? 1 : 0;
+    if (tm == 3 && p.tile == kTileOneShot && p.wpb == 4) {
+        // a checked block in FP64 terms in one pass, every split one tile
+        if constexpr (JL >= 16) {
+            if (p.jspan > kTileOneShot) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((regf_kernel_jl<JL, 3, false, kTileOneShot, 4>), grid,
+                               dim3(kBlock), 0, st, js, tgt, nt, p.jspan, out, cap, fc, reset,
+                               p.grid_x, p.splits);
+            return hipGetLastError();
+        }
+        return hipErrorInvalidValue;
+    }
     if (tm != 0 && (p.tile != kTile || p.wpb != 4)) return hipErrorInvalidValue;
     if (tm == 0 && est)
         return launch_mode0<JL, true>(js, tgt, nt, p, out, cap, fc, reset, st);

@@ -254,6 +254,18 @@ Plan plan_for(abyss_ctx_t c, int nt, int n, int prec, int cap, bool forced)
         tile = abyss::kTile;
         wpb = 4;
     }
+    // A checked block in FP64 terms in one pass (term mode 3) takes the
+    // one-shot tile too: when forced, or built in for 64 lanes per target
+    // (below 16 targets), as split precision does.
+    if (direct && !c->batch_member) {
+        const bool forced_shape = forced && (c->force_tile || c->force_wpb);
+        if (forced_shape ? c->force_tile == abyss::kTileOneShot &&
+                               (c->force_wpb == 0 || c->force_wpb == 4)
+                         : jl == 64) {
+            tile = abyss::kTileOneShot;
+            wpb = 4;
+        }
+    }
     // Built-in shape of blocks below 16 targets (64 lanes per target): the
     // one-shot tile, whose splits load their j at once (Plummer + Kroupa
     // N = 65536, m = 1..16: 12.8 -> 12.1 ... 14.3 -> 13.3 us per split

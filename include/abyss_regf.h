@@ -190,7 +190,9 @@ int abyss_ctx_set_tuning(abyss_ctx_t ctx, int i_per_lane, int j_splits, int j_un
  * workgroup (1, 2 or 4; a workgroup serves waves * 64 / lanes_per_target
  * targets), with tile <= 64 * waves; 0 = automatic for each.  Tile 1024 is
  * the one-shot tile of small blocks (4 waves; 16 or 64 lanes per target,
- * j-splits of at most 1024 j; other plans fall back to 256).  The
+ * j-splits of at most 1024 j; other plans fall back to 256), also for the
+ * one FP64-term pass of a checked block of at most 256 targets, which
+ * otherwise ignores a forced shape.  The
  * reference's compile-time BatchSize and NNB_per_block (src_mpi/def.h:77,79).
  * No shape changes a result bit. */
 int abyss_ctx_set_shape(abyss_ctx_t ctx, int tile, int waves_per_block);

@@ -94,8 +94,12 @@ def test_shapes_give_the_same_bits(plummer, prec):
         assert err["a_norm"]["max"] <= 1e-6 and err["jdot_norm"]["max"] <= 1e-6
 
 
-@pytest.mark.parametrize("prec", [regf.PREC_SPLIT])
+@pytest.mark.parametrize("prec", [regf.PREC_SPLIT, regf.PREC_CHECKED])
 def test_one_shot_tile_gives_the_same_bits(plummer, prec):
+    """The one-shot tile (each split's 1024 j loaded at once) against
+    256-record tiles: the same bits, split precision and checked blocks of
+    at most 256 targets (FP64 terms in one pass); built in below 16
+    targets; plans it cannot hold fall back to 256 j."""
     js = plummer
     rng = np.random.default_rng(23)
     cases = {"random1": rng.permutation(js.n)[:1].astype(np.int32),
