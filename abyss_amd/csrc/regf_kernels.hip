@@ -915,8 +915,9 @@ __global__ __launch_bounds__(256) void check_flag_kernel(const double *__restric
     }
 }
 
-// The recomputation with the flagged count read on the device (small
-// blocks: no host wait).  The grid covers every target; a workgroup whose
+// The recomputation with the flagged count read on the device (the
+// no-allocation query: no host wait).  The grid covers every target; a
+// workgroup whose
 // target block starts at or past the count exits before its first barrier.
 // Partials are indexed s * nt_stride + k.
 template <int JL>

@@ -298,6 +298,14 @@ Plan plan_for(abyss_ctx_t c, int nt, int n, int prec, int cap, bool forced)
         // takes 21.7 us, S = 1024 31.1 us)
         const int max_by_n = std::max(1, n / (4 * abyss::kTile));
         S = std::min(S, std::min(max_by_n, abyss::kMaxSplits));
+        // large blocks keep longer spans: where splits of 16 tiles (4096 j)
+        // still give 8 workgroups per CU, no more splits than that
+        // (Plummer + Kroupa N = 65536, m = 16384 / 65536: S = 43 -> 16 and
+        // 24 -> 16, 936 -> 910 and 3640 -> 3519 us per split launch; at
+        // N = 262144 the rule above already gives fewer)
+        const int s_long = std::max(1, n / (16 * abyss::kTile));
+        if (!tuned && !c->batch_member && (long long)p.grid_x * s_long >= 8LL * cu)
+            S = std::min(S, s_long);
     }
     if (S > 1) {
         const double per = (double)nt * ((double)cap * 4.0 + 6 * 8.0 + 4.0);
