@@ -882,6 +882,108 @@ __global__ __launch_bounds__(256) void finalize_kernel(QueryOut part, int S, int
                   mir, spread, flag);
 }
 
+// finalize_kernel for S <= G j-splits: G lanes per target, 64 / G targets
+// per wave.  Lane l of a group adds split l to zero and the butterfly runs
+// over the group's G lanes; in finalize_kernel's 64-lane butterfly the other
+// lanes hold zeros, which add exactly, so the sums (and spreads) have the
+// same bits.  Lists are merged the same way, G entries at a time.
+template <int G>
+__global__ __launch_bounds__(256) void finalize_group_kernel(QueryOut part, int S, int nt, int cap,
+                                                             double *__restrict__ acc,
+                                                             double *__restrict__ adot,
+                                                             int *__restrict__ nnb,
+                                                             int *__restrict__ list,
+                                                             FinalMirror mir,
+                                                             float *__restrict__ spread,
+                                                             CheckFlag flag)
+{
+    static_assert(G >= 8 && G <= 64 && (G & (G - 1)) == 0, "a group holds the 6 sums");
+    __shared__ int s_off[256 / G][G + 1];
+    const int sub = threadIdx.x & (G - 1);
+    const int slot = threadIdx.x / G;
+    const int q = blockIdx.x * (256 / G) + slot;
+    const bool active = q < nt;
+    const bool mine = active && sub < S;
+    const size_t o = (size_t)sub * nt + q;
+    double a[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) a[c] = 0.0;
+    if (mine) {
+        a[0] += part.acc[3 * o + 0];
+        a[1] += part.acc[3 * o + 1];
+        a[2] += part.acc[3 * o + 2];
+        a[3] += part.adot[3 * o + 0];
+        a[4] += part.adot[3 * o + 1];
+        a[5] += part.adot[3 * o + 2];
+    }
+#pragma unroll
+    for (int d = G / 2; d >= 1; d >>= 1)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a[c] += __shfl_xor(a[c], d, 64);
+    if (spread) {
+        float ea = 0.0f, ej = 0.0f;
+        if (mine) {
+            ea += part.spread[2 * o + 0];
+            ej += part.spread[2 * o + 1];
+        }
+#pragma unroll
+        for (int d = G / 2; d >= 1; d >>= 1) {
+            ea += __shfl_xor(ea, d, 64);
+            ej += __shfl_xor(ej, d, 64);
+        }
+        if (active && sub == 0) {
+            spread[2 * (size_t)q + 0] = ea;
+            spread[2 * (size_t)q + 1] = ej;
+            if (flag.counter && check_rule(ea, ej, a[0], a[1], a[2], a[3], a[4], a[5], flag.k2)) {
+                const int k = atomicAdd(flag.counter, 1);
+                flag.fq[k] = q;
+                flag.fi[k] = flag.tgt[q];
+            }
+        }
+    }
+
+    // split offsets of this target's list entries: a prefix over its group
+    const int c0 = mine ? part.cnt[o] : 0;
+    int incl = c0;
+#pragma unroll
+    for (int d = 1; d < G; d <<= 1) {
+        const int t = __shfl_up(incl, d, G);
+        if (sub >= d) incl += t;
+    }
+    const int total = __shfl(incl, G - 1, G);
+    if (sub < S) s_off[slot][sub] = incl - c0;
+    if (sub == 0) s_off[slot][S] = total;
+    __syncthreads();
+
+    if (!active) return;
+    const bool mirror = mir.acc != nullptr;
+    if (sub < 3) {
+        acc[(size_t)q * 3 + sub] = a[sub];
+        if (mirror) mir.acc[(size_t)q * 3 + sub] = a[sub];
+    } else if (sub < 6) {
+        adot[(size_t)q * 3 + sub - 3] = a[sub];
+        if (mirror) mir.adot[(size_t)q * 3 + sub - 3] = a[sub];
+    }
+    const int *off = s_off[slot];
+    const int ncopy = min(total, cap);
+    int *dst = list + (size_t)q * cap;
+    for (int e = sub; e < ncopy; e += G) {
+        // largest s with off[s] <= e (off is non-decreasing, off[0] = 0)
+        int lo = 0, hi = S - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (off[mid] <= e) lo = mid; else hi = mid - 1;
+        }
+        const int j = part.list[((size_t)lo * nt + q) * (size_t)cap + (e - off[lo])];
+        dst[e] = j;
+        if (mirror) mir.list[(size_t)q * cap + e] = j;
+    }
+    if (sub == 0) {
+        nnb[q] = total <= cap ? total : -total;
+        if (mirror) mir.nnb[q] = total <= cap ? total : -total;
+    }
+}
+
 // ---- checked precision ---------------------------------------------------
 
 // One thread per target.  A wave appends its flagged targets with one
@@ -1331,6 +1433,16 @@ hipError_t launch_finalize(const QueryOut &part, int S, int nt, int cap, double 
 {
     if (spread && !part.spread) return hipErrorInvalidValue;
     if (flag.counter && !spread) return hipErrorInvalidValue;
+    if (S <= 8) {
+        hipLaunchKernelGGL(finalize_group_kernel<8>, dim3((nt + 31) / 32), dim3(256), 0, st, part,
+                           S, nt, cap, acc, adot, nnb, list, mirror, spread, flag);
+        return hipGetLastError();
+    }
+    if (S <= 16) {
+        hipLaunchKernelGGL(finalize_group_kernel<16>, dim3((nt + 15) / 16), dim3(256), 0, st, part,
+                           S, nt, cap, acc, adot, nnb, list, mirror, spread, flag);
+        return hipGetLastError();
+    }
     dim3 grid((nt + 3) / 4);
     const size_t lds = 4 * (size_t)(S + 1) * sizeof(int);
     hipLaunchKernelGGL(finalize_kernel, grid, dim3(256), lds, st, part, S, nt, cap, acc, adot, nnb,
