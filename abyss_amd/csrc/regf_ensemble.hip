@@ -57,6 +57,9 @@ struct abyss_ens_s {
     // since the cluster's flagged count was last reset; a pass ends (and
     // its flagged targets are recomputed) once every block is done
     std::vector<int> chk_done;
+    // pairs per work item of the main pass (the server uses shorter items,
+    // so that its short batches drain sooner)
+    double item_pairs = 4194304.0;
 
     // Pinned host staging of a launch's descriptors, work items, header and
     // prefixes.  Two sets alternate: a launch fills one while the copies of
@@ -499,7 +502,7 @@ int abyss_ens_launch(abyss_ens_t e)
         const int eff = std::min(tpb, c->nt);
         const int tile = abyss::kTile;
         const int nround = ((c->n + tile - 1) / tile) * tile;
-        int jspan = (int)std::ceil(kItemPairs / eff / tile) * tile;
+        int jspan = (int)std::ceil(e->item_pairs / eff / tile) * tile;
         // 16/64-lane groups do only 16 or 4 steps per tile, too little to
         // hide the next tile's load: an item's time is ~1-2 us per tile
         // whatever its pairs.  Cap their items at kItemTilesSmall tiles so a
@@ -861,6 +864,10 @@ struct abyss_srv_s {
 
 namespace {
 
+// Pairs per work item of the server's batches (the batched ensemble uses
+// 4M): its time slices are ~25 ms, so shorter items drain them sooner.
+constexpr double kServeItemPairs = 1048576.0;
+
 // Worker: forms batches.  Every running request gets at least one target
 // block per batch; requests with the least remaining work go first, then
 // blocks are handed out until the batch's pair budget is used.  Batches are
@@ -1079,6 +1086,7 @@ int abyss_srv_create(int device, int cap, int max_clusters, unsigned long long h
         return ABYSS_ERR_OUT_OF_MEMORY;
     }
     s->ens = e;
+    e->item_pairs = kServeItemPairs;
     s->device = device;
     s->cap = cap;
     s->max_clusters = max_clusters;

@@ -422,8 +422,10 @@ def test_config5_served_mix_with_large_clusters():
     Plummer cluster of 1048576 particles; 48 regular-step requests (active
     blocks log-uniform in [1, min(N, 32768)]) submitted by four threads at
     Poisson times, under the default precision.  Every request's results
-    equal the cluster run alone bit for bit and meet the contract against
-    the oracle on sampled targets."""
+    equal the same request run alone on the server bit for bit, equal plain
+    batched-ensemble launches (other work items) to 1e-12 on 99.9 % of the
+    targets and within the contract on all (lists exactly), and meet the
+    contract against the oracle on sampled targets."""
     import time
     from abyss_amd import workloads
     sizes = list(np.exp(np.random.default_rng(5).uniform(np.log(4096), np.log(1048576),
@@ -465,6 +467,18 @@ def test_config5_served_mix_with_large_clusters():
         st = srv.stats()
         say(f"  stats {st}")
         assert st["completed"] == len(reqs)
+        # each request alone on the same server: the same bits
+        for q, (c, t) in enumerate(reqs):
+            tk, nt = srv.submit(ids[c], t)
+            alone = srv.wait(tk, nt)
+            got = results[q]
+            for a, b in zip(alone[:3], got[:3]):
+                assert np.array_equal(a, b), f"request {q} (cluster {c}) differs from alone"
+            for k in range(len(t)):
+                nk = min(abs(int(alone[2][k])), cap)
+                assert np.array_equal(alone[3][k, :nk], got[3][k, :nk]), f"request {q} list"
+    # and as plain batched-ensemble launches (longer work items, so other
+    # j-splits): the same lists, sums within 1e-12
     with regf.Ensemble(device=0, cap=cap, max_clusters=len(cl)) as ens:
         for c, (x, v, m, r2) in enumerate(cl):
             ens.upload(c, m, x, v, r2)
@@ -475,10 +489,16 @@ def test_config5_served_mix_with_large_clusters():
             ens.sync()
             alone = ens.fetch(c)
             got = results[q]
-            for a, b in zip(alone[:3], got[:3]):
-                assert np.array_equal(a, b), f"request {q} (cluster {c}) differs from alone"
+            assert np.array_equal(alone[2], got[2]), f"request {q} counts"
+            for a, b in zip(alone[:2], got[:2]):
+                # other FP64 summation orders; a target at the checked
+                # precision's flag threshold may be recomputed in one and
+                # not the other, so a few may differ within the contract
+                na = np.linalg.norm(a, axis=1)
+                rel = np.linalg.norm(b - a, axis=1) / np.where(na > 0, na, 1.0)
+                assert rel.max() <= 2e-6 and np.mean(rel <= 1e-12) >= 0.999, (q, rel.max())
             for k in range(len(t)):
                 nk = min(abs(int(alone[2][k])), cap)
                 assert np.array_equal(alone[3][k, :nk], got[3][k, :nk]), f"request {q} list"
             _sample_vs_oracle(cl[c], t, got, f"config5/req{q}", nsample=8, seed=q)
-    say("  every request equals its cluster alone and meets the contract")
+    say("  every request equals itself alone and meets the contract")
